@@ -1,0 +1,161 @@
+/*
+ * dic_hip.h -- C-ABI of the MI355X (gfx950) subset-correlation library.
+ *
+ * This is the drop-in boundary for Traditional-DIC's subset IC-GN hot path:
+ * the B-spline prefilter and sampler (src/interpolation/bspline.cpp), the
+ * reference-gradient cache (src/subset/seed/reliability_propagation.cpp:84-112)
+ * and the per-POI IC-GN solver family (src/subset/solver/icgn.cpp).  The C++
+ * host layer (dic_hip.hpp) mirrors the reference's SubsetSolver /
+ * BSplineImagePreprocessor / BSplineInterpolator surface on top of it.
+ *
+ * Conventions
+ *   - Every entry point is extern "C", never throws, and returns an int status
+ *     (DIC_OK = 0).  The message of the last failure on a context is returned
+ *     by dic_last_error().  Numerical outcomes per POI are NOT errors: they are
+ *     reported in dic_result.status exactly as the reference reports
+ *     SolverStatus in Displacement2D.
+ *   - Images are row-major float32, index y*width + x (dic::Image::data(),
+ *     include/dic/core/image.hpp:58-81).
+ *   - Handles own device memory; a context's calls run in order on its stream.
+ *     One context per host thread.
+ */
+#ifndef DIC_HIP_H
+#define DIC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes of the C-ABI calls ---- */
+enum {
+    DIC_OK = 0,
+    DIC_ERR_INVALID_ARGUMENT = 1,
+    DIC_ERR_HIP = 2,
+    DIC_ERR_UNSUPPORTED = 3,
+    DIC_ERR_BUDGET = 4,
+    DIC_ERR_NO_DEVICE = 5
+};
+
+/* ---- per-POI solver status: SolverStatus order, include/dic/core/types.hpp:35-40 ---- */
+enum { DIC_SUCCESS = 0, DIC_NOT_CONVERGED = 1, DIC_INVALID_INPUT = 2, DIC_NUMERICAL_FAILURE = 3 };
+
+/* ---- enumerations mirroring the reference config ---- */
+enum { DIC_PRECISION_FP64 = 0, DIC_PRECISION_MIXED = 1 };    /* plain FP64 / FP32-compensated + FP64 solve */
+enum { DIC_OBJECTIVE_ZNSSD = 0, DIC_OBJECTIVE_SSD = 1 };     /* CorrelationCriterionKind */
+enum { DIC_OPTIMIZER_ICGN = 0, DIC_OPTIMIZER_FGN = 1 };      /* SubsetOptimizationMethod */
+enum { DIC_GRADIENT_CENTRAL = 0, DIC_GRADIENT_BSPLINE_HALF = 1 };
+
+typedef struct dic_ctx dic_ctx;
+typedef struct dic_image dic_image;   /* device float32 W x H image */
+typedef struct dic_grad dic_grad;     /* device reference-gradient planes gx, gy (H x W) */
+typedef struct dic_coeffs dic_coeffs; /* device B-spline coefficients, padded (H+2b) x (W+2b) */
+
+/* SubsetConfig fields used by the solver (include/dic/subset/subset_config.hpp:33-50). */
+typedef struct dic_icgn_params {
+    int32_t subset_radius;        /* subset_radius */
+    int32_t max_iterations;       /* max_iterations */
+    double convergence_threshold; /* convergence_threshold on ||dp||_2 */
+    int32_t shape_order;          /* 1 = FirstOrder, 2 = SecondOrder (or use_second_order) */
+    int32_t objective;            /* DIC_OBJECTIVE_* */
+    int32_t optimizer;            /* DIC_OPTIMIZER_* (FGN: DIC_ERR_UNSUPPORTED) */
+    int32_t precision;            /* DIC_PRECISION_*; must match the coefficient/gradient handles */
+    int32_t masked;               /* 1: solve_with_mask semantics (roi required) */
+    int32_t min_valid_samples;    /* masked only */
+    double min_valid_sample_ratio;/* masked only */
+} dic_icgn_params;
+
+/* InitialDisplacement (include/dic/initialization/initializer.hpp:33-44). */
+typedef struct dic_initial {
+    double u, v, du_dx, du_dy, dv_dx, dv_dy, confidence;
+} dic_initial;
+
+/* Displacement2D (include/dic/core/result.hpp:32-51) + executed iterations. */
+typedef struct dic_result {
+    double x, y;
+    double p[12]; /* u v du_dx du_dy dv_dx dv_dy d2u_dx2 d2u_dxdy d2u_dy2 d2v_dx2 d2v_dxdy d2v_dy2 */
+    double correlation;
+    int32_t status;
+    int32_t valid;
+    int32_t iterations;
+    int32_t reserved;
+} dic_result;
+
+/* ---- context ---- */
+/* New boundary (the reference has no device): owns the HIP stream and tracks
+ * the HBM bytes its handles hold against hbm_budget (0 = unlimited). */
+int dic_ctx_create(int device, uint64_t hbm_budget_bytes, dic_ctx **out);
+void dic_ctx_destroy(dic_ctx *ctx);
+const char *dic_last_error(const dic_ctx *ctx);
+/* Run subsequent work on a caller-owned hipStream_t (NULL restores the context's own). */
+int dic_ctx_set_stream(dic_ctx *ctx, void *hip_stream);
+void *dic_ctx_stream(dic_ctx *ctx);
+int dic_ctx_synchronize(dic_ctx *ctx);
+uint64_t dic_ctx_bytes_in_use(const dic_ctx *ctx);
+int dic_device_count(int *count);
+const char *dic_version(void);
+
+/* ---- plain device buffers and events (new boundary; callers without a
+ * framework of their own use these to keep inputs resident in HBM and to
+ * time work on the context's stream) ---- */
+int dic_buffer_alloc(dic_ctx *ctx, size_t bytes, void **device_ptr);
+void dic_buffer_free(dic_ctx *ctx, void *device_ptr);
+int dic_buffer_upload(dic_ctx *ctx, void *device_dst, const void *host_src, size_t bytes);
+int dic_buffer_download(dic_ctx *ctx, void *host_dst, const void *device_src, size_t bytes);
+/* Records a new event on the context's stream; free with dic_event_destroy. */
+int dic_event_record(dic_ctx *ctx, void **event);
+int dic_event_elapsed_ms(void *start, void *end, float *ms);
+void dic_event_destroy(void *event);
+
+/* ---- images: replaces dic::Image(width, height, data) (src/core/image.cpp:128-139) ---- */
+int dic_image_upload(dic_ctx *ctx, const float *host, int width, int height, dic_image **out);
+/* Borrow device memory the caller keeps alive (no copy). */
+int dic_image_wrap_device(dic_ctx *ctx, const float *device_ptr, int width, int height, dic_image **out);
+void dic_image_free(dic_image *img);
+
+/* ---- reference gradients ----
+ * mode DIC_GRADIENT_CENTRAL replaces build_reference_gradient_cache
+ *   (src/subset/seed/reliability_propagation.cpp:84-112; the pipeline convention);
+ * mode DIC_GRADIENT_BSPLINE_HALF replaces the gradient_x/y planes of
+ *   BSplineImagePreprocessor::compute (src/interpolation/bspline.cpp:282-294;
+ *   used by standalone ICGNSolver::solve, icgn.cpp:441-443). */
+int dic_ref_gradient(dic_ctx *ctx, const dic_image *ref, int mode, int degree, int border, int precision,
+                     dic_grad **out);
+int dic_grad_download(dic_ctx *ctx, const dic_grad *g, double *gx_host, double *gy_host);
+void dic_grad_free(dic_grad *g);
+
+/* ---- B-spline prefilter: replaces BSplineImagePreprocessor::compute / compute_lazy
+ * coefficient formation (src/interpolation/bspline.cpp:270-318, :369-392).
+ * degree in {1,3,5}; border is raised to 3 (:261-268); exact = use_exact_prefilter.
+ * precision selects FP64 or FP32 storage (arithmetic is FP64 either way). ---- */
+int dic_bspline_prefilter(dic_ctx *ctx, const dic_image *img, int degree, int border, int exact, int precision,
+                          dic_coeffs **out);
+int dic_coeffs_info(const dic_coeffs *c, int *width, int *height, int *degree, int *border, int *precision);
+/* Copies the padded coefficient image to host as FP64, row-major (H+2b) x (W+2b). */
+int dic_coeffs_download(dic_ctx *ctx, const dic_coeffs *c, double *host);
+void dic_coeffs_free(dic_coeffs *c);
+
+/* ---- sampling hook: replaces BSplineInterpolator::value / gradient
+ * (src/interpolation/bspline.cpp:458-500).  xy: 2n host doubles; val: n; grad: 2n
+ * (either output may be NULL). ---- */
+int dic_bspline_sample(dic_ctx *ctx, const dic_coeffs *c, const double *xy, int n, double *val, double *grad);
+
+/* ---- IC-GN: replaces ICGNSolver::solve_with_interpolators / solve_with_mask
+ * (src/subset/solver/icgn.cpp:384-432 -> :446-1912), one POI per entry.
+ * Host-pointer form: copies in, runs, copies out, synchronises. ---- */
+int dic_icgn_solve_batch(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
+                         const dic_coeffs *def, const uint8_t *roi_host, const double *poi_xy,
+                         const dic_initial *init, int n, dic_result *out);
+/* Device-pointer form: all arrays already in HBM; enqueued on `stream`
+ * (NULL = the context's stream), no synchronisation. */
+int dic_icgn_solve_batch_device(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref,
+                                const dic_grad *grad, const dic_coeffs *def, const uint8_t *roi_device,
+                                const double *poi_xy_device, const dic_initial *init_device, int n,
+                                dic_result *out_device, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

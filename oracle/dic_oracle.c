@@ -1,0 +1,900 @@
+/*
+ * dic_oracle.c -- CPU FP64 restatement of Traditional-DIC's subset IC-GN path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline); the HIP
+ * product library never links or calls it.
+ *
+ * Reference files restated (paths inside the Traditional-DIC tree):
+ *   src/interpolation/bspline.cpp            B-spline basis, qk, prefilter, blocks, sampling
+ *   src/subset/solver/icgn.cpp               IC-GN 1st/2nd order x ZNSSD/SSD x plain/masked
+ *   src/subset/seed/reliability_propagation.cpp  gradient cache, POI grid
+ *   Eigen 3.4.0 (third-party, not vendored): LDLT (Cholesky/LDLT.h) and
+ *   FullPivLU (LU/FullPivLU.h) algorithms, restated from their published source.
+ *
+ * One deliberate algorithmic substitution: the reference divides FFT spectra
+ * (bspline.cpp:191-214, Eigen/kissfft) to deconvolve each padded periodic
+ * line.  Here the same circulant system is solved by the exact periodic
+ * recursive inverse (causal/anticausal first-order factors with closed-form
+ * periodic initial values; Unser 1999).  Both are the exact inverse of the
+ * same circulant matrix; the test suite pins this function against a numpy
+ * FFT deconvolution and against a dense circulant solve.
+ */
+#include "dic_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* B-spline basis: bspline.cpp:51-71, :342-367                               */
+/* ------------------------------------------------------------------------ */
+
+static int factorial_i(int n)
+{
+    int v = 1;
+    for (int i = 2; i <= n; ++i) v *= i;
+    return v;
+}
+
+static int combination_i(int n, int k)
+{
+    if (k < 0 || k > n) return 0;
+    return factorial_i(n) / (factorial_i(k) * factorial_i(n - k));
+}
+
+/* plus_power (bspline.cpp:68-71): strict x > 0, so the d-th derivative at an
+ * integer knot takes the left limit. */
+static double plus_power(double x, int power)
+{
+    return x > 0.0 ? pow(x, (double)power) : 0.0;
+}
+
+double orc_basis(double x, int m, int d)
+{
+    if (m < 0 || m > d) return 0.0;
+    const int dfac = factorial_i(d) / factorial_i(d - m);
+    double sum = 0.0;
+    for (int k = 0; k <= d + 1; ++k) {
+        const double sign = (k % 2 == 0) ? 1.0 : -1.0;
+        const double coef = sign * (double)combination_i(d + 1, k);
+        const double shift = ((double)(d + 1) / 2.0) - (double)k;
+        sum += coef * (double)dfac * plus_power(x + shift, d - m);
+    }
+    return sum / (double)factorial_i(d);
+}
+
+/* build_qk (bspline.cpp:320-340): qk(m, c) = (-1)^m beta^(m)(c - d/2) / m! */
+void orc_build_qk(int d, double *qk)
+{
+    const int n = d + 1, off = d / 2;
+    for (int m = 0; m < n; ++m)
+        for (int c = 0; c < n; ++c) {
+            const double sign = (m % 2 == 0) ? 1.0 : -1.0;
+            qk[m * n + c] = sign * orc_basis((double)(-off + c), m, d) / (double)factorial_i(m);
+        }
+}
+
+/* symmetric_index (bspline.cpp:78-89): half-sample symmetric, -1 -> 0 */
+int orc_symmetric_index(int value, int size)
+{
+    if (size <= 0) return 0;
+    const int period = 2 * size;
+    int w = value % period;
+    if (w < 0) w += period;
+    return w < size ? w : period - 1 - w;
+}
+
+static int clamp_i(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* ------------------------------------------------------------------------ */
+/* Prefilter: circular deconvolution of a padded line (bspline.cpp:177-244)  */
+/* ------------------------------------------------------------------------ */
+
+/* Poles of the sampled B-spline kernel 1/sum_k beta_d(k) z^k (|z| < 1).
+ * cubic: z = sqrt(3) - 2; quintic: z + 1/z = -13 +- sqrt(105). */
+static int bspline_poles(int d, double *z)
+{
+    if (d == 3) {
+        z[0] = sqrt(3.0) - 2.0;
+        return 1;
+    }
+    if (d == 5) {
+        const double w1 = -13.0 + sqrt(105.0), w2 = -13.0 - sqrt(105.0);
+        z[0] = 0.5 * (w1 + sqrt(w1 * w1 - 4.0));
+        z[1] = 0.5 * (w2 + sqrt(w2 * w2 - 4.0));
+        return 2;
+    }
+    return 0; /* degree 1: kernel is [1], identity */
+}
+
+void orc_deconvolve_periodic(double *s, int L, int d)
+{
+    double z[2];
+    const int np = bspline_poles(d, z);
+    if (np == 0 || L <= 0) return;
+    double gain = 1.0;
+    for (int i = 0; i < np; ++i) gain *= (1.0 - z[i]) * (1.0 - z[i]);
+    for (int i = 0; i < np; ++i) {
+        const double zi = z[i];
+        const double zl = pow(zi, (double)L);
+        /* causal: y[k] = s[k] + z y[k-1] (indices mod L) */
+        double acc = 0.0, zm = 1.0;
+        for (int m = 0; m < L; ++m) {
+            acc += zm * s[(L - m) % L];
+            zm *= zi;
+        }
+        s[0] = acc / (1.0 - zl);
+        for (int k = 1; k < L; ++k) s[k] += zi * s[k - 1];
+        /* anticausal: w[k] = y[k] + z w[k+1] (indices mod L) */
+        acc = 0.0;
+        zm = 1.0;
+        for (int m = 0; m < L; ++m) {
+            acc += zm * s[(L - 1 + m) % L];
+            zm *= zi;
+        }
+        s[L - 1] = acc / (1.0 - zl);
+        for (int k = L - 2; k >= 0; --k) s[k] += zi * s[k + 1];
+    }
+    for (int k = 0; k < L; ++k) s[k] *= gain;
+}
+
+/* form_coefficients (bspline.cpp:369-392) with the border clamp of the
+ * BSplineImagePreprocessor constructor (:261-268). */
+int orc_form_coefficients(const float *img, int W, int H, int d, int border, int exact, double *out)
+{
+    const int b = border < 3 ? 3 : border;
+    const int PW = W + 2 * b, PH = H + 2 * b;
+    for (int y = 0; y < PH; ++y) {
+        const int sy = exact ? orc_symmetric_index(y - b, H) : clamp_i(y - b, 0, H - 1);
+        for (int x = 0; x < PW; ++x) {
+            const int sx = exact ? orc_symmetric_index(x - b, W) : clamp_i(x - b, 0, W - 1);
+            out[(size_t)y * PW + x] = (double)img[(size_t)sy * W + sx];
+        }
+    }
+    if (exact) {
+        /* deconvolve_rows (:216-229) then deconvolve_columns (:231-244) */
+        for (int y = 0; y < PH; ++y) orc_deconvolve_periodic(out + (size_t)y * PW, PW, d);
+        double *col = (double *)malloc(sizeof(double) * (size_t)PH);
+        for (int x = 0; x < PW; ++x) {
+            for (int y = 0; y < PH; ++y) col[y] = out[(size_t)y * PW + x];
+            orc_deconvolve_periodic(col, PH, d);
+            for (int y = 0; y < PH; ++y) out[(size_t)y * PW + x] = col[y];
+        }
+        free(col);
+    }
+    return b;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Local polynomial blocks and sampling (bspline.cpp:96-175, :270-318, :394-500) */
+/* ------------------------------------------------------------------------ */
+
+/* extract_coefficient_block (:394-416) + build_local_polynomial_block (:418-424):
+ * block = qk * C * qk^T */
+static void build_block(const orc_precomputed *pre, int x, int y, double *blk)
+{
+    const int d = pre->degree, n = pre->n, off = d / 2, b = pre->border;
+    const int PW = pre->width + 2 * b, PH = pre->height + 2 * b;
+    const int top = y + b - off, left = x + b - off;
+    double c[36], t[36];
+    for (int r = 0; r < n; ++r) {
+        const int sy = clamp_i(top + r, 0, PH - 1);
+        for (int cc = 0; cc < n; ++cc) {
+            const int sx = clamp_i(left + cc, 0, PW - 1);
+            c[r * n + cc] = pre->coeffs[(size_t)sy * PW + sx];
+        }
+    }
+    /* t = qk * C */
+    for (int r = 0; r < n; ++r)
+        for (int k = 0; k < n; ++k) {
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s += pre->qk[r * n + j] * c[j * n + k];
+            t[r * n + k] = s;
+        }
+    /* blk = t * qk^T */
+    for (int r = 0; r < n; ++r)
+        for (int s2 = 0; s2 < n; ++s2) {
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += t[r * n + k] * pre->qk[s2 * n + k];
+            blk[r * n + s2] = s;
+        }
+}
+
+/* evaluate_polynomial (:96-113) */
+static double eval_poly(const double *blk, int n, double dx, double dy)
+{
+    double v = 0.0, yp = 1.0;
+    for (int r = 0; r < n; ++r) {
+        double xp = 1.0;
+        for (int c = 0; c < n; ++c) {
+            v += blk[r * n + c] * yp * xp;
+            xp *= dx;
+        }
+        yp *= dy;
+    }
+    return v;
+}
+
+/* evaluate_polynomial_dx (:115-132) */
+static double eval_poly_dx(const double *blk, int n, double dx, double dy)
+{
+    double v = 0.0, yp = 1.0;
+    for (int r = 0; r < n; ++r) {
+        double xp = 1.0;
+        for (int c = 1; c < n; ++c) {
+            v += (double)c * blk[r * n + c] * yp * xp;
+            xp *= dx;
+        }
+        yp *= dy;
+    }
+    return v;
+}
+
+/* evaluate_polynomial_dy (:134-151) */
+static double eval_poly_dy(const double *blk, int n, double dx, double dy)
+{
+    double v = 0.0, yp = 1.0;
+    for (int r = 1; r < n; ++r) {
+        double xp = 1.0;
+        for (int c = 0; c < n; ++c) {
+            v += (double)r * blk[r * n + c] * yp * xp;
+            xp *= dx;
+        }
+        yp *= dy;
+    }
+    return v;
+}
+
+int orc_precompute(const float *img, int W, int H, int d, int border, int exact, int with_blocks,
+                   orc_precomputed *pre)
+{
+    memset(pre, 0, sizeof(*pre));
+    if (W <= 0 || H <= 0 || img == NULL) return -1;
+    if (d != 1 && d != 3 && d != 5) return -2; /* validate_degree (:42-49) */
+    const int b = border < 3 ? 3 : border;
+    pre->width = W;
+    pre->height = H;
+    pre->degree = d;
+    pre->border = b;
+    pre->n = d + 1;
+    orc_build_qk(d, pre->qk);
+    pre->coeffs = (double *)malloc(sizeof(double) * (size_t)(W + 2 * b) * (size_t)(H + 2 * b));
+    pre->grad_x = (double *)calloc((size_t)W * H, sizeof(double));
+    pre->grad_y = (double *)calloc((size_t)W * H, sizeof(double));
+    if (!pre->coeffs || !pre->grad_x || !pre->grad_y) return -3;
+    orc_form_coefficients(img, W, H, d, border, exact, pre->coeffs);
+    if (with_blocks) {
+        const int n = pre->n;
+        pre->blocks = (double *)malloc(sizeof(double) * (size_t)W * H * n * n);
+        if (!pre->blocks) return -3;
+        pre->has_blocks = 1;
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                double *blk = pre->blocks + ((size_t)y * W + x) * n * n;
+                build_block(pre, x, y, blk);
+                /* gradients at the pixel centre dx = dy = 0.5 (:289-294) */
+                if (n > 1) {
+                    pre->grad_x[(size_t)y * W + x] = eval_poly_dx(blk, n, 0.5, 0.5);
+                    pre->grad_y[(size_t)y * W + x] = eval_poly_dy(blk, n, 0.5, 0.5);
+                }
+            }
+    }
+    return 0;
+}
+
+void orc_precomputed_free(orc_precomputed *pre)
+{
+    free(pre->coeffs);
+    free(pre->blocks);
+    free(pre->grad_x);
+    free(pre->grad_y);
+    memset(pre, 0, sizeof(*pre));
+}
+
+/* value (:458-474): clamp floor to the image, then the local polynomial */
+double orc_value(const orc_precomputed *pre, double x, double y)
+{
+    const int ix = clamp_i((int)floor(x), 0, pre->width - 1);
+    const int iy = clamp_i((int)floor(y), 0, pre->height - 1);
+    const double dx = x - (double)ix, dy = y - (double)iy;
+    const int n = pre->n;
+    if (pre->has_blocks) return eval_poly(pre->blocks + ((size_t)iy * pre->width + ix) * n * n, n, dx, dy);
+    double blk[36];
+    build_block(pre, ix, iy, blk); /* build_local_block_on_demand (:153-175) */
+    return eval_poly(blk, n, dx, dy);
+}
+
+/* gradient (:476-500) */
+void orc_gradient(const orc_precomputed *pre, double x, double y, double *g)
+{
+    const int ix = clamp_i((int)floor(x), 0, pre->width - 1);
+    const int iy = clamp_i((int)floor(y), 0, pre->height - 1);
+    const double dx = x - (double)ix, dy = y - (double)iy;
+    const int n = pre->n;
+    double tmp[36];
+    const double *blk;
+    if (pre->has_blocks) {
+        blk = pre->blocks + ((size_t)iy * pre->width + ix) * n * n;
+    } else {
+        build_block(pre, ix, iy, tmp);
+        blk = tmp;
+    }
+    g[0] = eval_poly_dx(blk, n, dx, dy);
+    g[1] = eval_poly_dy(blk, n, dx, dy);
+}
+
+void orc_value_batch(const orc_precomputed *pre, const double *xy, int n, double *val, double *grad)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        if (val) val[i] = orc_value(pre, xy[2 * i], xy[2 * i + 1]);
+        if (grad) orc_gradient(pre, xy[2 * i], xy[2 * i + 1], grad + 2 * i);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reference gradient cache: reliability_propagation.cpp:84-112              */
+/* ------------------------------------------------------------------------ */
+void orc_reference_gradient_cache(const float *img, int W, int H, double *gx, double *gy)
+{
+    for (int y = 0; y < H; ++y) {
+        const int ym = y - 1 < 0 ? 0 : y - 1;
+        const int yp = y + 1 > H - 1 ? H - 1 : y + 1;
+        const double yden = (double)((yp - ym) > 1 ? (yp - ym) : 1);
+        for (int x = 0; x < W; ++x) {
+            const int xm = x - 1 < 0 ? 0 : x - 1;
+            const int xp = x + 1 > W - 1 ? W - 1 : x + 1;
+            const double xden = (double)((xp - xm) > 1 ? (xp - xm) : 1);
+            gx[(size_t)y * W + x] = ((double)img[(size_t)y * W + xp] - (double)img[(size_t)y * W + xm]) / xden;
+            gy[(size_t)y * W + x] = ((double)img[(size_t)yp * W + x] - (double)img[(size_t)ym * W + x]) / yden;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Eigen 3.4.0 LDLT (Cholesky/LDLT.h, ldlt_inplace<Lower>::unblocked and    */
+/* _solve_impl_transposed), restated.  Left-looking, diagonal pivoting on   */
+/* the not-yet-updated diagonal, lower triangle storage (row-major here).   */
+/* ------------------------------------------------------------------------ */
+enum { SIGN_POS = 0, SIGN_NEG = 1, SIGN_ZERO = 2, SIGN_INDEF = 3 };
+
+int orc_ldlt_factor(double *a, int n, int *tr, int *is_positive)
+{
+    int sign = SIGN_ZERO, found_zero_pivot = 0, ret = 1;
+    double temp[12];
+#define A(i, j) a[(i) * n + (j)]
+    for (int k = 0; k < n; ++k) {
+        int big = k;
+        double bigv = fabs(A(k, k));
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(A(i, i)) > bigv) {
+                bigv = fabs(A(i, i));
+                big = i;
+            }
+        tr[k] = big;
+        if (k != big) {
+            for (int j = 0; j < k; ++j) {
+                double t = A(k, j);
+                A(k, j) = A(big, j);
+                A(big, j) = t;
+            }
+            for (int i = big + 1; i < n; ++i) {
+                double t = A(i, k);
+                A(i, k) = A(i, big);
+                A(i, big) = t;
+            }
+            double t = A(k, k);
+            A(k, k) = A(big, big);
+            A(big, big) = t;
+            for (int i = k + 1; i < big; ++i) {
+                double t2 = A(i, k);
+                A(i, k) = A(big, i);
+                A(big, i) = t2;
+            }
+        }
+        const int rs = n - k - 1;
+        if (k > 0) {
+            for (int j = 0; j < k; ++j) temp[j] = A(j, j) * A(k, j);
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s += A(k, j) * temp[j];
+            A(k, k) -= s;
+            for (int i = k + 1; i < n; ++i) {
+                double s2 = 0.0;
+                for (int j = 0; j < k; ++j) s2 += A(i, j) * temp[j];
+                A(i, k) -= s2;
+            }
+        }
+        const double akk = A(k, k);
+        const int pivot_valid = fabs(akk) > 0.0;
+        if (k == 0 && !pivot_valid) {
+            sign = SIGN_ZERO;
+            for (int j = 0; j < n; ++j) tr[j] = j;
+            ret = 0;
+            break;
+        }
+        if (rs > 0 && pivot_valid) {
+            for (int i = k + 1; i < n; ++i) A(i, k) /= akk;
+        } else if (rs > 0) {
+            for (int i = k + 1; i < n; ++i) ret = ret && (A(i, k) == 0.0);
+        }
+        if (found_zero_pivot && pivot_valid) ret = 0;
+        else if (!pivot_valid) found_zero_pivot = 1;
+        if (sign == SIGN_POS) {
+            if (akk < 0.0) sign = SIGN_INDEF;
+        } else if (sign == SIGN_NEG) {
+            if (akk > 0.0) sign = SIGN_INDEF;
+        } else if (sign == SIGN_ZERO) {
+            if (akk > 0.0) sign = SIGN_POS;
+            else if (akk < 0.0) sign = SIGN_NEG;
+        }
+    }
+#undef A
+    *is_positive = (sign == SIGN_POS || sign == SIGN_ZERO);
+    return ret ? 0 : 1;
+}
+
+void orc_ldlt_solve(const double *a, int n, const int *tr, const double *rhs, double *x)
+{
+    for (int i = 0; i < n; ++i) x[i] = rhs[i];
+    for (int k = 0; k < n; ++k)
+        if (tr[k] != k) {
+            double t = x[k];
+            x[k] = x[tr[k]];
+            x[tr[k]] = t;
+        }
+    for (int i = 0; i < n; ++i) {
+        double s = x[i];
+        for (int j = 0; j < i; ++j) s -= a[i * n + j] * x[j];
+        x[i] = s;
+    }
+    for (int i = 0; i < n; ++i) {
+        const double dii = a[i * n + i];
+        if (fabs(dii) > DBL_MIN) x[i] /= dii;
+        else x[i] = 0.0;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = x[i];
+        for (int j = i + 1; j < n; ++j) s -= a[j * n + i] * x[j];
+        x[i] = s;
+    }
+    for (int k = n - 1; k >= 0; --k)
+        if (tr[k] != k) {
+            double t = x[k];
+            x[k] = x[tr[k]];
+            x[tr[k]] = t;
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Eigen 3.4.0 FullPivLU (LU/FullPivLU.h computeInPlace, rank, _solve_impl)  */
+/* restated for 6x6: complete pivoting, first maximum in column-major scan,  */
+/* rank threshold = |maxpivot| * 6 * DBL_EPSILON.                            */
+/* ------------------------------------------------------------------------ */
+int orc_fullpivlu_inverse6(const double *m, double *inv)
+{
+    const int n = 6;
+    double lu[36];
+    int rt[6], ct[6];
+    memcpy(lu, m, sizeof(lu));
+    int nonzero = n;
+    double maxpivot = 0.0;
+    for (int k = 0; k < n; ++k) {
+        int br = k, bc = k;
+        double best = -1.0;
+        for (int c = k; c < n; ++c)
+            for (int r = k; r < n; ++r) {
+                const double s = fabs(lu[r * n + c]);
+                if (s > best) {
+                    best = s;
+                    br = r;
+                    bc = c;
+                }
+            }
+        if (best == 0.0) {
+            nonzero = k;
+            for (int i = k; i < n; ++i) rt[i] = ct[i] = i;
+            break;
+        }
+        if (best > maxpivot) maxpivot = best;
+        rt[k] = br;
+        ct[k] = bc;
+        if (k != br)
+            for (int c = 0; c < n; ++c) {
+                double t = lu[k * n + c];
+                lu[k * n + c] = lu[br * n + c];
+                lu[br * n + c] = t;
+            }
+        if (k != bc)
+            for (int r = 0; r < n; ++r) {
+                double t = lu[r * n + k];
+                lu[r * n + k] = lu[r * n + bc];
+                lu[r * n + bc] = t;
+            }
+        const double piv = lu[k * n + k];
+        for (int r = k + 1; r < n; ++r) lu[r * n + k] /= piv;
+        for (int r = k + 1; r < n; ++r)
+            for (int c = k + 1; c < n; ++c) lu[r * n + c] -= lu[r * n + k] * lu[k * n + c];
+    }
+    const double thr = fabs(maxpivot) * ((double)n * DBL_EPSILON);
+    int rank = 0;
+    for (int i = 0; i < nonzero; ++i) rank += fabs(lu[i * n + i]) > thr;
+    if (rank != n) return 0;
+    for (int col = 0; col < n; ++col) {
+        double c[6];
+        for (int i = 0; i < n; ++i) c[i] = (i == col) ? 1.0 : 0.0;
+        for (int k = 0; k < n; ++k)
+            if (rt[k] != k) {
+                double t = c[k];
+                c[k] = c[rt[k]];
+                c[rt[k]] = t;
+            }
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) c[i] -= lu[i * n + j] * c[j];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) c[i] -= lu[i * n + j] * c[j];
+            c[i] /= lu[i * n + i];
+        }
+        for (int k = n - 1; k >= 0; --k)
+            if (ct[k] != k) {
+                double t = c[k];
+                c[k] = c[ct[k]];
+                c[ct[k]] = t;
+            }
+        for (int i = 0; i < n; ++i) inv[i * n + col] = c[i];
+    }
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Warp updates: icgn.cpp:83-120 (affine), :196-357 (second order)           */
+/* ------------------------------------------------------------------------ */
+void orc_ic_affine_update(const double *p, const double *d, double *out)
+{
+    const double du = p[0], dv = p[1], dudx = p[2], dudy = p[3], dvdx = p[4], dvdy = p[5];
+    const double dp0 = d[0], dp1 = d[1], dp2 = d[2], dp3 = d[3], dp4 = d[4], dp5 = d[5];
+    const double den = dp2 + dp5 + dp2 * dp5 - dp3 * dp4 + 1.0;
+    if (fabs(den) <= 1e-12) {
+        for (int i = 0; i < 6; ++i) out[i] = NAN;
+        return;
+    }
+    out[0] = du - ((dudx + 1.0) * (dp0 + dp0 * dp5 - dp1 * dp3)) / den - (dudy * (dp1 - dp0 * dp4 + dp1 * dp2)) / den;
+    out[1] = dv - ((dvdy + 1.0) * (dp1 - dp0 * dp4 + dp1 * dp2)) / den - (dvdx * (dp0 + dp0 * dp5 - dp1 * dp3)) / den;
+    out[2] = ((dp5 + 1.0) * (dudx + 1.0)) / den - (dp4 * dudy) / den - 1.0;
+    out[3] = (dudy * (dp2 + 1.0)) / den - (dp3 * (dudx + 1.0)) / den;
+    out[4] = (dvdx * (dp5 + 1.0)) / den - (dp4 * (dvdy + 1.0)) / den;
+    out[5] = ((dp2 + 1.0) * (dvdy + 1.0)) / den - (dp3 * dvdx) / den - 1.0;
+}
+
+/* build_second_order_warp_matrix (icgn.cpp:196-261), row-major 6x6 */
+void orc_second_order_warp_matrix(const double *p, double *G)
+{
+    const double u = p[0], v = p[1], ux = p[2], uy = p[3], vx = p[4], vy = p[5];
+    const double uxx = p[6], uxy = p[7], uyy = p[8], vxx = p[9], vxy = p[10], vyy = p[11];
+    memset(G, 0, sizeof(double) * 36);
+#define G_(r, c) G[(r) * 6 + (c)]
+    G_(0, 0) = u; G_(0, 1) = 1.0 + ux; G_(0, 2) = uy; G_(0, 3) = 0.5 * uxx; G_(0, 4) = uxy; G_(0, 5) = 0.5 * uyy;
+    G_(1, 0) = v; G_(1, 1) = vx; G_(1, 2) = 1.0 + vy; G_(1, 3) = 0.5 * vxx; G_(1, 4) = vxy; G_(1, 5) = 0.5 * vyy;
+    G_(2, 0) = 1.0;
+    G_(3, 0) = u * u;
+    G_(3, 1) = 2.0 * u * (1.0 + ux);
+    G_(3, 2) = 2.0 * u * uy;
+    G_(3, 3) = 1.0 + 2.0 * ux + ux * ux + u * uxx;
+    G_(3, 4) = 2.0 * (uy + ux * uy + u * uxy);
+    G_(3, 5) = uy * uy + u * uyy;
+    G_(4, 0) = u * v;
+    G_(4, 1) = v + u * vx + v * ux;
+    G_(4, 2) = u + u * vy + v * uy;
+    G_(4, 3) = vx + ux * vx + 0.5 * (u * vxx + v * uxx);
+    G_(4, 4) = 1.0 + vy + ux + ux * vy + uy * vx + u * vxy + v * uxy;
+    G_(4, 5) = uy + uy * vy + 0.5 * (u * vyy + v * uyy);
+    G_(5, 0) = v * v;
+    G_(5, 1) = 2.0 * v * vx;
+    G_(5, 2) = 2.0 * v * (1.0 + vy);
+    G_(5, 3) = vx * vx + v * vxx;
+    G_(5, 4) = 2.0 * (vx + vx * vy + v * vxy);
+    G_(5, 5) = 1.0 + 2.0 * vy + vy * vy + v * vyy;
+#undef G_
+}
+
+/* inverse_compositional_second_order_update (icgn.cpp:289-357):
+ * G_new = G(p) * G(dp)^-1 * P^-1, FullPivLU inverse, additive fallback;
+ * extract_from_warp_matrix (:265-287). */
+void orc_ic_second_order_update(const double *p, const double *d, double *out)
+{
+    double Gp[36], Gd[36], Gi[36], T[36];
+    orc_second_order_warp_matrix(p, Gp);
+    orc_second_order_warp_matrix(d, Gd);
+    if (!orc_fullpivlu_inverse6(Gd, Gi)) {
+        for (int i = 0; i < 12; ++i) out[i] = p[i] - d[i];
+        return;
+    }
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) {
+            double s = 0.0;
+            for (int k = 0; k < 6; ++k) s += Gp[r * 6 + k] * Gi[k * 6 + c];
+            T[r * 6 + c] = s;
+        }
+    /* (T * P^-1)(r, c): column 1 <- T(r,0), column 2 <- T(r,1), column 0 <- T(r,2) */
+    double Gn[2][6];
+    for (int r = 0; r < 2; ++r) {
+        Gn[r][0] = T[r * 6 + 2];
+        Gn[r][1] = T[r * 6 + 0];
+        Gn[r][2] = T[r * 6 + 1];
+        Gn[r][3] = T[r * 6 + 3];
+        Gn[r][4] = T[r * 6 + 4];
+        Gn[r][5] = T[r * 6 + 5];
+    }
+    out[0] = Gn[0][0];
+    out[2] = Gn[0][1] - 1.0;
+    out[3] = Gn[0][2];
+    out[6] = 2.0 * Gn[0][3];
+    out[7] = Gn[0][4];
+    out[8] = 2.0 * Gn[0][5];
+    out[1] = Gn[1][0];
+    out[4] = Gn[1][1];
+    out[5] = Gn[1][2] - 1.0;
+    out[9] = 2.0 * Gn[1][3];
+    out[10] = Gn[1][4];
+    out[11] = 2.0 * Gn[1][5];
+}
+
+/* ------------------------------------------------------------------------ */
+/* IC-GN per POI: icgn.cpp:446-627 (1st ZNSSD), :629-818 (1st ZNSSD masked), */
+/* :832-988 (1st SSD), :990-1154 (1st SSD masked), :1168-1362 (2nd ZNSSD),   */
+/* :1364-1562 (2nd ZNSSD masked), :1576-1739 (2nd SSD), :1741-1912 (masked)  */
+/* All eight share one body here; they differ only in P, the steepest-descent */
+/* vector, the warp, the objective and the sample filter.                     */
+/* ------------------------------------------------------------------------ */
+static int all_finite(const double *v, int n)
+{
+    for (int i = 0; i < n; ++i)
+        if (!isfinite(v[i])) return 0;
+    return 1;
+}
+
+void orc_icgn_solve(const orc_config *cfg, const float *ref, const float *def_unused, int W, int H,
+                    const double *gxp, const double *gyp, const orc_precomputed *dpre, const uint8_t *roi,
+                    double px, double py, const orc_initial *init, orc_result *res)
+{
+    (void)def_unused;
+    const int P = cfg->second_order ? 12 : 6;
+    const int znssd = cfg->objective == 0;
+    memset(res, 0, sizeof(*res));
+    res->x = px;
+    res->y = py;
+    res->p[0] = init->u;
+    res->p[1] = init->v;
+    res->correlation = init->confidence;
+    res->status = ORC_INVALID_INPUT;
+    res->valid = 0;
+    res->iterations = 0;
+
+    if (W <= 0 || H <= 0 || dpre == NULL || dpre->width != W || dpre->height != H || cfg->subset_radius < 1 ||
+        cfg->max_iterations < 0)
+        return;
+
+    const int cx = (int)round(px), cy = (int)round(py);
+    const int r = cfg->subset_radius;
+    if (roi == NULL) {
+        if (cx - r < 0 || cy - r < 0 || cx + r >= W || cy + r >= H) return;
+    } else {
+        if (!(cx >= 0 && cy >= 0 && cx < W && cy < H && roi[(size_t)cy * W + cx])) return;
+    }
+
+    const int cap = (2 * r + 1) * (2 * r + 1);
+    double *lx = (double *)malloc(sizeof(double) * cap);
+    double *ly = (double *)malloc(sizeof(double) * cap);
+    double *fv = (double *)malloc(sizeof(double) * cap);
+    double *fn = (double *)malloc(sizeof(double) * cap);
+    double *sd = (double *)malloc(sizeof(double) * cap * P);
+    double *gv = (double *)malloc(sizeof(double) * cap);
+    int ns = 0, full = 0;
+    double fmean = 0.0;
+    for (int dy = -r; dy <= r; ++dy)
+        for (int dx = -r; dx <= r; ++dx) {
+            if (dx * dx + dy * dy > r * r) continue;
+            ++full;
+            const int x = cx + dx, y = cy + dy;
+            if (roi != NULL) {
+                if (!(x >= 0 && y >= 0 && x < W && y < H)) continue;
+                if (!roi[(size_t)y * W + x]) continue;
+            }
+            lx[ns] = (double)dx;
+            ly[ns] = (double)dy;
+            fv[ns] = (double)ref[(size_t)y * W + x];
+            const double gx = gxp[(size_t)y * W + x], gy = gyp[(size_t)y * W + x];
+            double *s = sd + (size_t)ns * P;
+            s[0] = gx;
+            s[1] = gy;
+            s[2] = gx * lx[ns];
+            s[3] = gx * ly[ns];
+            s[4] = gy * lx[ns];
+            s[5] = gy * ly[ns];
+            if (P == 12) { /* second_order_steepest_descent (:142-165) */
+                const double dx22 = lx[ns] * lx[ns] * 0.5, dy22 = ly[ns] * ly[ns] * 0.5, dxy = lx[ns] * ly[ns];
+                s[6] = gx * dx22;
+                s[7] = gx * dxy;
+                s[8] = gx * dy22;
+                s[9] = gy * dx22;
+                s[10] = gy * dxy;
+                s[11] = gy * dy22;
+            }
+            fmean += fv[ns];
+            ++ns;
+        }
+
+    int min_needed = P;
+    if (roi != NULL) {
+        int ms = (int)ceil(cfg->min_valid_sample_ratio * (double)full);
+        if (cfg->min_valid_samples > ms) ms = cfg->min_valid_samples;
+        if (ms > min_needed) min_needed = ms;
+    }
+    double fnorm = 1.0;
+    double Hm[144];
+    int tr[12], ispos = 0;
+    if (ns < min_needed) goto done;
+
+    if (znssd) {
+        fmean /= (double)ns;
+        double acc = 0.0;
+        for (int i = 0; i < ns; ++i) {
+            const double df = fv[i] - fmean;
+            acc += df * df;
+        }
+        fnorm = sqrt(acc);
+        if (fnorm <= 1e-12) {
+            res->status = ORC_NUMERICAL_FAILURE;
+            goto done;
+        }
+        for (int i = 0; i < ns; ++i) fn[i] = (fv[i] - fmean) / fnorm;
+    }
+
+    memset(Hm, 0, sizeof(Hm));
+    for (int i = 0; i < ns; ++i) {
+        const double *s = sd + (size_t)i * P;
+        for (int a = 0; a < P; ++a)
+            for (int b2 = 0; b2 < P; ++b2) Hm[a * P + b2] += s[a] * s[b2];
+    }
+    if (znssd) {
+        const double sc = 2.0 / (fnorm * fnorm);
+        for (int k = 0; k < P * P; ++k) Hm[k] *= sc;
+    }
+    if (orc_ldlt_factor(Hm, P, tr, &ispos) != 0 || !ispos) {
+        res->status = ORC_NUMERICAL_FAILURE;
+        goto done;
+    }
+
+    {
+        double p[12] = {0};
+        p[0] = init->u;
+        p[1] = init->v;
+        p[2] = init->du_dx;
+        p[3] = init->du_dy;
+        p[4] = init->dv_dx;
+        p[5] = init->dv_dy;
+        double corr = INFINITY;
+        int converged = 0, it;
+        const double Wm1 = (double)(W - 1), Hm1 = (double)(H - 1);
+        for (it = 0; it < cfg->max_iterations; ++it) {
+            res->iterations = it + 1;
+            double gmean = 0.0;
+            for (int i = 0; i < ns; ++i) {
+                double wx, wy;
+                if (P == 6) {
+                    wx = (double)cx + lx[i] + p[0] + p[2] * lx[i] + p[3] * ly[i];
+                    wy = (double)cy + ly[i] + p[1] + p[4] * lx[i] + p[5] * ly[i];
+                } else { /* warp_second_order (:167-182) */
+                    const double dx22 = lx[i] * lx[i] * 0.5, dy22 = ly[i] * ly[i] * 0.5, dxy = lx[i] * ly[i];
+                    wx = (double)cx + lx[i] + p[0] + p[2] * lx[i] + p[3] * ly[i] + p[6] * dx22 + p[7] * dxy +
+                         p[8] * dy22;
+                    wy = (double)cy + ly[i] + p[1] + p[4] * lx[i] + p[5] * ly[i] + p[9] * dx22 + p[10] * dxy +
+                         p[11] * dy22;
+                }
+                /* warped_point_in_bounds (:35-40) */
+                if (!(wx >= 0.0 && wy >= 0.0 && wx <= Wm1 && wy <= Hm1)) {
+                    res->status = ORC_INVALID_INPUT;
+                    goto done;
+                }
+                gv[i] = orc_value(dpre, wx, wy);
+                gmean += gv[i];
+            }
+            double G[12] = {0};
+            if (znssd) {
+                gmean /= (double)ns;
+                double acc = 0.0; /* vector_norm (:73-81) */
+                for (int i = 0; i < ns; ++i) {
+                    const double dg = gv[i] - gmean;
+                    acc += dg * dg;
+                }
+                const double gnorm = sqrt(acc);
+                if (gnorm <= 1e-12) {
+                    res->status = ORC_NUMERICAL_FAILURE;
+                    goto done;
+                }
+                corr = 0.0;
+                for (int i = 0; i < ns; ++i) {
+                    const double e = fn[i] - (gv[i] - gmean) / gnorm;
+                    const double *s = sd + (size_t)i * P;
+                    for (int a = 0; a < P; ++a) G[a] += e * s[a];
+                    corr += e * e;
+                }
+                const double sc = 2.0 / fnorm;
+                for (int a = 0; a < P; ++a) G[a] *= sc;
+            } else {
+                corr = 0.0;
+                for (int i = 0; i < ns; ++i) {
+                    const double e = fv[i] - gv[i];
+                    const double *s = sd + (size_t)i * P;
+                    for (int a = 0; a < P; ++a) G[a] += e * s[a];
+                    corr += e * e;
+                }
+            }
+            double dp[12];
+            orc_ldlt_solve(Hm, P, tr, G, dp);
+            for (int a = 0; a < P; ++a) dp[a] = -dp[a];
+            if (!all_finite(dp, P)) {
+                res->status = ORC_NUMERICAL_FAILURE;
+                goto done;
+            }
+            double nrm = 0.0;
+            for (int a = 0; a < P; ++a) nrm += dp[a] * dp[a];
+            nrm = sqrt(nrm);
+            double pn[12];
+            if (P == 6) orc_ic_affine_update(p, dp, pn);
+            else orc_ic_second_order_update(p, dp, pn);
+            memcpy(p, pn, sizeof(double) * P);
+            if (!all_finite(p, P)) {
+                res->status = ORC_NUMERICAL_FAILURE;
+                goto done;
+            }
+            if (nrm < cfg->convergence_threshold) {
+                converged = 1;
+                break;
+            }
+        }
+        for (int a = 0; a < P; ++a) res->p[a] = p[a];
+        res->correlation = corr;
+        if (!converged && isfinite(corr)) converged = 1; /* :620-622 */
+        res->status = converged ? ORC_SUCCESS : ORC_NOT_CONVERGED;
+        res->valid = converged;
+    }
+done:
+    free(lx);
+    free(ly);
+    free(fv);
+    free(fn);
+    free(sd);
+    free(gv);
+}
+
+void orc_icgn_solve_batch(const orc_config *cfg, const float *ref, int W, int H, const double *gx, const double *gy,
+                          const orc_precomputed *dpre, const uint8_t *roi, const double *xy, const orc_initial *init,
+                          int n, orc_result *out, int threads)
+{
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 8) num_threads(threads)
+#else
+    (void)threads;
+#endif
+    for (int i = 0; i < n; ++i)
+        orc_icgn_solve(cfg, ref, NULL, W, H, gx, gy, dpre, roi, xy[2 * i], xy[2 * i + 1], &init[i], &out[i]);
+}
+
+int orc_poi_grid(int W, int H, int spacing, double *xy)
+{
+    if (spacing < 0) spacing = 0;
+    const int step = spacing + 1;
+    const int gw = (W + step - 1) / step, gh = (H + step - 1) / step;
+    if (xy)
+        for (int i = 0; i < gw * gh; ++i) {
+            xy[2 * i] = (double)((i % gw) * step);
+            xy[2 * i + 1] = (double)((i / gw) * step);
+        }
+    return gw * gh;
+}

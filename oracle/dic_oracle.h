@@ -1,0 +1,128 @@
+/*
+ * dic_oracle.h -- CPU FP64 restatement of Traditional-DIC's subset-correlation
+ * hot path (B-spline prefilter + sampling, reference gradients, IC-GN solvers).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path and
+ * the timed CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  Nothing in the product library links it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the Traditional-DIC source tree).  Arithmetic is FP64 throughout, with the
+ * reference's serial summation order (dy-outer / dx-inner subset loops,
+ * two-pass mean/norm) so that its rounding behaviour is close to the
+ * reference's; bit identity with Eigen is not claimed.
+ */
+#ifndef DIC_ORACLE_H
+#define DIC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* SolverStatus enum order: include/dic/core/types.hpp:35-40 */
+enum { ORC_SUCCESS = 0, ORC_NOT_CONVERGED = 1, ORC_INVALID_INPUT = 2, ORC_NUMERICAL_FAILURE = 3 };
+
+/* Displacement2D (include/dic/core/result.hpp:32-51) plus the executed
+ * iteration count.  Layout is shared with the HIP library's dic_result. */
+typedef struct orc_result {
+    double x, y;
+    double p[12]; /* u v du_dx du_dy dv_dx dv_dy d2u_dx2 d2u_dxdy d2u_dy2 d2v_dx2 d2v_dxdy d2v_dy2 */
+    double correlation;
+    int32_t status;
+    int32_t valid;
+    int32_t iterations;
+    int32_t reserved;
+} orc_result;
+
+/* InitialDisplacement (include/dic/initialization/initializer.hpp:33-44):
+ * the six first-order parameters and the confidence used on failure exits. */
+typedef struct orc_initial {
+    double u, v, du_dx, du_dy, dv_dx, dv_dy, confidence;
+} orc_initial;
+
+/* SubsetConfig fields read by ICGNSolver (include/dic/subset/subset_config.hpp:33-50). */
+typedef struct orc_config {
+    int32_t subset_radius;
+    int32_t max_iterations;
+    double convergence_threshold;
+    int32_t second_order;      /* shape_function == SecondOrder || use_second_order */
+    int32_t objective;         /* 0 = ZNSSD, 1 = SSD */
+    int32_t min_valid_samples; /* masked variants only */
+    int32_t reserved;
+    double min_valid_sample_ratio;
+} orc_config;
+
+/* BSplinePrecomputedImage (include/dic/interpolation/bspline.hpp:69-81), row-major. */
+typedef struct orc_precomputed {
+    int32_t width, height, degree, border, n;
+    int32_t has_blocks;
+    double qk[36];     /* n x n, row = derivative order */
+    double *coeffs;    /* (height + 2*border) x (width + 2*border), row-major */
+    double *blocks;    /* width*height blocks of n*n (row = dy power, col = dx power), or NULL */
+    double *grad_x;    /* height x width, B-spline derivative at (+0.5,+0.5) */
+    double *grad_y;
+} orc_precomputed;
+
+/* --- B-spline (src/interpolation/bspline.cpp) --- */
+double orc_basis(double x, int derivative_order, int degree);              /* :342-367 */
+void orc_build_qk(int degree, double *qk_out);                             /* :320-340 */
+int orc_symmetric_index(int value, int size);                              /* :78-89 */
+/* Circular deconvolution of one periodic line by the sampled B-spline kernel
+ * (:177-214), restated as the exact periodic recursive inverse. */
+void orc_deconvolve_periodic(double *line, int length, int degree);
+/* form_coefficients (:369-392): pad + rows + columns.  out is
+ * (H+2b) x (W+2b) with b = max(3, border). Returns b. */
+int orc_form_coefficients(const float *image, int width, int height, int degree,
+                          int border, int exact, double *out);
+/* compute (:270-300) when with_blocks != 0, else compute_lazy (:302-318). */
+int orc_precompute(const float *image, int width, int height, int degree, int border,
+                   int exact, int with_blocks, orc_precomputed *out);
+void orc_precomputed_free(orc_precomputed *pre);
+double orc_value(const orc_precomputed *pre, double x, double y);                 /* :458-474 */
+void orc_gradient(const orc_precomputed *pre, double x, double y, double *gxgy);  /* :476-500 */
+void orc_value_batch(const orc_precomputed *pre, const double *xy, int n, double *val, double *grad);
+
+/* --- reference gradients --- */
+/* build_reference_gradient_cache: src/subset/seed/reliability_propagation.cpp:84-112 */
+void orc_reference_gradient_cache(const float *image, int width, int height, double *gx, double *gy);
+
+/* --- dense small-matrix kernels (Eigen 3.4.0 semantics) --- */
+/* LDLT with diagonal pivoting; returns info (0 ok), *is_positive */
+int orc_ldlt_factor(double *a, int n, int *transpositions, int *is_positive);
+void orc_ldlt_solve(const double *a, int n, const int *transpositions, const double *rhs, double *x);
+/* FullPivLU invertibility + inverse of a 6x6 matrix; returns 1 if invertible */
+int orc_fullpivlu_inverse6(const double *m, double *inv);
+
+/* --- warp updates (src/subset/solver/icgn.cpp) --- */
+void orc_ic_affine_update(const double *p, const double *dp, double *out);        /* :83-120 */
+void orc_second_order_warp_matrix(const double *p, double *g);                    /* :196-261 */
+void orc_ic_second_order_update(const double *p, const double *dp, double *out);  /* :289-357 */
+
+/* --- IC-GN solver (src/subset/solver/icgn.cpp:361-1912) --- */
+/* grad_x/grad_y: H x W planes used for the reference gradient
+ * (reference_gradient_at, :55-71).  roi may be NULL (unmasked variants);
+ * when non-NULL the masked variants run. */
+void orc_icgn_solve(const orc_config *cfg,
+                    const float *reference, const float *deformed_unused,
+                    int width, int height,
+                    const double *grad_x, const double *grad_y,
+                    const orc_precomputed *deformed,
+                    const uint8_t *roi,
+                    double px, double py, const orc_initial *init, orc_result *out);
+/* Batch over POIs; threads <= 0 means all OpenMP threads. */
+void orc_icgn_solve_batch(const orc_config *cfg, const float *reference,
+                          int width, int height,
+                          const double *grad_x, const double *grad_y,
+                          const orc_precomputed *deformed, const uint8_t *roi,
+                          const double *poi_xy, const orc_initial *init, int n,
+                          orc_result *out, int threads);
+
+/* POI grid of ReliabilityPropagation::propagate (reliability_propagation.cpp:154-176). */
+int orc_poi_grid(int width, int height, int spacing, double *xy_out /* may be NULL */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

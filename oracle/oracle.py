@@ -1,0 +1,232 @@
+"""ctypes loader of the CPU FP64 oracle (libdic_oracle.so, built from dic_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker and the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdic_oracle.so")
+
+RESULT_DTYPE = np.dtype([
+    ("x", "<f8"), ("y", "<f8"), ("p", "<f8", (12,)), ("correlation", "<f8"),
+    ("status", "<i4"), ("valid", "<i4"), ("iterations", "<i4"), ("reserved", "<i4"),
+])
+INITIAL_DTYPE = np.dtype([
+    ("u", "<f8"), ("v", "<f8"), ("du_dx", "<f8"), ("du_dy", "<f8"),
+    ("dv_dx", "<f8"), ("dv_dy", "<f8"), ("confidence", "<f8"),
+])
+
+
+class OrcConfig(ctypes.Structure):
+    _fields_ = [("subset_radius", ctypes.c_int32), ("max_iterations", ctypes.c_int32),
+                ("convergence_threshold", ctypes.c_double), ("second_order", ctypes.c_int32),
+                ("objective", ctypes.c_int32), ("min_valid_samples", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("min_valid_sample_ratio", ctypes.c_double)]
+
+
+class OrcPrecomputed(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("degree", ctypes.c_int32),
+                ("border", ctypes.c_int32), ("n", ctypes.c_int32), ("has_blocks", ctypes.c_int32),
+                ("qk", ctypes.c_double * 36), ("coeffs", ctypes.c_void_p), ("blocks", ctypes.c_void_p),
+                ("grad_x", ctypes.c_void_p), ("grad_y", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C oracle`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    lib.orc_basis.argtypes = [cd, ci, ci]
+    lib.orc_basis.restype = cd
+    lib.orc_build_qk.argtypes = [ci, vp]
+    lib.orc_symmetric_index.argtypes = [ci, ci]
+    lib.orc_symmetric_index.restype = ci
+    lib.orc_deconvolve_periodic.argtypes = [vp, ci, ci]
+    lib.orc_form_coefficients.argtypes = [vp, ci, ci, ci, ci, ci, vp]
+    lib.orc_form_coefficients.restype = ci
+    lib.orc_precompute.argtypes = [vp, ci, ci, ci, ci, ci, ci, ctypes.POINTER(OrcPrecomputed)]
+    lib.orc_precompute.restype = ci
+    lib.orc_precomputed_free.argtypes = [ctypes.POINTER(OrcPrecomputed)]
+    lib.orc_value.argtypes = [ctypes.POINTER(OrcPrecomputed), cd, cd]
+    lib.orc_value.restype = cd
+    lib.orc_value_batch.argtypes = [ctypes.POINTER(OrcPrecomputed), vp, ci, vp, vp]
+    lib.orc_reference_gradient_cache.argtypes = [vp, ci, ci, vp, vp]
+    lib.orc_ldlt_factor.argtypes = [vp, ci, vp, ctypes.POINTER(ci)]
+    lib.orc_ldlt_factor.restype = ci
+    lib.orc_ldlt_solve.argtypes = [vp, ci, vp, vp, vp]
+    lib.orc_fullpivlu_inverse6.argtypes = [vp, vp]
+    lib.orc_fullpivlu_inverse6.restype = ci
+    lib.orc_ic_affine_update.argtypes = [vp, vp, vp]
+    lib.orc_second_order_warp_matrix.argtypes = [vp, vp]
+    lib.orc_ic_second_order_update.argtypes = [vp, vp, vp]
+    lib.orc_icgn_solve_batch.argtypes = [ctypes.POINTER(OrcConfig), vp, ci, ci, vp, vp,
+                                         ctypes.POINTER(OrcPrecomputed), vp, vp, vp, ci, vp, ci]
+    lib.orc_poi_grid.argtypes = [ci, ci, ci, vp]
+    lib.orc_poi_grid.restype = ci
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+def basis(x: float, m: int, degree: int) -> float:
+    return load().orc_basis(float(x), int(m), int(degree))
+
+
+def build_qk(degree: int) -> np.ndarray:
+    n = degree + 1
+    out = np.zeros(36)
+    load().orc_build_qk(degree, _p(out))
+    return out[: n * n].reshape(n, n).copy()
+
+
+def symmetric_index(v: int, size: int) -> int:
+    return load().orc_symmetric_index(int(v), int(size))
+
+
+def deconvolve_periodic(line: np.ndarray, degree: int) -> np.ndarray:
+    a = np.ascontiguousarray(line, np.float64).copy()
+    load().orc_deconvolve_periodic(_p(a), a.size, int(degree))
+    return a
+
+
+def form_coefficients(img: np.ndarray, degree: int = 5, border: int = 3, exact: bool = True) -> np.ndarray:
+    img = np.ascontiguousarray(img, np.float32)
+    H, W = img.shape
+    b = max(3, border)
+    out = np.empty((H + 2 * b, W + 2 * b))
+    load().orc_form_coefficients(_p(img), W, H, degree, border, int(exact), _p(out))
+    return out
+
+
+class Precomputed:
+    """BSplineImagePreprocessor::compute (with_blocks) or compute_lazy."""
+
+    def __init__(self, img: np.ndarray, degree: int = 5, border: int = 3, exact: bool = True,
+                 with_blocks: bool = True):
+        self._img = np.ascontiguousarray(img, np.float32)
+        H, W = self._img.shape
+        self.s = OrcPrecomputed()
+        rc = load().orc_precompute(_p(self._img), W, H, degree, border, int(exact), int(with_blocks),
+                                   ctypes.byref(self.s))
+        if rc != 0:
+            raise ValueError(f"orc_precompute failed ({rc})")
+        self.width, self.height = W, H
+
+    def __del__(self):
+        try:
+            load().orc_precomputed_free(ctypes.byref(self.s))
+        except Exception:
+            pass
+
+    def coefficients(self) -> np.ndarray:
+        b = self.s.border
+        n = (self.height + 2 * b) * (self.width + 2 * b)
+        buf = (ctypes.c_double * n).from_address(self.s.coeffs)
+        return np.ctypeslib.as_array(buf).reshape(self.height + 2 * b, self.width + 2 * b).copy()
+
+    def gradients(self):
+        n = self.width * self.height
+        gx = np.ctypeslib.as_array((ctypes.c_double * n).from_address(self.s.grad_x)).reshape(self.height, self.width)
+        gy = np.ctypeslib.as_array((ctypes.c_double * n).from_address(self.s.grad_y)).reshape(self.height, self.width)
+        return gx.copy(), gy.copy()
+
+    def value(self, x: float, y: float) -> float:
+        return load().orc_value(ctypes.byref(self.s), float(x), float(y))
+
+    def sample(self, xy: np.ndarray):
+        xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+        n = xy.shape[0]
+        val = np.empty(n)
+        grad = np.empty((n, 2))
+        load().orc_value_batch(ctypes.byref(self.s), _p(xy), n, _p(val), _p(grad))
+        return val, grad
+
+
+def reference_gradient_cache(img: np.ndarray):
+    img = np.ascontiguousarray(img, np.float32)
+    H, W = img.shape
+    gx = np.empty((H, W))
+    gy = np.empty((H, W))
+    load().orc_reference_gradient_cache(_p(img), W, H, _p(gx), _p(gy))
+    return gx, gy
+
+
+def ldlt(a: np.ndarray):
+    m = np.ascontiguousarray(a, np.float64).copy()
+    n = m.shape[0]
+    tr = np.zeros(n, np.int32)
+    pos = ctypes.c_int(0)
+    info = load().orc_ldlt_factor(_p(m), n, _p(tr), ctypes.byref(pos))
+    return m, tr, info, bool(pos.value)
+
+
+def ldlt_solve(factor: np.ndarray, tr: np.ndarray, rhs: np.ndarray) -> np.ndarray:
+    n = factor.shape[0]
+    x = np.empty(n)
+    r = np.ascontiguousarray(rhs, np.float64)
+    load().orc_ldlt_solve(_p(np.ascontiguousarray(factor)), n, _p(np.ascontiguousarray(tr, np.int32)), _p(r), _p(x))
+    return x
+
+
+def fullpivlu_inverse6(m: np.ndarray):
+    a = np.ascontiguousarray(m, np.float64)
+    inv = np.zeros((6, 6))
+    ok = load().orc_fullpivlu_inverse6(_p(a), _p(inv))
+    return bool(ok), inv
+
+
+def ic_affine_update(p, dp):
+    p = np.ascontiguousarray(p, np.float64)
+    dp = np.ascontiguousarray(dp, np.float64)
+    out = np.empty(6)
+    load().orc_ic_affine_update(_p(p), _p(dp), _p(out))
+    return out
+
+
+def second_order_warp_matrix(p):
+    p = np.ascontiguousarray(p, np.float64)
+    g = np.empty((6, 6))
+    load().orc_second_order_warp_matrix(_p(p), _p(g))
+    return g
+
+
+def ic_second_order_update(p, dp):
+    p = np.ascontiguousarray(p, np.float64)
+    dp = np.ascontiguousarray(dp, np.float64)
+    out = np.empty(12)
+    load().orc_ic_second_order_update(_p(p), _p(dp), _p(out))
+    return out
+
+
+def icgn_solve(reference: np.ndarray, pre: Precomputed, grad_x: np.ndarray, grad_y: np.ndarray,
+               poi_xy: np.ndarray, init: np.ndarray, radius: int = 15, max_iterations: int = 30,
+               threshold: float = 1e-3, shape_order: int = 1, objective: int = 0, roi: np.ndarray | None = None,
+               min_valid_samples: int = 12, min_valid_sample_ratio: float = 0.5, threads: int = 0) -> np.ndarray:
+    ref = np.ascontiguousarray(reference, np.float32)
+    H, W = ref.shape
+    cfg = OrcConfig(int(radius), int(max_iterations), float(threshold), 1 if shape_order == 2 else 0, int(objective),
+                    int(min_valid_samples), 0, float(min_valid_sample_ratio))
+    xy = np.ascontiguousarray(poi_xy, np.float64).reshape(-1, 2)
+    ini = np.ascontiguousarray(init, INITIAL_DTYPE)
+    gx = np.ascontiguousarray(grad_x, np.float64)
+    gy = np.ascontiguousarray(grad_y, np.float64)
+    r = np.ascontiguousarray(roi, np.uint8) if roi is not None else None
+    out = np.zeros(xy.shape[0], RESULT_DTYPE)
+    load().orc_icgn_solve_batch(ctypes.byref(cfg), _p(ref), W, H, _p(gx), _p(gy), ctypes.byref(pre.s), _p(r),
+                                _p(xy), _p(ini), xy.shape[0], _p(out), int(threads))
+    return out

@@ -1,0 +1,269 @@
+"""HIP path vs the CPU oracle, through the C-ABI (libdic_hip.so).
+
+Bars: FP64 prefilter / sampling within 1e-12; FP64 IC-GN with identical
+statuses and valid flags, |du|, |dv| <= 1e-5 px and |dZNCC| <= 1e-6
+(ZNCC = 1 - ZNSSD / 2, so |dZNSSD| <= 2e-6); mixed precision within
+1e-3 px of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import gradient_init, read_gray
+
+pytestmark = pytest.mark.gpu
+
+TOL_UV = 1e-5
+TOL_ZNCC = 1e-6
+
+
+def _report(msg):
+    print(msg, flush=True)
+
+
+def _ini(pkg, u, v, du_dx=0.0, du_dy=0.0, dv_dx=0.0, dv_dy=0.0, conf=0.0):
+    return pkg.initial_guesses(u, v, du_dx, du_dy, dv_dx, dv_dy, conf)
+
+
+def _solver(pkg, ctx, radius=15, order=1, objective="znssd", degree=3, max_it=30, thr=1e-3, precision="fp64",
+            gradient="central"):
+    cfg = pkg.SubsetConfig(subset_radius=radius, max_iterations=max_it, convergence_threshold=thr,
+                           shape_function="second_order" if order == 2 else "first_order", objective=objective)
+    cfg.image_precompute.degree = degree
+    return pkg.ICGNSolver(cfg, precision=precision, gradient=gradient, context=ctx)
+
+
+def _oracle(orc, ref, dfm, xy, ini, radius=15, order=1, objective="znssd", degree=3, max_it=30, thr=1e-3,
+            roi=None, gradient="central"):
+    pre = orc.Precomputed(dfm, degree=degree)
+    if gradient == "central":
+        gx, gy = orc.reference_gradient_cache(ref)
+    else:
+        gx, gy = orc.Precomputed(ref, degree=degree).gradients()
+    return orc.icgn_solve(ref, pre, gx, gy, xy, ini, radius=radius, max_iterations=max_it, threshold=thr,
+                          shape_order=order, objective=1 if objective == "ssd" else 0, roi=roi)
+
+
+def _assert_parity(gpu, cpu, tol_uv=TOL_UV, tol_zncc=TOL_ZNCC, zncc=True):
+    assert np.array_equal(gpu["status"], cpu["status"]), np.nonzero(gpu["status"] != cpu["status"])
+    assert np.array_equal(gpu["valid"], cpu["valid"])
+    ok = cpu["status"] == 0
+    du = np.abs(gpu["p"][ok, 0] - cpu["p"][ok, 0]).max(initial=0)
+    dv = np.abs(gpu["p"][ok, 1] - cpu["p"][ok, 1]).max(initial=0)
+    assert du <= tol_uv and dv <= tol_uv, (du, dv)
+    if zncc:
+        dz = (np.abs(gpu["correlation"][ok] - cpu["correlation"][ok]) / 2.0).max(initial=0)
+        assert dz <= tol_zncc, dz
+    bad = ~ok
+    # failure exits keep the initial u, v and report the confidence
+    assert np.array_equal(gpu["p"][bad, :2], cpu["p"][bad, :2])
+    return du, dv
+
+
+# ------------------------------------------------------------- B-spline ---
+
+@pytest.mark.parametrize("degree", [1, 3, 5])
+@pytest.mark.parametrize("exact", [True, False])
+def test_prefilter_parity(pkg, ctx, orc, degree, exact):
+    rng = np.random.default_rng(degree)
+    for shape in ((37, 53), (130, 257)):
+        img = rng.random(shape).astype(np.float32)
+        dimg = ctx.upload_image(img)
+        for precision, tol in ((pkg.PRECISION_FP64, 1e-12), (pkg.PRECISION_MIXED, 2e-6)):
+            c = ctx.prefilter(dimg, degree, 3, exact, precision)
+            got = c.download()
+            want = orc.form_coefficients(img, degree, 3, exact)
+            err = np.abs(got - want).max()
+            _report(f"prefilter d={degree} exact={exact} {shape} prec={precision}: {err:.2e}")
+            assert err <= tol
+
+
+def test_prefilter_wide_image(pkg, ctx, orc):
+    """Padded rows longer than 4096 samples use more than 64 KiB of LDS."""
+    rng = np.random.default_rng(9)
+    img = rng.random((24, 4100)).astype(np.float32)
+    c = ctx.prefilter(ctx.upload_image(img), 5, 3, True, pkg.PRECISION_FP64)
+    assert np.abs(c.download() - orc.form_coefficients(img, 5, 3, True)).max() <= 1e-12
+
+
+@pytest.mark.parametrize("degree", [1, 3, 5])
+def test_reference_gradients(pkg, ctx, orc, degree):
+    rng = np.random.default_rng(30 + degree)
+    img = rng.random((40, 61)).astype(np.float32)
+    dimg = ctx.upload_image(img)
+    gx, gy = ctx.ref_gradient(dimg, pkg.GRADIENT_CENTRAL, degree, 3, pkg.PRECISION_FP64).download()
+    ox, oy = orc.reference_gradient_cache(img)
+    assert np.array_equal(gx, ox) and np.array_equal(gy, oy)  # bit-exact
+    gx, gy = ctx.ref_gradient(dimg, pkg.GRADIENT_BSPLINE_HALF, degree, 3, pkg.PRECISION_FP64).download()
+    ox, oy = orc.Precomputed(img, degree=degree).gradients()
+    assert np.abs(gx - ox).max() <= 1e-11 and np.abs(gy - oy).max() <= 1e-11
+
+
+@pytest.mark.parametrize("degree", [1, 3, 5])
+def test_sampling_parity(pkg, ctx, orc, degree):
+    rng = np.random.default_rng(40 + degree)
+    img = rng.random((50, 70)).astype(np.float32)
+    c = ctx.prefilter(ctx.upload_image(img), degree, 3, True, pkg.PRECISION_FP64)
+    pts = np.column_stack([rng.uniform(-0.5, 69.5, 2000), rng.uniform(-0.5, 49.5, 2000)])
+    pts[:5] = [[0, 0], [69, 49], [12, 3], [69, 0.5], [68.999, 48.25]]
+    val, grad = ctx.sample(c, pts)
+    ov, og = orc.Precomputed(img, degree=degree).sample(pts)
+    assert np.abs(val - ov).max() <= 1e-12
+    assert np.abs(grad - og).max() <= 1e-11
+
+
+# ---------------------------------------------------------------- IC-GN ---
+
+@pytest.fixture(scope="module")
+def small_pair(synth):
+    W = H = 192
+    f = synth.Field(synth.SYNTH_AFFINE_SINUSOID, u0=1.3, v0=-0.7, ux=2e-3, uy=1e-3, vx=-1.5e-3, vy=2.5e-3,
+                    xc=96, yc=96, au=0.4, lu=120.0, av=0.3, lv=150.0)
+    ref = synth.quantize_unit(synth.render(W, H, 700, 17))
+    dfm = synth.quantize_unit(synth.render(W, H, 700, 17, f))
+    xy = synth.poi_grid(W, H, 6)  # includes margin POIs (InvalidInput)
+    u, v, _ = f.displacement(xy[:, 0], xy[:, 1])
+    return ref, dfm, xy, np.round(u), np.round(v)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("objective", ["znssd", "ssd"])
+@pytest.mark.parametrize("degree", [1, 3, 5])
+def test_icgn_fp64_parity(pkg, ctx, orc, small_pair, order, objective, degree):
+    ref, dfm, xy, u0, v0 = small_pair
+    ini = _ini(pkg, u0, v0, conf=0.25)
+    thr = 1e-3 if objective == "znssd" else 1e-4
+    gpu = _solver(pkg, ctx, 15, order, objective, degree, thr=thr).solve_batch(ref, dfm, xy, ini)
+    cpu = _oracle(orc, ref, dfm, xy, ini, 15, order, objective, degree, thr=thr)
+    du, dv = _assert_parity(gpu, cpu, zncc=(objective == "znssd"))
+    if objective == "ssd":
+        ok = cpu["status"] == 0
+        rel = np.abs(gpu["correlation"][ok] - cpu["correlation"][ok]) / np.maximum(cpu["correlation"][ok], 1e-12)
+        assert rel.max() < 1e-6
+    it_same = np.mean(gpu["iterations"] == cpu["iterations"])
+    _report(f"fp64 order={order} {objective} d={degree}: n={len(xy)} ok={np.sum(cpu['status'] == 0)} "
+            f"du={du:.2e} dv={dv:.2e} same-iterations={it_same:.4f}")
+    assert it_same > 0.99
+
+
+def test_icgn_bspline_gradient_mode(pkg, ctx, orc, small_pair):
+    """Standalone ICGNSolver::solve differentiates the reference B-spline at +0.5 px."""
+    ref, dfm, xy, u0, v0 = small_pair
+    ini = _ini(pkg, u0, v0)
+    gpu = _solver(pkg, ctx, 15, 1, "znssd", 5, gradient="bspline").solve_batch(ref, dfm, xy, ini)
+    cpu = _oracle(orc, ref, dfm, xy, ini, 15, 1, "znssd", 5, gradient="bspline")
+    _assert_parity(gpu, cpu)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_icgn_masked_parity(pkg, ctx, orc, small_pair, order):
+    ref, dfm, xy, u0, v0 = small_pair
+    H, W = ref.shape
+    yy, xx = np.mgrid[0:H, 0:W]
+    roi = (((xx - 100) ** 2 + (yy - 90) ** 2) < 70 ** 2).astype(np.uint8)
+    roi[80:100, 60:75] = 0
+    ini = _ini(pkg, u0, v0, conf=0.5)
+    cfg = pkg.SubsetConfig(subset_radius=15, max_iterations=30, convergence_threshold=1e-3,
+                           shape_function="second_order" if order == 2 else "first_order")
+    cfg.image_precompute.degree = 3
+    gpu = pkg.ICGNSolver(cfg, context=ctx).solve_batch(ref, dfm, xy, ini, roi=roi)
+    cpu = _oracle(orc, ref, dfm, xy, ini, 15, order, "znssd", 3, roi=roi)
+    _assert_parity(gpu, cpu)
+    assert (cpu["status"] == 0).sum() > 100 and (cpu["status"] == 2).sum() > 100
+
+
+def test_icgn_failure_statuses(pkg, ctx, orc, synth):
+    ref = synth.quantize_unit(synth.render(128, 128, 260, 5))
+    flat = np.full_like(ref, 0.5)
+    xy = np.array([[5, 64], [64, 64], [64, 64], [64, 64]], float)
+    ini = _ini(pkg, [0.5, 0.0, 200.0, 0.25], [-0.25, 0.0, 0.0, 0.0], conf=0.7)
+    for a, b, kw in ((ref, ref, {}), (flat, ref, {}), (ref, ref, {"max_it": 0}), (ref, ref, {"radius": 1})):
+        r = kw.get("radius", 15)
+        gpu = _solver(pkg, ctx, r, max_it=kw.get("max_it", 30)).solve_batch(a, b, xy, ini)
+        cpu = _oracle(orc, a, b, xy, ini, r, max_it=kw.get("max_it", 30))
+        assert np.array_equal(gpu["status"], cpu["status"]), (kw, gpu["status"], cpu["status"])
+        assert np.array_equal(gpu["iterations"], cpu["iterations"])
+        both_inf = np.isinf(gpu["correlation"]) & np.isinf(cpu["correlation"])
+        assert np.all(both_inf | (np.abs(gpu["correlation"] - cpu["correlation"]) <= 2e-6))
+    # size mismatch between the images: every POI InvalidInput with its initial guess
+    s = _solver(pkg, ctx)
+    r = s.solve_batch(ref, ref[:100], xy, ini)
+    assert (r["status"] == 2).all() and np.array_equal(r["p"][:, 0], ini["u"])
+
+
+def test_plain_512_parity_and_accuracy(pkg, ctx, orc, synth):
+    """512^2 8-bit speckle, affine, radius 15, first order, bicubic (2304 POIs)."""
+    wl = synth.plain_512()
+    gpu = _solver(pkg, ctx, 15, 1, "znssd", 3).solve_batch(wl.reference, wl.deformed, wl.poi_xy, wl.init)
+    cpu = _oracle(orc, wl.reference, wl.deformed, wl.poi_xy, wl.init, 15, 1, "znssd", 3)
+    du, dv = _assert_parity(gpu, cpu)
+    u, v, _ = wl.truth()
+    assert (gpu["status"] == 0).all() and np.abs(gpu["p"][:, 0] - u).max() < 0.03
+    mixed = _solver(pkg, ctx, 15, 1, "znssd", 3, precision="mixed").solve_batch(wl.reference, wl.deformed,
+                                                                               wl.poi_xy, wl.init)
+    assert np.array_equal(mixed["status"], cpu["status"])
+    em = np.abs(mixed["p"][:, :2] - cpu["p"][:, :2]).max()
+    _report(f"plain_512: fp64 du={du:.2e} dv={dv:.2e}; mixed max|d|={em:.2e}")
+    assert em <= 1e-3
+
+
+@pytest.mark.parametrize("name", ["reference_disparity", "left_temporal", "deformed_disparity"])
+def test_golden_fields_gpu(pkg, ctx, orc, stereo, name):
+    """All 7654 valid POIs of each reference golden field (radius 41, quintic, 1e-6)."""
+    ref = read_gray(stereo["img_00_L"])
+    a, b = stereo["pair_" + name]
+    dfm = read_gray(stereo["img_" + b])
+    rows = stereo["field_" + name]
+    g = gradient_init(rows)
+    ini = _ini(pkg, rows[:, 2], rows[:, 3], *g.T)
+    solver = _solver(pkg, ctx, 41, 1, "znssd", 5, max_it=50, thr=1e-6)
+    gpu = solver.solve_batch(ref, dfm, rows[:, :2], ini)
+    assert (gpu["status"] == 0).all()
+    du = np.abs(gpu["p"][:, 0] - rows[:, 2])
+    dv = np.abs(gpu["p"][:, 1] - rows[:, 3])
+    dc = np.abs(gpu["correlation"] - rows[:, 4])
+    ok = (du <= 1e-5) & (dv <= 1e-5) & (dc <= 2e-6)
+    _report(f"golden {name}: {ok.sum()}/{ok.size} within 1e-5 px; p99 du={np.percentile(du, 99):.2e}")
+    assert (~ok).sum() <= 1
+    # and against the oracle on a slice, at the parity bar
+    sel = np.arange(0, rows.shape[0], 23)
+    cpu = _oracle(orc, ref, dfm, rows[sel, :2], ini[sel], 41, 1, "znssd", 5, max_it=50, thr=1e-6)
+    _assert_parity(gpu[sel], cpu)
+
+
+def test_device_entry_matches_host_entry(pkg, ctx, synth):
+    wl = synth.plain_512()
+    s = _solver(pkg, ctx, 15, 1, "znssd", 3)
+    host = s.solve_batch(wl.reference, wl.deformed, wl.poi_xy, wl.init)
+    pair = s.prepare(wl.reference, wl.deformed)
+    dxy = ctx.to_device(np.ascontiguousarray(wl.poi_xy))
+    dini = ctx.to_device(wl.init)
+    dout = ctx.alloc(wl.poi_xy.shape[0] * pkg.RESULT_DTYPE.itemsize)
+    params = pkg.make_params(s.config, pkg.PRECISION_FP64)
+    for _ in range(2):  # repeatable, bit for bit
+        ctx.solve_device(params, pair.ref, pair.grad, pair.coeffs, dxy.ptr, dini.ptr, wl.poi_xy.shape[0], dout.ptr)
+        ctx.synchronize()
+        dev = dout.download(pkg.RESULT_DTYPE)
+        assert dev.tobytes() == host.tobytes()
+
+
+def test_large_field_2048_fp64_and_mixed(pkg, ctx, orc, synth):
+    """2048^2 float32 speckle, sinusoidal field, noise sigma 2, radius 20, second
+    order, quintic, step 5 (161604 POIs): FP64 vs oracle on a bounded sample,
+    mixed vs FP64 on all POIs."""
+    wl = synth.large_field_2048()
+    n = wl.poi_xy.shape[0]
+    assert n == 161604
+    fp64 = _solver(pkg, ctx, 20, 2, "znssd", 5).solve_batch(wl.reference, wl.deformed, wl.poi_xy, wl.init)
+    mixed = _solver(pkg, ctx, 20, 2, "znssd", 5, precision="mixed").solve_batch(wl.reference, wl.deformed,
+                                                                              wl.poi_xy, wl.init)
+    sel = np.arange(0, n, 97)
+    cpu = _oracle(orc, wl.reference, wl.deformed, wl.poi_xy[sel], wl.init[sel], 20, 2, "znssd", 5)
+    du, dv = _assert_parity(fp64[sel], cpu)
+    assert np.array_equal(mixed["status"], fp64["status"])
+    ok = fp64["status"] == 0
+    em = np.abs(mixed["p"][ok, :2] - fp64["p"][ok, :2]).max()
+    u, v, _ = wl.truth()
+    rms = np.sqrt(np.mean((fp64["p"][ok, 0] - u[ok]) ** 2 + (fp64["p"][ok, 1] - v[ok]) ** 2))
+    _report(f"large_field_2048: ok={ok.sum()}/{n} fp64-vs-oracle du={du:.2e} dv={dv:.2e} "
+            f"mixed-vs-fp64 max={em:.2e} rms-vs-truth={rms:.3e}")
+    assert ok.mean() > 0.999 and em <= 1e-3 and rms < 0.05

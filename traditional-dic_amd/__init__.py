@@ -1,0 +1,19 @@
+"""MI355X (gfx950) subset-correlation hot path of Traditional-DIC.
+
+libdic_hip.so (csrc/, C-ABI in include/dic_hip.h) holds the B-spline
+prefilter, the reference-gradient kernels and the IC-GN kernels; this
+package is the Python host layer over that C-ABI.
+"""
+from . import _lib
+from ._lib import (Context, DicError, INITIAL_DTYPE, RESULT_DTYPE, IcgnParams, load_library, device_count,
+                   PRECISION_FP64, PRECISION_MIXED, SUCCESS, NOT_CONVERGED, INVALID_INPUT, NUMERICAL_FAILURE,
+                   GRADIENT_CENTRAL, GRADIENT_BSPLINE_HALF)
+from .subset import (ICGNSolver, SubsetConfig, BSplinePrecomputeConfig, config_from_dict, compute_grid,
+                     initial_guesses, make_params, results_to_dict, PreparedPair)
+
+__all__ = [
+    "Context", "DicError", "INITIAL_DTYPE", "RESULT_DTYPE", "IcgnParams", "load_library", "device_count",
+    "PRECISION_FP64", "PRECISION_MIXED", "SUCCESS", "NOT_CONVERGED", "INVALID_INPUT", "NUMERICAL_FAILURE",
+    "GRADIENT_CENTRAL", "GRADIENT_BSPLINE_HALF", "ICGNSolver", "SubsetConfig", "BSplinePrecomputeConfig",
+    "config_from_dict", "compute_grid", "initial_guesses", "make_params", "results_to_dict", "PreparedPair",
+]

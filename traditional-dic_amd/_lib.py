@@ -1,0 +1,361 @@
+"""ctypes binding of libdic_hip.so (the C-ABI declared in include/dic_hip.h).
+
+The shared library is built in-tree by ``make -C traditional-dic_amd``.  There
+is no fallback: if the library or a GPU is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdic_hip.so")
+SYNTH_PATH = os.path.join(_HERE, "libdic_synth.so")
+
+# ---- status codes (include/dic_hip.h) ----
+DIC_OK = 0
+DIC_ERR_INVALID_ARGUMENT = 1
+DIC_ERR_HIP = 2
+DIC_ERR_UNSUPPORTED = 3
+DIC_ERR_BUDGET = 4
+DIC_ERR_NO_DEVICE = 5
+
+# SolverStatus order (include/dic/core/types.hpp:35-40 of the reference)
+SUCCESS, NOT_CONVERGED, INVALID_INPUT, NUMERICAL_FAILURE = 0, 1, 2, 3
+
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+OBJECTIVE_ZNSSD, OBJECTIVE_SSD = 0, 1
+OPTIMIZER_ICGN, OPTIMIZER_FGN = 0, 1
+GRADIENT_CENTRAL, GRADIENT_BSPLINE_HALF = 0, 1
+
+EXPORTED_SYMBOLS = (
+    "dic_version", "dic_device_count", "dic_ctx_create", "dic_ctx_destroy", "dic_last_error",
+    "dic_ctx_set_stream", "dic_ctx_stream", "dic_ctx_synchronize", "dic_ctx_bytes_in_use",
+    "dic_image_upload", "dic_image_wrap_device", "dic_image_free", "dic_ref_gradient",
+    "dic_grad_download", "dic_grad_free", "dic_bspline_prefilter", "dic_coeffs_info",
+    "dic_coeffs_download", "dic_coeffs_free", "dic_bspline_sample", "dic_icgn_solve_batch",
+    "dic_icgn_solve_batch_device", "dic_buffer_alloc", "dic_buffer_free", "dic_buffer_upload",
+    "dic_buffer_download", "dic_event_record", "dic_event_elapsed_ms", "dic_event_destroy",
+)
+
+# Displacement2D + iterations (dic_result); InitialDisplacement (dic_initial)
+RESULT_DTYPE = np.dtype([
+    ("x", "<f8"), ("y", "<f8"), ("p", "<f8", (12,)), ("correlation", "<f8"),
+    ("status", "<i4"), ("valid", "<i4"), ("iterations", "<i4"), ("reserved", "<i4"),
+])
+INITIAL_DTYPE = np.dtype([
+    ("u", "<f8"), ("v", "<f8"), ("du_dx", "<f8"), ("du_dy", "<f8"),
+    ("dv_dx", "<f8"), ("dv_dy", "<f8"), ("confidence", "<f8"),
+])
+assert RESULT_DTYPE.itemsize == 136 and INITIAL_DTYPE.itemsize == 56
+
+
+class IcgnParams(ctypes.Structure):
+    _fields_ = [
+        ("subset_radius", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int32),
+        ("convergence_threshold", ctypes.c_double),
+        ("shape_order", ctypes.c_int32),
+        ("objective", ctypes.c_int32),
+        ("optimizer", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("masked", ctypes.c_int32),
+        ("min_valid_samples", ctypes.c_int32),
+        ("min_valid_sample_ratio", ctypes.c_double),
+    ]
+
+
+class DicError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[dic_hip error {code}] {message}")
+        self.code = code
+
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int
+_dp = ctypes.POINTER(ctypes.c_double)
+
+_SIGNATURES = {
+    "dic_version": ([], ctypes.c_char_p),
+    "dic_device_count": ([ctypes.POINTER(ctypes.c_int)], _i32),
+    "dic_ctx_create": ([_i32, ctypes.c_uint64, ctypes.POINTER(_vp)], _i32),
+    "dic_ctx_destroy": ([_vp], None),
+    "dic_last_error": ([_vp], ctypes.c_char_p),
+    "dic_ctx_set_stream": ([_vp, _vp], _i32),
+    "dic_ctx_stream": ([_vp], _vp),
+    "dic_ctx_synchronize": ([_vp], _i32),
+    "dic_ctx_bytes_in_use": ([_vp], ctypes.c_uint64),
+    "dic_image_upload": ([_vp, _vp, _i32, _i32, ctypes.POINTER(_vp)], _i32),
+    "dic_image_wrap_device": ([_vp, _vp, _i32, _i32, ctypes.POINTER(_vp)], _i32),
+    "dic_image_free": ([_vp], None),
+    "dic_ref_gradient": ([_vp, _vp, _i32, _i32, _i32, _i32, ctypes.POINTER(_vp)], _i32),
+    "dic_grad_download": ([_vp, _vp, _vp, _vp], _i32),
+    "dic_grad_free": ([_vp], None),
+    "dic_bspline_prefilter": ([_vp, _vp, _i32, _i32, _i32, _i32, ctypes.POINTER(_vp)], _i32),
+    "dic_coeffs_info": ([_vp] + [ctypes.POINTER(ctypes.c_int)] * 5, _i32),
+    "dic_coeffs_download": ([_vp, _vp, _vp], _i32),
+    "dic_coeffs_free": ([_vp], None),
+    "dic_bspline_sample": ([_vp, _vp, _vp, _i32, _vp, _vp], _i32),
+    "dic_icgn_solve_batch": ([_vp, ctypes.POINTER(IcgnParams), _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp], _i32),
+    "dic_icgn_solve_batch_device": (
+        [_vp, ctypes.POINTER(IcgnParams), _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp], _i32),
+    "dic_buffer_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], _i32),
+    "dic_buffer_free": ([_vp, _vp], None),
+    "dic_buffer_upload": ([_vp, _vp, _vp, ctypes.c_size_t], _i32),
+    "dic_buffer_download": ([_vp, _vp, _vp, ctypes.c_size_t], _i32),
+    "dic_event_record": ([_vp, ctypes.POINTER(_vp)], _i32),
+    "dic_event_elapsed_ms": ([_vp, _vp, ctypes.POINTER(ctypes.c_float)], _i32),
+    "dic_event_destroy": ([_vp], None),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libdic_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build it with `make -C traditional-dic_amd` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = ctypes.CDLL(path)
+    for name, (argtypes, restype) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
+    _lib = lib
+    return lib
+
+
+def device_count() -> int:
+    lib = load_library()
+    n = ctypes.c_int(0)
+    rc = lib.dic_device_count(ctypes.byref(n))
+    return n.value if rc == DIC_OK else 0
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class Context:
+    """Owns a dic_ctx (device, stream, HBM budget)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        self.lib = load_library()
+        h = _vp()
+        rc = self.lib.dic_ctx_create(int(device), int(hbm_budget_bytes), ctypes.byref(h))
+        if rc != DIC_OK:
+            raise DicError(rc, "dic_ctx_create failed (no HIP device?)")
+        self.handle = h
+
+    def check(self, rc: int):
+        if rc != DIC_OK:
+            msg = self.lib.dic_last_error(self.handle)
+            raise DicError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.dic_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_stream(self, stream_handle: int | None):
+        self.check(self.lib.dic_ctx_set_stream(self.handle, stream_handle or None))
+
+    def stream(self) -> int:
+        return self.lib.dic_ctx_stream(self.handle) or 0
+
+    def synchronize(self):
+        self.check(self.lib.dic_ctx_synchronize(self.handle))
+
+    def bytes_in_use(self) -> int:
+        return int(self.lib.dic_ctx_bytes_in_use(self.handle))
+
+    # ---- device buffers and events ----
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        p = _vp()
+        self.check(self.lib.dic_buffer_alloc(self.handle, int(nbytes), ctypes.byref(p)))
+        return DeviceBuffer(self, p.value, int(nbytes))
+
+    def to_device(self, arr: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr)
+        buf = self.alloc(a.nbytes)
+        self.check(self.lib.dic_buffer_upload(self.handle, buf.ptr, _ptr(a), a.nbytes))
+        return buf
+
+    def record_event(self) -> int:
+        e = _vp()
+        self.check(self.lib.dic_event_record(self.handle, ctypes.byref(e)))
+        return e.value
+
+    def elapsed_ms(self, start: int, end: int) -> float:
+        ms = ctypes.c_float()
+        rc = self.lib.dic_event_elapsed_ms(start, end, ctypes.byref(ms))
+        if rc != DIC_OK:
+            raise DicError(rc, "event timing failed")
+        return float(ms.value)
+
+    def destroy_event(self, e: int):
+        self.lib.dic_event_destroy(e)
+
+    # ---- handles ----
+    def upload_image(self, img: np.ndarray) -> "DeviceImage":
+        a = np.ascontiguousarray(img, dtype=np.float32)
+        if a.ndim != 2:
+            raise ValueError("Image must be a 2D numpy array.")
+        h = _vp()
+        self.check(self.lib.dic_image_upload(self.handle, _ptr(a), a.shape[1], a.shape[0], ctypes.byref(h)))
+        return DeviceImage(self, h, a.shape[1], a.shape[0])
+
+    def wrap_image(self, device_ptr: int, width: int, height: int) -> "DeviceImage":
+        h = _vp()
+        self.check(self.lib.dic_image_wrap_device(self.handle, device_ptr, width, height, ctypes.byref(h)))
+        return DeviceImage(self, h, width, height)
+
+    def prefilter(self, img: "DeviceImage", degree: int = 5, border: int = 3, exact: bool = True,
+                  precision: int = PRECISION_FP64) -> "Coefficients":
+        h = _vp()
+        self.check(self.lib.dic_bspline_prefilter(self.handle, img.handle, degree, border, int(exact), precision,
+                                                  ctypes.byref(h)))
+        return Coefficients(self, h)
+
+    def ref_gradient(self, img: "DeviceImage", mode: int = GRADIENT_CENTRAL, degree: int = 5, border: int = 3,
+                     precision: int = PRECISION_FP64) -> "Gradient":
+        h = _vp()
+        self.check(self.lib.dic_ref_gradient(self.handle, img.handle, mode, degree, border, precision,
+                                             ctypes.byref(h)))
+        return Gradient(self, h, img.width, img.height)
+
+    def sample(self, coeffs: "Coefficients", xy: np.ndarray):
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        n = xy.shape[0]
+        val = np.empty(n, np.float64)
+        grad = np.empty((n, 2), np.float64)
+        self.check(self.lib.dic_bspline_sample(self.handle, coeffs.handle, _ptr(xy), n, _ptr(val), _ptr(grad)))
+        return val, grad
+
+    def solve(self, params: IcgnParams, ref: "DeviceImage", grad: "Gradient", coeffs: "Coefficients",
+              poi_xy: np.ndarray, init: np.ndarray, roi: np.ndarray | None = None) -> np.ndarray:
+        xy = np.ascontiguousarray(poi_xy, dtype=np.float64).reshape(-1, 2)
+        ini = np.ascontiguousarray(init, dtype=INITIAL_DTYPE)
+        n = xy.shape[0]
+        if ini.shape[0] != n:
+            raise ValueError("one initial guess per POI is required")
+        out = np.zeros(n, RESULT_DTYPE)
+        roi_p = None
+        if roi is not None:
+            r = np.ascontiguousarray(roi, dtype=np.uint8)
+            roi_p = _ptr(r)
+        self.check(self.lib.dic_icgn_solve_batch(self.handle, ctypes.byref(params), ref.handle, grad.handle,
+                                                 coeffs.handle, roi_p, _ptr(xy), _ptr(ini), n, _ptr(out)))
+        return out
+
+    def solve_device(self, params: IcgnParams, ref: "DeviceImage", grad: "Gradient", coeffs: "Coefficients",
+                     xy_ptr: int, init_ptr: int, n: int, out_ptr: int, roi_ptr: int | None = None,
+                     stream: int | None = None):
+        self.check(self.lib.dic_icgn_solve_batch_device(self.handle, ctypes.byref(params), ref.handle, grad.handle,
+                                                        coeffs.handle, roi_ptr, xy_ptr, init_ptr, int(n), out_ptr,
+                                                        stream))
+
+
+class DeviceBuffer:
+    """Raw HBM allocation of a context (dic_buffer_alloc)."""
+
+    def __init__(self, ctx: Context, ptr: int, nbytes: int):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def download(self, dtype, count: int | None = None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else count
+        out = np.empty(n, dt)
+        self.ctx.check(self.ctx.lib.dic_buffer_download(self.ctx.handle, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def upload(self, arr: np.ndarray):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        self.ctx.check(self.ctx.lib.dic_buffer_upload(self.ctx.handle, self.ptr, _ptr(a), a.nbytes))
+
+    def free(self):
+        if self.ptr and self.ctx.handle:
+            self.ctx.lib.dic_buffer_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class _Handle:
+    _free = None
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def free(self):
+        if self.handle and self.ctx.handle:
+            getattr(self.ctx.lib, self._free)(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceImage(_Handle):
+    _free = "dic_image_free"
+
+    def __init__(self, ctx, handle, width, height):
+        super().__init__(ctx, handle)
+        self.width, self.height = width, height
+
+
+class Gradient(_Handle):
+    _free = "dic_grad_free"
+
+    def __init__(self, ctx, handle, width, height):
+        super().__init__(ctx, handle)
+        self.width, self.height = width, height
+
+    def download(self):
+        gx = np.empty((self.height, self.width), np.float64)
+        gy = np.empty_like(gx)
+        self.ctx.check(self.ctx.lib.dic_grad_download(self.ctx.handle, self.handle, _ptr(gx), _ptr(gy)))
+        return gx, gy
+
+
+class Coefficients(_Handle):
+    _free = "dic_coeffs_free"
+
+    def info(self):
+        v = [ctypes.c_int() for _ in range(5)]
+        self.ctx.check(self.ctx.lib.dic_coeffs_info(self.handle, *[ctypes.byref(x) for x in v]))
+        w, h, d, b, p = (x.value for x in v)
+        return {"width": w, "height": h, "degree": d, "border": b, "precision": p}
+
+    def download(self) -> np.ndarray:
+        i = self.info()
+        out = np.empty((i["height"] + 2 * i["border"], i["width"] + 2 * i["border"]), np.float64)
+        self.ctx.check(self.ctx.lib.dic_coeffs_download(self.ctx.handle, self.handle, _ptr(out)))
+        return out

@@ -1,0 +1,90 @@
+// dic_internal.hpp -- handle layouts and kernel launchers shared by the
+// translation units of libdic_hip.so.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "dic_hip.h"
+
+struct dic_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    std::atomic<uint64_t> in_use{0};
+    std::string error;
+    std::mutex table_mutex;
+    // circular-subset offset tables, one per radius: packed (dx, dy) int16 pairs
+    std::map<int, std::pair<int *, int>> subset_tables;
+    // sizes of dic_buffer_alloc allocations, for budget accounting
+    std::map<void *, size_t> buffers;
+    // scratch used by the host-pointer entry points
+    void *staging = nullptr;
+    size_t staging_bytes = 0;
+};
+
+struct dic_image {
+    dic_ctx *ctx = nullptr;
+    int width = 0, height = 0;
+    float *data = nullptr;
+    bool owned = false;
+    size_t bytes = 0;
+};
+
+struct dic_grad {
+    dic_ctx *ctx = nullptr;
+    int width = 0, height = 0, precision = 0;
+    void *gx = nullptr, *gy = nullptr;
+    size_t bytes = 0;
+};
+
+struct dic_coeffs {
+    dic_ctx *ctx = nullptr;
+    int width = 0, height = 0, degree = 5, border = 3, precision = 0;
+    int pitch = 0, rows = 0; // padded width and height
+    void *data = nullptr;
+    size_t bytes = 0;
+};
+
+namespace dic {
+
+// device memory accounting against the context budget
+hipError_t ctx_malloc(dic_ctx *ctx, void **ptr, size_t bytes);
+void ctx_free(dic_ctx *ctx, void *ptr, size_t bytes);
+int set_error(dic_ctx *ctx, int code, const std::string &msg);
+int check_hip(dic_ctx *ctx, hipError_t e, const char *what);
+
+// one-line sample table of a circular subset (dy outer, dx inner, dx^2+dy^2 <= r^2)
+int get_subset_table(dic_ctx *ctx, int radius, const int **table, int *count);
+
+// bspline.hip
+hipError_t launch_prefilter(const float *img, int W, int H, int degree, int border, int exact, int precision,
+                            void *out, double *tmp_a, double *tmp_b, hipStream_t s);
+hipError_t launch_gradient_central(const float *img, int W, int H, int precision, void *gx, void *gy, hipStream_t s);
+hipError_t launch_gradient_bspline(const dic_coeffs *c, int precision, void *gx, void *gy, hipStream_t s);
+hipError_t launch_sample(const dic_coeffs *c, const double *xy, int n, double *val, double *grad, hipStream_t s);
+
+// icgn.hip
+struct IcgnLaunch {
+    const dic_icgn_params *params;
+    int width, height;
+    const float *ref;
+    const void *gx, *gy;
+    const void *coeffs;
+    int pitch, border, degree, precision;
+    const uint8_t *roi;
+    const int *table;
+    int table_count;
+    const double *poi_xy;
+    const dic_initial *init;
+    dic_result *out;
+    int n;
+};
+hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s);
+
+} // namespace dic

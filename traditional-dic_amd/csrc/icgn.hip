@@ -1,0 +1,926 @@
+// icgn.hip -- per-POI inverse-compositional Gauss-Newton (IC-GN) for gfx950.
+//
+// Replaces ICGNSolver::solve_{first,second}_order_{znssd,ssd}[_masked]
+// (src/subset/solver/icgn.cpp:446-1912).  Mapping: one 64-lane wavefront
+// (one workgroup) per POI.  Lanes stride over the circular subset's samples
+// (sample table in dy-outer/dx-inner order, icgn.cpp:485-489); every sum the
+// reference forms serially is a per-lane partial followed by a butterfly
+// (xor-shuffle) reduction, after which every lane holds the identical total.
+// The Hessian is accumulated and LDL^T-factored once per POI (Eigen 3.4 LDLT
+// semantics, icgn.cpp:532-542); per iteration lane 0 solves for dp, applies
+// the inverse-compositional update (icgn.cpp:83-120 / :289-357) and
+// publishes the new parameters through LDS.
+//
+// Precision variants
+//   R = double (plain drop-in): FP64 everywhere, the reference's two-pass
+//     mean / norm / residual structure (icgn.cpp:577-592), deformed samples
+//     kept in LDS between the passes.
+//   R = float (mixed): FP32 coefficients, gradients, warp offsets (relative to
+//     the integer part of u, v), sampling and per-lane Kahan-compensated sums;
+//     FP64 cross-lane reductions, Hessian factorisation, solve, warp update
+//     and parameter accumulator.  ZNSSD sums are taken in one pass about the
+//     shift K = mean(f):  sum(g-K), sum((g-K)^2), sum(fhat (g-K)),
+//     sum((g-K) sd); mean, norm, G and C follow algebraically.
+#include <cfloat>
+#include <cmath>
+
+#include "bspline_device.hpp"
+#include "dic_internal.hpp"
+
+namespace dic {
+namespace {
+
+constexpr double kEps = 1e-12; // icgn.cpp:13
+
+template <typename R> struct Acc;
+template <> struct Acc<double> {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double v) { s += v; }
+    __device__ __forceinline__ double value() const { return s; }
+};
+template <> struct Acc<float> { // Kahan-compensated FP32 partial
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float v)
+    {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <int P, typename R> __device__ __forceinline__ void steepest_descent(R gx, R gy, R lx, R ly, R *sd)
+{
+    sd[0] = gx;
+    sd[1] = gy;
+    sd[2] = gx * lx;
+    sd[3] = gx * ly;
+    sd[4] = gy * lx;
+    sd[5] = gy * ly;
+    if constexpr (P == 12) { // second_order_steepest_descent, icgn.cpp:142-165
+        const R dx22 = lx * lx * R(0.5), dy22 = ly * ly * R(0.5), dxy = lx * ly;
+        sd[6] = gx * dx22;
+        sd[7] = gx * dxy;
+        sd[8] = gx * dy22;
+        sd[9] = gy * dx22;
+        sd[10] = gy * dxy;
+        sd[11] = gy * dy22;
+    }
+}
+
+struct IcgnArgs {
+    int W, H, pitch, b;
+    const float *ref;
+    const void *gx, *gy, *coeffs;
+    const uint8_t *roi;
+    const int *table;
+    int nfull;
+    int radius, max_it;
+    double thr;
+    int min_needed;
+    const double *xy;
+    const dic_initial *init;
+    dic_result *out;
+    int n;
+};
+
+struct IcgnShared {
+    double H[144];  // Hessian, then its LDL^T factor (lower triangle)
+    double lu[36];  // 6x6 scratch for the second-order update
+    double inv[36];
+    double gp[36];
+    double p[12];   // current warp parameters (FP64 accumulator)
+    double x[12];   // solve work vector
+    double dp[12];  // parameter increment
+    double pn[12];  // updated parameters
+    double g6[6];   // 6-vector work area of the 6x6 inverse
+    double red[32]; // wave-reduced sums of the current pass
+    double ssd[12], sfsd[12];
+    double fmean, fnorm, sfh, sfh2, corr;
+    int tr[12];
+    int rt[6], ct[6];
+    int ns, state;
+};
+
+__device__ __forceinline__ double uniform_f64(double v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float uniform_f32(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// ---- Eigen 3.4 LDLT (ldlt_inplace<Lower>::unblocked), lane 0 only ----
+template <int P> __device__ void ldlt_factor(double *a, int *tr, double *temp, int *ok, int *is_positive)
+{
+    int sign = 2 /*zero*/, found_zero = 0, ret = 1;
+    for (int k = 0; k < P; ++k) {
+        int big = k;
+        double bigv = fabs(a[k * P + k]);
+        for (int i = k + 1; i < P; ++i)
+            if (fabs(a[i * P + i]) > bigv) {
+                bigv = fabs(a[i * P + i]);
+                big = i;
+            }
+        tr[k] = big;
+        if (k != big) {
+            for (int j = 0; j < k; ++j) {
+                const double t = a[k * P + j];
+                a[k * P + j] = a[big * P + j];
+                a[big * P + j] = t;
+            }
+            for (int i = big + 1; i < P; ++i) {
+                const double t = a[i * P + k];
+                a[i * P + k] = a[i * P + big];
+                a[i * P + big] = t;
+            }
+            const double t = a[k * P + k];
+            a[k * P + k] = a[big * P + big];
+            a[big * P + big] = t;
+            for (int i = k + 1; i < big; ++i) {
+                const double t2 = a[i * P + k];
+                a[i * P + k] = a[big * P + i];
+                a[big * P + i] = t2;
+            }
+        }
+        if (k > 0) {
+            for (int j = 0; j < k; ++j) temp[j] = a[j * P + j] * a[k * P + j];
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s += a[k * P + j] * temp[j];
+            a[k * P + k] -= s;
+            for (int i = k + 1; i < P; ++i) {
+                double s2 = 0.0;
+                for (int j = 0; j < k; ++j) s2 += a[i * P + j] * temp[j];
+                a[i * P + k] -= s2;
+            }
+        }
+        const double akk = a[k * P + k];
+        const bool valid = fabs(akk) > 0.0;
+        if (k == 0 && !valid) {
+            sign = 2;
+            for (int j = 0; j < P; ++j) tr[j] = j;
+            ret = 0;
+            break;
+        }
+        if (k + 1 < P && valid) {
+            for (int i = k + 1; i < P; ++i) a[i * P + k] /= akk;
+        } else if (k + 1 < P) {
+            for (int i = k + 1; i < P; ++i) ret = ret && (a[i * P + k] == 0.0);
+        }
+        if (found_zero && valid) ret = 0;
+        else if (!valid) found_zero = 1;
+        if (sign == 0) {
+            if (akk < 0.0) sign = 3;
+        } else if (sign == 1) {
+            if (akk > 0.0) sign = 3;
+        } else if (sign == 2) {
+            if (akk > 0.0) sign = 0;
+            else if (akk < 0.0) sign = 1;
+        }
+    }
+    *ok = ret;
+    *is_positive = (sign == 0 || sign == 2);
+}
+
+// inverse_compositional_affine_update, icgn.cpp:83-120
+__device__ __forceinline__ void ic_affine_update(const double *p, const double *d, double *o)
+{
+    const double du = p[0], dv = p[1], dudx = p[2], dudy = p[3], dvdx = p[4], dvdy = p[5];
+    const double dp0 = d[0], dp1 = d[1], dp2 = d[2], dp3 = d[3], dp4 = d[4], dp5 = d[5];
+    const double den = dp2 + dp5 + dp2 * dp5 - dp3 * dp4 + 1.0;
+    if (fabs(den) <= kEps) {
+        for (int i = 0; i < 6; ++i) o[i] = __builtin_nan("");
+        return;
+    }
+    o[0] = du - ((dudx + 1.0) * (dp0 + dp0 * dp5 - dp1 * dp3)) / den - (dudy * (dp1 - dp0 * dp4 + dp1 * dp2)) / den;
+    o[1] = dv - ((dvdy + 1.0) * (dp1 - dp0 * dp4 + dp1 * dp2)) / den - (dvdx * (dp0 + dp0 * dp5 - dp1 * dp3)) / den;
+    o[2] = ((dp5 + 1.0) * (dudx + 1.0)) / den - (dp4 * dudy) / den - 1.0;
+    o[3] = (dudy * (dp2 + 1.0)) / den - (dp3 * (dudx + 1.0)) / den;
+    o[4] = (dvdx * (dp5 + 1.0)) / den - (dp4 * (dvdy + 1.0)) / den;
+    o[5] = ((dp2 + 1.0) * (dvdy + 1.0)) / den - (dp3 * dvdx) / den - 1.0;
+}
+
+// build_second_order_warp_matrix, icgn.cpp:196-261 (row-major 6x6)
+__device__ void warp_matrix(const double *p, double *G)
+{
+    const double u = p[0], v = p[1], ux = p[2], uy = p[3], vx = p[4], vy = p[5];
+    const double uxx = p[6], uxy = p[7], uyy = p[8], vxx = p[9], vxy = p[10], vyy = p[11];
+    G[0] = u; G[1] = 1.0 + ux; G[2] = uy; G[3] = 0.5 * uxx; G[4] = uxy; G[5] = 0.5 * uyy;
+    G[6] = v; G[7] = vx; G[8] = 1.0 + vy; G[9] = 0.5 * vxx; G[10] = vxy; G[11] = 0.5 * vyy;
+    G[12] = 1.0; G[13] = 0.0; G[14] = 0.0; G[15] = 0.0; G[16] = 0.0; G[17] = 0.0;
+    G[18] = u * u;
+    G[19] = 2.0 * u * (1.0 + ux);
+    G[20] = 2.0 * u * uy;
+    G[21] = 1.0 + 2.0 * ux + ux * ux + u * uxx;
+    G[22] = 2.0 * (uy + ux * uy + u * uxy);
+    G[23] = uy * uy + u * uyy;
+    G[24] = u * v;
+    G[25] = v + u * vx + v * ux;
+    G[26] = u + u * vy + v * uy;
+    G[27] = vx + ux * vx + 0.5 * (u * vxx + v * uxx);
+    G[28] = 1.0 + vy + ux + ux * vy + uy * vx + u * vxy + v * uxy;
+    G[29] = uy + uy * vy + 0.5 * (u * vyy + v * uyy);
+    G[30] = v * v;
+    G[31] = 2.0 * v * vx;
+    G[32] = 2.0 * v * (1.0 + vy);
+    G[33] = vx * vx + v * vxx;
+    G[34] = 2.0 * (vx + vx * vy + v * vxy);
+    G[35] = 1.0 + 2.0 * vy + vy * vy + v * vyy;
+}
+
+// Eigen 3.4 FullPivLU of a 6x6 (complete pivoting, first maximum in
+// column-major scan) + rank test + inverse; returns false if not invertible.
+__device__ bool fullpivlu_inverse6(double *lu, int *rt, int *ct, double *inv, double *c)
+{
+    int nonzero = 6;
+    double maxpivot = 0.0;
+    for (int k = 0; k < 6; ++k) {
+        int br = k, bc = k;
+        double best = -1.0;
+        for (int c = k; c < 6; ++c)
+            for (int r = k; r < 6; ++r) {
+                const double s = fabs(lu[r * 6 + c]);
+                if (s > best) {
+                    best = s;
+                    br = r;
+                    bc = c;
+                }
+            }
+        if (best == 0.0) {
+            nonzero = k;
+            for (int i = k; i < 6; ++i) rt[i] = ct[i] = i;
+            break;
+        }
+        if (best > maxpivot) maxpivot = best;
+        rt[k] = br;
+        ct[k] = bc;
+        if (k != br)
+            for (int c = 0; c < 6; ++c) {
+                const double t = lu[k * 6 + c];
+                lu[k * 6 + c] = lu[br * 6 + c];
+                lu[br * 6 + c] = t;
+            }
+        if (k != bc)
+            for (int r = 0; r < 6; ++r) {
+                const double t = lu[r * 6 + k];
+                lu[r * 6 + k] = lu[r * 6 + bc];
+                lu[r * 6 + bc] = t;
+            }
+        const double piv = lu[k * 6 + k];
+        for (int r = k + 1; r < 6; ++r) lu[r * 6 + k] /= piv;
+        for (int r = k + 1; r < 6; ++r)
+            for (int c = k + 1; c < 6; ++c) lu[r * 6 + c] -= lu[r * 6 + k] * lu[k * 6 + c];
+    }
+    const double thr = fabs(maxpivot) * (6.0 * DBL_EPSILON);
+    int rank = 0;
+    for (int i = 0; i < nonzero; ++i) rank += fabs(lu[i * 6 + i]) > thr;
+    if (rank != 6) return false;
+    for (int col = 0; col < 6; ++col) {
+        for (int i = 0; i < 6; ++i) c[i] = (i == col) ? 1.0 : 0.0;
+        for (int k = 0; k < 6; ++k)
+            if (rt[k] != k) {
+                const double t = c[k];
+                c[k] = c[rt[k]];
+                c[rt[k]] = t;
+            }
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < i; ++j) c[i] -= lu[i * 6 + j] * c[j];
+        for (int i = 5; i >= 0; --i) {
+            for (int j = i + 1; j < 6; ++j) c[i] -= lu[i * 6 + j] * c[j];
+            c[i] /= lu[i * 6 + i];
+        }
+        for (int k = 5; k >= 0; --k)
+            if (ct[k] != k) {
+                const double t = c[k];
+                c[k] = c[ct[k]];
+                c[ct[k]] = t;
+            }
+        for (int i = 0; i < 6; ++i) inv[i * 6 + col] = c[i];
+    }
+    return true;
+}
+
+// inverse_compositional_second_order_update, icgn.cpp:289-357, lane 0 only,
+// 6x6 scratch in LDS.
+__device__ void ic_second_order_update(const double *p, const double *d, double *o, IcgnShared &sh)
+{
+    warp_matrix(p, sh.gp);
+    warp_matrix(d, sh.lu);
+    if (!fullpivlu_inverse6(sh.lu, sh.rt, sh.ct, sh.inv, sh.g6)) {
+        for (int i = 0; i < 12; ++i) o[i] = p[i] - d[i];
+        return;
+    }
+    // rows 0 and 1 of G(p) * G(dp)^-1 * P^-1 (extract_from_warp_matrix reads
+    // only those): column 0 <- product column 2, 1 <- 0, 2 <- 1, 3..5 identity
+    auto T = [&](int r, int c) {
+        double acc = 0.0;
+#pragma unroll 1
+        for (int k = 0; k < 6; ++k) acc += sh.gp[r * 6 + k] * sh.inv[k * 6 + c];
+        return acc;
+    };
+    o[0] = T(0, 2);
+    o[2] = T(0, 0) - 1.0;
+    o[3] = T(0, 1);
+    o[6] = 2.0 * T(0, 3);
+    o[7] = T(0, 4);
+    o[8] = 2.0 * T(0, 5);
+    o[1] = T(1, 2);
+    o[4] = T(1, 0);
+    o[5] = T(1, 1) - 1.0;
+    o[9] = 2.0 * T(1, 3);
+    o[10] = T(1, 4);
+    o[11] = 2.0 * T(1, 5);
+}
+
+template <int D, typename R, typename CS>
+__device__ __forceinline__ R sample_at(const CS *__restrict__ coeffs, int pitch, int b, int ix, int iy, R tx, R ty)
+{
+    using S = Spline<D>;
+    R wx[S::n], wy[S::n];
+    S::weights(tx, wx);
+    S::weights(ty, wy);
+    const CS *base = coeffs + (size_t)(iy + b - S::off) * pitch + (ix + b - S::off);
+    R v = R(0);
+#pragma unroll
+    for (int j = 0; j < S::n; ++j) {
+        R rv = R(0);
+#pragma unroll
+        for (int k = 0; k < S::n; ++k) rv += wx[k] * (R)base[(size_t)j * pitch + k];
+        v += wy[j] * rv;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void decode(int t, int &dx, int &dy)
+{
+    dx = (int)(short)(t & 0xffff);
+    dy = t >> 16;
+}
+
+// Sum over the wave; lane 0 stores the total to LDS.
+__device__ __forceinline__ void reduce_to(double v, double *dst, int lane)
+{
+    v = wave_sum(v);
+    if (lane == 0) *dst = v;
+}
+
+// ---- per-iteration serial step, lane 0 only (icgn.cpp:584-610) ----
+// Reads the reduced sums of the sampling pass from sh.red, forms G and C,
+// solves dp = -H^-1 G with the stored LDL^T factor, checks finiteness,
+// applies the inverse-compositional update and sets sh.state:
+// 0 continue, 1 converged, 2 numerical failure.
+template <int P, int OBJ, bool TWOPASS>
+__device__ __forceinline__ void lane0_step(IcgnShared &sh, double thr)
+{
+    constexpr bool ZNSSD = OBJ == DIC_OBJECTIVE_ZNSSD;
+    double *x = sh.x;
+    double C;
+    if constexpr (TWOPASS) { // red = [G(P) unscaled, C]
+        const double sc = 2.0 / sh.fnorm;
+#pragma unroll 1
+        for (int j = 0; j < P; ++j) x[j] = sh.red[j] * sc;
+        C = sh.red[P];
+    } else if constexpr (ZNSSD) { // red = [S1, S2, SFG, S(g-K)sd (P)]
+        const double S1 = sh.red[0], S2 = sh.red[1], SFG = sh.red[2];
+        const double md = S1 / (double)sh.ns;
+        const double var = S2 - S1 * md;
+        if (!(var > 0.0) || sqrt(var) <= kEps) {
+            sh.state = 2;
+            return;
+        }
+        const double gnorm = sqrt(var), sc = 2.0 / sh.fnorm;
+#pragma unroll 1
+        for (int j = 0; j < P; ++j) x[j] = sc * (sh.sfsd[j] - (sh.red[3 + j] - md * sh.ssd[j]) / gnorm);
+        C = sh.sfh2 - 2.0 * (SFG - md * sh.sfh) / gnorm + var / (gnorm * gnorm);
+    } else { // SSD: red = [C, G(P)]
+        C = sh.red[0];
+#pragma unroll 1
+        for (int j = 0; j < P; ++j) x[j] = sh.red[1 + j];
+    }
+    sh.corr = C;
+    // Eigen LDLT::solve: P b, L^-1, D^+ (|d| > DBL_MIN), L^-T, P^T
+#pragma unroll 1
+    for (int k = 0; k < P; ++k) {
+        const int t = sh.tr[k];
+        if (t != k) {
+            const double v = x[k];
+            x[k] = x[t];
+            x[t] = v;
+        }
+    }
+#pragma unroll 1
+    for (int i = 0; i < P; ++i) {
+        double s = x[i];
+#pragma unroll 1
+        for (int j = 0; j < i; ++j) s -= sh.H[i * P + j] * x[j];
+        x[i] = s;
+    }
+#pragma unroll 1
+    for (int i = 0; i < P; ++i) {
+        const double d = sh.H[i * P + i];
+        x[i] = fabs(d) > DBL_MIN ? x[i] / d : 0.0;
+    }
+#pragma unroll 1
+    for (int i = P - 1; i >= 0; --i) {
+        double s = x[i];
+#pragma unroll 1
+        for (int j = i + 1; j < P; ++j) s -= sh.H[j * P + i] * x[j];
+        x[i] = s;
+    }
+#pragma unroll 1
+    for (int k = P - 1; k >= 0; --k) {
+        const int t = sh.tr[k];
+        if (t != k) {
+            const double v = x[k];
+            x[k] = x[t];
+            x[t] = v;
+        }
+    }
+    bool fin = true;
+    double nrm = 0.0;
+#pragma unroll 1
+    for (int j = 0; j < P; ++j) {
+        const double d = -x[j];
+        sh.dp[j] = d;
+        fin = fin && isfinite(d);
+        nrm += d * d;
+    }
+    if (!fin) {
+        sh.state = 2;
+        return;
+    }
+    nrm = sqrt(nrm);
+    if constexpr (P == 6) {
+        ic_affine_update(sh.p, sh.dp, sh.pn);
+    } else {
+        ic_second_order_update(sh.p, sh.dp, sh.pn, sh);
+    }
+    bool pf = true;
+#pragma unroll 1
+    for (int j = 0; j < P; ++j) {
+        const double v = sh.pn[j];
+        sh.p[j] = v;
+        pf = pf && isfinite(v);
+    }
+    sh.state = !pf ? 2 : (nrm < thr ? 1 : 0);
+}
+
+template <int P, int OBJ, int D, typename R, bool MASKED>
+__global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
+{
+    constexpr bool ZNSSD = OBJ == DIC_OBJECTIVE_ZNSSD;
+    constexpr bool FP64 = sizeof(R) == 8;
+    constexpr bool TWOPASS = ZNSSD && FP64;
+    __shared__ IcgnShared sh;
+    extern __shared__ double gbuf[];
+
+    const int lane = threadIdx.x;
+    const int poi = blockIdx.x;
+    if (poi >= a.n) return;
+    const R *__restrict__ gxp = (const R *)a.gx;
+    const R *__restrict__ gyp = (const R *)a.gy;
+    const R *__restrict__ cf = (const R *)a.coeffs;
+    const int W = a.W, Hh = a.H;
+    const double px = a.xy[2 * poi], py = a.xy[2 * poi + 1];
+    const int cx = (int)round(px), cy = (int)round(py);
+
+    // sample s of the full circle is used by this POI (masked: inside the image and the ROI)
+    auto usable = [&](int x, int y) -> bool {
+        if constexpr (MASKED) return x >= 0 && y >= 0 && x < W && y < Hh && a.roi[(size_t)y * W + x];
+        return true;
+    };
+
+    int status = DIC_INVALID_INPUT;
+    int iterations = 0;
+    bool early = W <= 0 || Hh <= 0 || a.radius < 1 || a.max_it < 0;
+    if (!early) {
+        if constexpr (!MASKED) {
+            early = cx - a.radius < 0 || cy - a.radius < 0 || cx + a.radius >= W || cy + a.radius >= Hh;
+        } else {
+            early = !usable(cx, cy);
+        }
+    }
+
+    // ---- reference statistics: icgn.cpp:485-530 ----
+    if (!early) {
+        double fsum_l = 0.0;
+        int cnt = 0;
+        for (int s = lane; s < a.nfull; s += 64) {
+            int dx, dy;
+            decode(a.table[s], dx, dy);
+            const int x = cx + dx, y = cy + dy;
+            if (!usable(x, y)) continue;
+            fsum_l += (double)a.ref[(size_t)y * W + x];
+            ++cnt;
+        }
+        const int ns = wave_sum_i(cnt);
+        if (ns < a.min_needed) {
+            early = true;
+        } else if constexpr (ZNSSD) {
+            const double fmean = wave_sum(fsum_l) / (double)ns;
+            double acc = 0.0;
+            for (int s = lane; s < a.nfull; s += 64) {
+                int dx, dy;
+                decode(a.table[s], dx, dy);
+                const int x = cx + dx, y = cy + dy;
+                if (!usable(x, y)) continue;
+                const double df = (double)a.ref[(size_t)y * W + x] - fmean;
+                acc += df * df;
+            }
+            const double fnorm = sqrt(wave_sum(acc));
+            if (fnorm <= kEps) {
+                early = true;
+                status = DIC_NUMERICAL_FAILURE;
+            }
+            if (lane == 0) {
+                sh.fmean = fmean;
+                sh.fnorm = fnorm;
+            }
+        } else {
+            if (lane == 0) {
+                sh.fmean = 0.0;
+                sh.fnorm = 1.0;
+            }
+        }
+        if (lane == 0) sh.ns = ns;
+    }
+
+    // ---- Hessian H = c * sum sd sd^T (icgn.cpp:532-536), two rows at a time
+    // for P = 12; the one-pass variant also needs sum sd, sum fhat sd,
+    // sum fhat, sum fhat^2 ----
+    if (!early) {
+        __syncthreads();
+        const double fmean = sh.fmean, fnorm = sh.fnorm;
+        const double hscale = ZNSSD ? 2.0 / (fnorm * fnorm) : 1.0;
+        constexpr int RG = (P == 6) ? 6 : 2;
+#pragma unroll
+        for (int r0 = 0; r0 < P; r0 += RG) {
+            Acc<R> acc[RG][P];
+            for (int s = lane; s < a.nfull; s += 64) {
+                int dx, dy;
+                decode(a.table[s], dx, dy);
+                const int x = cx + dx, y = cy + dy;
+                if (!usable(x, y)) continue;
+                R sd[P];
+                steepest_descent<P, R>(gxp[(size_t)y * W + x], gyp[(size_t)y * W + x], (R)dx, (R)dy, sd);
+#pragma unroll
+                for (int i = 0; i < RG; ++i)
+#pragma unroll
+                    for (int j = r0 + i; j < P; ++j) acc[i][j].add(sd[r0 + i] * sd[j]);
+            }
+#pragma unroll
+            for (int i = 0; i < RG; ++i)
+#pragma unroll
+                for (int j = r0 + i; j < P; ++j) {
+                    const double h = wave_sum(acc[i][j].value()) * hscale;
+                    if (lane == 0) {
+                        sh.H[(r0 + i) * P + j] = h;
+                        sh.H[j * P + (r0 + i)] = h;
+                    }
+                }
+        }
+        if constexpr (!TWOPASS && ZNSSD) {
+            Acc<R> a_sd[P], a_fsd[P], a_f, a_f2;
+            const R fm = (R)fmean, fin = (R)(1.0 / fnorm);
+            for (int s = lane; s < a.nfull; s += 64) {
+                int dx, dy;
+                decode(a.table[s], dx, dy);
+                const int x = cx + dx, y = cy + dy;
+                if (!usable(x, y)) continue;
+                R sd[P];
+                steepest_descent<P, R>(gxp[(size_t)y * W + x], gyp[(size_t)y * W + x], (R)dx, (R)dy, sd);
+                const R fh = ((R)a.ref[(size_t)y * W + x] - fm) * fin;
+                a_f.add(fh);
+                a_f2.add(fh * fh);
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    a_sd[j].add(sd[j]);
+                    a_fsd[j].add(fh * sd[j]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                reduce_to(a_sd[j].value(), &sh.ssd[j], lane);
+                reduce_to(a_fsd[j].value(), &sh.sfsd[j], lane);
+            }
+            reduce_to(a_f.value(), &sh.sfh, lane);
+            reduce_to(a_f2.value(), &sh.sfh2, lane);
+        }
+        __syncthreads();
+        if (lane == 0) {
+            int ok, pos;
+            ldlt_factor<P>(sh.H, sh.tr, sh.x, &ok, &pos);
+            sh.state = (ok && pos) ? 0 : 1;
+            const dic_initial ini = a.init[poi];
+#pragma unroll
+            for (int j = 0; j < 12; ++j) sh.p[j] = 0.0;
+            sh.p[0] = ini.u;
+            sh.p[1] = ini.v;
+            sh.p[2] = ini.du_dx;
+            sh.p[3] = ini.du_dy;
+            sh.p[4] = ini.dv_dx;
+            sh.p[5] = ini.dv_dy;
+            sh.corr = __builtin_inf();
+        }
+        __syncthreads();
+        if (sh.state != 0) {
+            early = true;
+            status = DIC_NUMERICAL_FAILURE;
+        }
+    }
+
+    bool solved = false;
+    if (!early) {
+        const double Wm1 = (double)(W - 1), Hm1 = (double)(Hh - 1);
+        const int ns = sh.ns;
+        const double fmean = uniform_f64(sh.fmean), fnorm = uniform_f64(sh.fnorm);
+        const R fm = (R)fmean, fin = (R)(1.0 / fnorm);
+        bool converged = false, failed = false;
+        for (int it = 0; it < a.max_it; ++it) {
+            iterations = it + 1;
+            // current parameters, uniform across the wave
+            double p[12];
+#pragma unroll
+            for (int j = 0; j < 12; ++j) p[j] = (j < P) ? uniform_f64(sh.p[j]) : 0.0;
+            // a translation beyond any image puts every warped sample outside
+            // (and would overflow the integer split below)
+            const bool wild = !(fabs(p[0]) < 1e7 && fabs(p[1]) < 1e7);
+            const double ui = wild ? 0.0 : floor(p[0]), vi = wild ? 0.0 : floor(p[1]);
+            const R uf = (R)(p[0] - ui), vf = (R)(p[1] - vi);
+            const int bx = cx + (int)ui, by = cy + (int)vi;
+            R q[12];
+#pragma unroll
+            for (int j = 2; j < 12; ++j) q[j] = (R)p[j];
+
+            bool oob = wild;
+            Acc<R> s1, s2, sfg, sg[P];
+            for (int s = wild ? a.nfull : lane; s < a.nfull; s += 64) {
+                int dx, dy;
+                decode(a.table[s], dx, dy);
+                const int x = cx + dx, y = cy + dy;
+                if (!usable(x, y)) continue;
+                int ix, iy;
+                R tx, ty;
+                if constexpr (FP64) { // warp + bounds exactly as icgn.cpp:35-40, :559-575
+                    const double lx = (double)dx, ly = (double)dy;
+                    double wx = (double)cx + lx + p[0] + p[2] * lx + p[3] * ly;
+                    double wy = (double)cy + ly + p[1] + p[4] * lx + p[5] * ly;
+                    if constexpr (P == 12) {
+                        const double dx22 = lx * lx * 0.5, dy22 = ly * ly * 0.5, dxy = lx * ly;
+                        wx += p[6] * dx22 + p[7] * dxy + p[8] * dy22;
+                        wy += p[9] * dx22 + p[10] * dxy + p[11] * dy22;
+                    }
+                    if (!(wx >= 0.0 && wy >= 0.0 && wx <= Wm1 && wy <= Hm1)) {
+                        oob = true;
+                        continue;
+                    }
+                    ix = min(max((int)floor(wx), 0), W - 1);
+                    iy = min(max((int)floor(wy), 0), Hh - 1);
+                    tx = (R)(wx - (double)ix);
+                    ty = (R)(wy - (double)iy);
+                } else { // FP32 offsets about the integer translation
+                    const R lx = (R)dx, ly = (R)dy;
+                    R xl = lx + uf + q[2] * lx + q[3] * ly;
+                    R yl = ly + vf + q[4] * lx + q[5] * ly;
+                    if constexpr (P == 12) {
+                        const R dx22 = lx * lx * R(0.5), dy22 = ly * ly * R(0.5), dxy = lx * ly;
+                        xl += q[6] * dx22 + q[7] * dxy + q[8] * dy22;
+                        yl += q[9] * dx22 + q[10] * dxy + q[11] * dy22;
+                    }
+                    // clamp before the int conversion; anything clamped is out of bounds
+                    const R fx = fmin(fmax(floor(xl), R(-1e7)), R(1e7));
+                    const R fy = fmin(fmax(floor(yl), R(-1e7)), R(1e7));
+                    ix = bx + (int)fx;
+                    iy = by + (int)fy;
+                    tx = xl - fx;
+                    ty = yl - fy;
+                    // x' = ix + tx must lie in [0, W-1] (tx in [0, 1))
+                    if (ix < 0 || iy < 0 || ix > W - 1 || iy > Hh - 1 || (ix == W - 1 && tx > R(0)) ||
+                        (iy == Hh - 1 && ty > R(0))) {
+                        oob = true;
+                        continue;
+                    }
+                }
+                const R g = sample_at<D, R, R>(cf, a.pitch, a.b, ix, iy, tx, ty);
+                if constexpr (TWOPASS) {
+                    gbuf[s] = g;
+                    s1.add(g);
+                } else {
+                    R sd[P];
+                    steepest_descent<P, R>(gxp[(size_t)y * W + x], gyp[(size_t)y * W + x], (R)dx, (R)dy, sd);
+                    if constexpr (ZNSSD) {
+                        const R fh = ((R)a.ref[(size_t)y * W + x] - fm) * fin;
+                        const R d = g - fm;
+                        s1.add(d);
+                        s2.add(d * d);
+                        sfg.add(fh * d);
+#pragma unroll
+                        for (int j = 0; j < P; ++j) sg[j].add(d * sd[j]);
+                    } else { // SSD residual e = f - g, icgn.cpp:945-950
+                        const R e = (R)a.ref[(size_t)y * W + x] - g;
+                        s2.add(e * e);
+#pragma unroll
+                        for (int j = 0; j < P; ++j) sg[j].add(e * sd[j]);
+                    }
+                }
+            }
+            if (__any(oob)) {
+                status = DIC_INVALID_INPUT;
+                failed = true;
+                break;
+            }
+            if constexpr (TWOPASS) {
+                // two-pass mean / norm / residual, icgn.cpp:577-592
+                const double gmean = wave_sum(s1.value()) / (double)ns;
+                double acc = 0.0;
+                for (int s = lane; s < a.nfull; s += 64) {
+                    if constexpr (MASKED) {
+                        int dx, dy;
+                        decode(a.table[s], dx, dy);
+                        if (!usable(cx + dx, cy + dy)) continue;
+                    }
+                    const double dg = gbuf[s] - gmean;
+                    acc += dg * dg;
+                }
+                const double gnorm = sqrt(wave_sum(acc));
+                if (gnorm <= kEps) {
+                    status = DIC_NUMERICAL_FAILURE;
+                    failed = true;
+                    break;
+                }
+                double gl[P];
+#pragma unroll
+                for (int j = 0; j < P; ++j) gl[j] = 0.0;
+                double cl = 0.0;
+                for (int s = lane; s < a.nfull; s += 64) {
+                    int dx, dy;
+                    decode(a.table[s], dx, dy);
+                    const int x = cx + dx, y = cy + dy;
+                    if (!usable(x, y)) continue;
+                    double sd[P];
+                    steepest_descent<P, double>((double)gxp[(size_t)y * W + x], (double)gyp[(size_t)y * W + x],
+                                                (double)dx, (double)dy, sd);
+                    const double fh = ((double)a.ref[(size_t)y * W + x] - fmean) / fnorm;
+                    const double e = fh - (gbuf[s] - gmean) / gnorm;
+#pragma unroll
+                    for (int j = 0; j < P; ++j) gl[j] += e * sd[j];
+                    cl += e * e;
+                }
+#pragma unroll
+                for (int j = 0; j < P; ++j) reduce_to(gl[j], &sh.red[j], lane);
+                reduce_to(cl, &sh.red[P], lane);
+            } else if constexpr (ZNSSD) {
+                reduce_to(s1.value(), &sh.red[0], lane);
+                reduce_to(s2.value(), &sh.red[1], lane);
+                reduce_to(sfg.value(), &sh.red[2], lane);
+#pragma unroll
+                for (int j = 0; j < P; ++j) reduce_to(sg[j].value(), &sh.red[3 + j], lane);
+            } else {
+                reduce_to(s2.value(), &sh.red[0], lane);
+#pragma unroll
+                for (int j = 0; j < P; ++j) reduce_to(sg[j].value(), &sh.red[1 + j], lane);
+            }
+            if (lane == 0) lane0_step<P, OBJ, TWOPASS>(sh, a.thr);
+            __syncthreads();
+            const int st = sh.state;
+            __syncthreads();
+            if (st == 2) {
+                status = DIC_NUMERICAL_FAILURE;
+                failed = true;
+                break;
+            }
+            if (st == 1) {
+                converged = true;
+                break;
+            }
+        }
+        if (!failed) {
+            // running out of iterations still reports Success (icgn.cpp:620-622)
+            if (!converged && isfinite(sh.corr)) converged = true;
+            status = converged ? DIC_SUCCESS : DIC_NOT_CONVERGED;
+            solved = true;
+        }
+    }
+
+    if (lane == 0) {
+        const dic_initial ini = a.init[poi];
+        dic_result r;
+        r.x = px;
+        r.y = py;
+#pragma unroll
+        for (int j = 0; j < 12; ++j) r.p[j] = 0.0;
+        if (solved) {
+#pragma unroll
+            for (int j = 0; j < P; ++j) r.p[j] = sh.p[j];
+            r.correlation = sh.corr;
+        } else { // failure exits keep the initial u, v (icgn.cpp:455-462)
+            r.p[0] = ini.u;
+            r.p[1] = ini.v;
+            r.correlation = ini.confidence;
+        }
+        r.status = status;
+        r.valid = (status == DIC_SUCCESS) ? 1 : 0;
+        r.iterations = iterations;
+        r.reserved = 0;
+        a.out[poi] = r;
+    }
+}
+
+template <int P, int OBJ, int D, typename R, bool MASKED>
+hipError_t launch_variant(const IcgnArgs &args, hipStream_t s)
+{
+    constexpr bool TWOPASS = OBJ == DIC_OBJECTIVE_ZNSSD && sizeof(R) == 8;
+    const size_t lds = TWOPASS ? sizeof(double) * (size_t)args.nfull : 0;
+    auto kern = icgn_kernel<P, OBJ, D, R, MASKED>;
+    if (lds > 48 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(args.n), dim3(64), lds, s, args);
+    return hipGetLastError();
+}
+
+template <int P, int OBJ, int D, typename R>
+hipError_t dispatch_masked(const IcgnArgs &args, bool masked, hipStream_t s)
+{
+    return masked ? launch_variant<P, OBJ, D, R, true>(args, s) : launch_variant<P, OBJ, D, R, false>(args, s);
+}
+
+template <int P, int OBJ, int D>
+hipError_t dispatch_precision(const IcgnArgs &args, int precision, bool masked, hipStream_t s)
+{
+    return precision == DIC_PRECISION_FP64 ? dispatch_masked<P, OBJ, D, double>(args, masked, s)
+                                           : dispatch_masked<P, OBJ, D, float>(args, masked, s);
+}
+
+template <int P, int OBJ>
+hipError_t dispatch_degree(const IcgnArgs &args, int degree, int precision, bool masked, hipStream_t s)
+{
+    switch (degree) {
+    case 1: return dispatch_precision<P, OBJ, 1>(args, precision, masked, s);
+    case 3: return dispatch_precision<P, OBJ, 3>(args, precision, masked, s);
+    case 5: return dispatch_precision<P, OBJ, 5>(args, precision, masked, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace
+
+hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    const dic_icgn_params &pr = *L.params;
+    const int P = pr.shape_order == 2 ? 12 : 6;
+    IcgnArgs a;
+    a.W = L.width;
+    a.H = L.height;
+    a.pitch = L.pitch;
+    a.b = L.border;
+    a.ref = L.ref;
+    a.gx = L.gx;
+    a.gy = L.gy;
+    a.coeffs = L.coeffs;
+    a.roi = L.roi;
+    a.table = L.table;
+    a.nfull = L.table_count;
+    a.radius = pr.subset_radius;
+    a.max_it = pr.max_iterations;
+    a.thr = pr.convergence_threshold;
+    int needed = P;
+    if (pr.masked) {
+        int ms = (int)std::ceil(pr.min_valid_sample_ratio * (double)L.table_count);
+        if (pr.min_valid_samples > ms) ms = pr.min_valid_samples;
+        if (ms > needed) needed = ms;
+    }
+    a.min_needed = needed;
+    a.xy = L.poi_xy;
+    a.init = L.init;
+    a.out = L.out;
+    a.n = L.n;
+    const bool masked = pr.masked != 0;
+    const bool ssd = pr.objective == DIC_OBJECTIVE_SSD;
+    if (P == 6)
+        return ssd ? dispatch_degree<6, DIC_OBJECTIVE_SSD>(a, L.degree, L.precision, masked, s)
+                   : dispatch_degree<6, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, s);
+    return ssd ? dispatch_degree<12, DIC_OBJECTIVE_SSD>(a, L.degree, L.precision, masked, s)
+               : dispatch_degree<12, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, s);
+}
+
+} // namespace dic

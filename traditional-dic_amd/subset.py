@@ -1,0 +1,251 @@
+"""Host-side mirror of Traditional-DIC's subset solver surface on the HIP library.
+
+Reference interfaces mirrored (paths inside the Traditional-DIC tree):
+  SubsetConfig                  include/dic/subset/subset_config.hpp:33-50
+  config_from_dict              bindings/python/bind_subset.cpp:158-248
+  ICGNSolver::solve / solve_with_interpolators / solve_with_mask
+                                src/subset/solver/icgn.cpp:363-432
+  results_to_dict keys          bindings/python/bind_subset.cpp:102-153
+  POI grid + margin defaults    src/subset/seed/reliability_propagation.cpp:154-176, :318-323
+
+The numerics run in libdic_hip.so; nothing here computes a correlation.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import INITIAL_DTYPE, RESULT_DTYPE, IcgnParams
+
+
+@dataclass
+class BSplinePrecomputeConfig:
+    """include/dic/interpolation/bspline.hpp:42-67 (defaults: quintic, border 3, exact prefilter)."""
+    degree: int = 5
+    border: int = 3
+    use_exact_prefilter: bool = True
+
+
+@dataclass
+class SubsetConfig:
+    """Fields of dic::SubsetConfig that reach the solver (defaults as in subset_config.hpp:33-50)."""
+    subset_radius: int = 15
+    max_iterations: int = 30
+    convergence_threshold: float = 1e-3
+    use_second_order: bool = False
+    propagation_spacing: int = 5
+    propagation_max_znssd: float = 0.5
+    truncate_roi_subsets: bool = False
+    min_valid_sample_ratio: float = 0.5
+    min_valid_samples: int = 12
+    image_precompute: BSplinePrecomputeConfig = field(default_factory=BSplinePrecomputeConfig)
+    shape_function: str = "first_order"   # first_order | second_order
+    optimizer: str = "icgn"               # icgn | forward_gauss_newton
+    objective: str = "znssd"              # znssd | ssd
+
+    @property
+    def shape_order(self) -> int:
+        return 2 if (self.shape_function == "second_order" or self.use_second_order) else 1
+
+
+def config_from_dict(d: dict | None) -> SubsetConfig:
+    """Restates config_from_dict (bind_subset.cpp:158-248) for the fields on this path,
+    with the same validation messages.  Interpolation degree maps only 1 and 3
+    (any other value keeps the quintic default, :195-200)."""
+    cfg = SubsetConfig()
+    if d is None:
+        return cfg
+    if not isinstance(d, dict):
+        raise RuntimeError("config must be a dict (from YAML) or None.")
+    s = d.get("subset")
+    if isinstance(s, dict) and "radius" in s:
+        cfg.subset_radius = int(s["radius"])
+    o = d.get("optimization")
+    if isinstance(o, dict):
+        cfg.max_iterations = int(o.get("max_iterations", cfg.max_iterations))
+        cfg.convergence_threshold = float(o.get("convergence_threshold", cfg.convergence_threshold))
+        if "method" in o:
+            m = str(o["method"])
+            if m == "icgn":
+                cfg.optimizer = "icgn"
+            elif m in ("forward_gauss_newton", "fgn"):
+                cfg.optimizer = "forward_gauss_newton"
+            else:
+                raise RuntimeError("subset optimization.method must be 'icgn', 'fgn', or 'forward_gauss_newton'.")
+    c = d.get("correlation")
+    if isinstance(c, dict) and "criterion" in c:
+        crit = str(c["criterion"])
+        if crit not in ("ssd", "znssd"):
+            raise RuntimeError("subset correlation.criterion must be 'ssd' or 'znssd'.")
+        cfg.objective = crit
+    it = d.get("interpolation")
+    if isinstance(it, dict) and "degree" in it:
+        deg = int(it["degree"])
+        if deg in (1, 3):
+            cfg.image_precompute.degree = deg
+    rp = d.get("reliability_propagation")
+    if isinstance(rp, dict):
+        cfg.propagation_spacing = int(rp.get("spacing", cfg.propagation_spacing))
+        cfg.propagation_max_znssd = float(rp.get("max_znssd", cfg.propagation_max_znssd))
+    sf = d.get("shape_function")
+    if isinstance(sf, dict) and "order" in sf:
+        order = str(sf["order"])
+        if order in ("first_order", "1"):
+            cfg.shape_function = "first_order"
+        elif order in ("second_order", "2"):
+            cfg.shape_function = "second_order"
+        else:
+            raise RuntimeError("subset shape_function.order must be 'first_order'/'1' or 'second_order'/'2'.")
+    return cfg
+
+
+def make_params(cfg: SubsetConfig, precision: int, masked: bool = False) -> IcgnParams:
+    p = IcgnParams()
+    p.subset_radius = int(cfg.subset_radius)
+    p.max_iterations = int(cfg.max_iterations)
+    p.convergence_threshold = float(cfg.convergence_threshold)
+    p.shape_order = cfg.shape_order
+    p.objective = _lib.OBJECTIVE_SSD if cfg.objective == "ssd" else _lib.OBJECTIVE_ZNSSD
+    p.optimizer = _lib.OPTIMIZER_FGN if cfg.optimizer == "forward_gauss_newton" else _lib.OPTIMIZER_ICGN
+    p.precision = int(precision)
+    p.masked = 1 if masked else 0
+    p.min_valid_samples = int(cfg.min_valid_samples)
+    p.min_valid_sample_ratio = float(cfg.min_valid_sample_ratio)
+    return p
+
+
+def initial_guesses(u, v, du_dx=0.0, du_dy=0.0, dv_dx=0.0, dv_dy=0.0, confidence=0.0) -> np.ndarray:
+    """Array of InitialDisplacement records (initializer.hpp:33-44)."""
+    u = np.atleast_1d(np.asarray(u, np.float64))
+    out = np.zeros(u.shape[0], INITIAL_DTYPE)
+    out["u"] = u
+    out["v"] = v
+    out["du_dx"] = du_dx
+    out["du_dy"] = du_dy
+    out["dv_dx"] = dv_dx
+    out["dv_dy"] = dv_dy
+    out["confidence"] = confidence
+    return out
+
+
+class PreparedPair:
+    """Device-resident state of one (reference, deformed) pair: the float32
+    images, the reference gradient planes and the deformed image's B-spline
+    coefficients (the reference's gradient cache + BSplinePrecomputedImage)."""
+
+    def __init__(self, ctx: _lib.Context, reference: np.ndarray, deformed: np.ndarray, cfg: SubsetConfig,
+                 precision: int, gradient_mode: int = _lib.GRADIENT_CENTRAL):
+        ref = np.ascontiguousarray(reference, np.float32)
+        dfm = np.ascontiguousarray(deformed, np.float32)
+        self.ctx = ctx
+        self.width, self.height = ref.shape[1], ref.shape[0]
+        self.precision = precision
+        self.ref = ctx.upload_image(ref)
+        self.deformed = ctx.upload_image(dfm)
+        ip = cfg.image_precompute
+        self.grad = ctx.ref_gradient(self.ref, gradient_mode, ip.degree, ip.border, precision)
+        self.coeffs = ctx.prefilter(self.deformed, ip.degree, ip.border, ip.use_exact_prefilter, precision)
+
+    def free(self):
+        for h in (self.coeffs, self.grad, self.deformed, self.ref):
+            h.free()
+
+
+class ICGNSolver:
+    """IC-GN on the GPU behind the reference's ICGNSolver call pattern.
+
+    precision: "fp64" (plain drop-in, the reference's arithmetic) or "mixed"
+    (FP32 sampling and compensated sums, FP64 solve/update).
+    gradient: "central" (pipeline convention, reliability_propagation.cpp:84-112)
+    or "bspline" (standalone solve(), bspline.cpp:289-294)."""
+
+    def __init__(self, config: SubsetConfig | dict | None = None, precision: str = "fp64",
+                 gradient: str = "central", device: int = 0, context: _lib.Context | None = None):
+        self.config = config if isinstance(config, SubsetConfig) else config_from_dict(config)
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = _lib.PRECISION_FP64 if precision == "fp64" else _lib.PRECISION_MIXED
+        self.gradient_mode = _lib.GRADIENT_CENTRAL if gradient == "central" else _lib.GRADIENT_BSPLINE_HALF
+        self.ctx = context if context is not None else _lib.Context(device)
+        self._pair = None
+        self._pair_key = None
+
+    def prepare(self, reference: np.ndarray, deformed: np.ndarray) -> PreparedPair:
+        key = (id(reference), id(deformed))
+        if self._pair is None or self._pair_key != key:
+            if self._pair is not None:
+                self._pair.free()
+            self._pair = PreparedPair(self.ctx, reference, deformed, self.config, self.precision, self.gradient_mode)
+            self._pair_key = key
+        return self._pair
+
+    def solve_batch(self, reference, deformed, points, initial, roi=None) -> np.ndarray:
+        """Many POIs at once; returns RESULT_DTYPE records (Displacement2D + iterations).
+        With roi (and config.truncate_roi_subsets) the masked variants run (solve_with_mask)."""
+        if reference.shape != deformed.shape:
+            # icgn.cpp:464-470: every POI reports InvalidInput with the initial u, v
+            xy = np.asarray(points, np.float64).reshape(-1, 2)
+            ini = np.ascontiguousarray(initial, INITIAL_DTYPE)
+            out = np.zeros(xy.shape[0], RESULT_DTYPE)
+            out["x"], out["y"] = xy[:, 0], xy[:, 1]
+            out["p"][:, 0], out["p"][:, 1] = ini["u"], ini["v"]
+            out["correlation"] = ini["confidence"]
+            out["status"] = _lib.INVALID_INPUT
+            return out
+        pair = self.prepare(reference, deformed)
+        masked = roi is not None
+        params = make_params(self.config, self.precision, masked)
+        return self.ctx.solve(params, pair.ref, pair.grad, pair.coeffs, points, initial,
+                              None if roi is None else np.asarray(roi, np.uint8))
+
+    def solve(self, reference, deformed, point, initial) -> np.void:
+        return self.solve_batch(reference, deformed, np.asarray(point, np.float64).reshape(1, 2),
+                                np.atleast_1d(np.asarray(initial, INITIAL_DTYPE)))[0]
+
+    solve_with_interpolators = solve
+
+    def solve_with_mask(self, reference, deformed, roi, point, initial) -> np.void:
+        return self.solve_batch(reference, deformed, np.asarray(point, np.float64).reshape(1, 2),
+                                np.atleast_1d(np.asarray(initial, INITIAL_DTYPE)), roi=roi)[0]
+
+
+def results_to_dict(results: np.ndarray) -> dict:
+    """Same keys as bind_subset.cpp:102-153 (first-order parameters only)."""
+    return {
+        "x": results["x"].copy(), "y": results["y"].copy(),
+        "u": results["p"][:, 0].copy(), "v": results["p"][:, 1].copy(),
+        "du_dx": results["p"][:, 2].copy(), "du_dy": results["p"][:, 3].copy(),
+        "dv_dx": results["p"][:, 4].copy(), "dv_dy": results["p"][:, 5].copy(),
+        "correlation": results["correlation"].copy(), "valid": results["valid"].astype(bool),
+    }
+
+
+def compute_grid(reference: np.ndarray, deformed: np.ndarray, initial_u: np.ndarray, initial_v: np.ndarray,
+                 config: dict | SubsetConfig | None = None, precision: str = "fp64",
+                 solver: ICGNSolver | None = None) -> dict:
+    """Grid solve with caller-supplied initial guesses on the propagation grid
+    (step = spacing + 1, row-major, reliability_propagation.cpp:154-176).
+    POIs whose subset crosses the image margin keep the reference defaults
+    (valid = False, u = v = 0; :318-323).  initial_u / initial_v have one
+    entry per grid point."""
+    solver = solver or ICGNSolver(config, precision)
+    cfg = solver.config
+    H, W = reference.shape
+    step = cfg.propagation_spacing + 1
+    gw = (W + step - 1) // step
+    gh = (H + step - 1) // step
+    i = np.arange(gw * gh)
+    xy = np.stack([(i % gw) * step, (i // gw) * step], axis=1).astype(np.float64)
+    r = cfg.subset_radius
+    inside = (xy[:, 0] - r >= 0) & (xy[:, 1] - r >= 0) & (xy[:, 0] + r < W) & (xy[:, 1] + r < H)
+    res = np.zeros(xy.shape[0], RESULT_DTYPE)
+    res["x"], res["y"] = xy[:, 0], xy[:, 1]
+    res["status"] = _lib.INVALID_INPUT
+    idx = np.nonzero(inside)[0]
+    if idx.size:
+        ini = initial_guesses(np.asarray(initial_u, np.float64).ravel()[idx],
+                              np.asarray(initial_v, np.float64).ravel()[idx])
+        res[idx] = solver.solve_batch(reference, deformed, xy[idx], ini)
+    return results_to_dict(res)

@@ -1,0 +1,213 @@
+"""Seeded synthetic speckle workloads (ctypes over libdic_synth.so).
+
+Images follow I(X) = 30 + sum_k A_k exp(-|X - c_k|^2 / s_k^2) with
+c_k ~ U(-8, W+8) x U(-8, H+8), s_k ~ U(2, 4), A_k ~ U(60, 200) from a PCG32
+stream; deformed frames are the analytic render I(X(x)) under a known field.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import SYNTH_PATH
+
+SYNTH_IDENTITY, SYNTH_AFFINE, SYNTH_SINUSOID, SYNTH_AFFINE_SINUSOID = 0, 1, 2, 3
+
+
+class _Speckle(ctypes.Structure):
+    _fields_ = [("nblobs", ctypes.c_int32), ("reserved", ctypes.c_int32), ("seed", ctypes.c_uint64),
+                ("background", ctypes.c_double), ("smin", ctypes.c_double), ("smax", ctypes.c_double),
+                ("amin", ctypes.c_double), ("amax", ctypes.c_double)]
+
+
+class _Field(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in ("u0", "v0", "ux", "uy", "vx", "vy", "xc", "yc", "au", "lu", "av", "lv")]
+
+
+_slib = None
+
+
+def _load():
+    global _slib
+    if _slib is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise ImportError(f"{SYNTH_PATH} is missing: build it with `make -C traditional-dic_amd`")
+        lib = ctypes.CDLL(SYNTH_PATH)
+        lib.synth_render.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Speckle), ctypes.POINTER(_Field),
+                                     ctypes.c_void_p]
+        lib.synth_render.restype = ctypes.c_int
+        lib.synth_add_noise.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_uint64,
+                                        ctypes.c_uint64]
+        lib.synth_quantize_unit.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        lib.synth_pcg32_first.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        lib.synth_pcg32_first.restype = ctypes.c_uint32
+        _slib = lib
+    return _slib
+
+
+@dataclass
+class Field:
+    """Displacement field of the deformed frame (see synth.h)."""
+    kind: int = SYNTH_IDENTITY
+    u0: float = 0.0
+    v0: float = 0.0
+    ux: float = 0.0
+    uy: float = 0.0
+    vx: float = 0.0
+    vy: float = 0.0
+    xc: float = 0.0
+    yc: float = 0.0
+    au: float = 0.0
+    lu: float = 1.0
+    av: float = 0.0
+    lv: float = 1.0
+
+    def _c(self) -> _Field:
+        f = _Field()
+        f.kind = self.kind
+        for n in ("u0", "v0", "ux", "uy", "vx", "vy", "xc", "yc", "au", "lu", "av", "lv"):
+            setattr(f, n, float(getattr(self, n)))
+        return f
+
+    def displacement(self, X, Y):
+        """u, v and [du/dx, du/dy, dv/dx, dv/dy] at reference points (arrays);
+        the same closed forms as synth_displacement() in synth.c."""
+        X = np.asarray(X, np.float64).ravel()
+        Y = np.asarray(Y, np.float64).ravel()
+        g = np.zeros((X.size, 4))
+        u = np.zeros_like(X)
+        v = np.zeros_like(X)
+        if self.kind in (SYNTH_AFFINE, SYNTH_AFFINE_SINUSOID):
+            u = self.u0 + self.ux * (X - self.xc) + self.uy * (Y - self.yc)
+            v = self.v0 + self.vx * (X - self.xc) + self.vy * (Y - self.yc)
+            g[:] = [self.ux, self.uy, self.vx, self.vy]
+        if self.kind == SYNTH_SINUSOID:
+            k1, k2 = 2 * np.pi / self.ux, 2 * np.pi / self.vy
+            u = self.u0 * np.sin(k1 * X)
+            v = self.v0 * np.cos(k2 * Y)
+            g[:, 0] = self.u0 * k1 * np.cos(k1 * X)
+            g[:, 3] = -self.v0 * k2 * np.sin(k2 * Y)
+        if self.kind == SYNTH_AFFINE_SINUSOID:
+            k1, k2 = 2 * np.pi / self.lu, 2 * np.pi / self.lv
+            u = u + self.au * np.sin(k1 * X)
+            v = v + self.av * np.cos(k2 * Y)
+            g[:, 0] += self.au * k1 * np.cos(k1 * X)
+            g[:, 3] -= self.av * k2 * np.sin(k2 * Y)
+        return u, v, g
+
+
+def affine(u0, v0, ux=0.0, uy=0.0, vx=0.0, vy=0.0, xc=0.0, yc=0.0) -> Field:
+    return Field(SYNTH_AFFINE, u0, v0, ux, uy, vx, vy, xc, yc)
+
+
+def sinusoid(amp_u, wavelength_u, amp_v, wavelength_v) -> Field:
+    # u = amp_u sin(2 pi X / wavelength_u), v = amp_v cos(2 pi Y / wavelength_v)
+    return Field(SYNTH_SINUSOID, u0=amp_u, ux=wavelength_u, v0=amp_v, vy=wavelength_v)
+
+
+def render(width: int, height: int, nblobs: int, seed: int, field: Field | None = None,
+           background=30.0, smin=2.0, smax=4.0, amin=60.0, amax=200.0) -> np.ndarray:
+    lib = _load()
+    sp = _Speckle(int(nblobs), 0, int(seed), background, smin, smax, amin, amax)
+    out = np.empty((height, width), np.float32)
+    f = field._c() if field is not None else None
+    rc = lib.synth_render(int(width), int(height), ctypes.byref(sp), ctypes.byref(f) if f is not None else None,
+                          out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"synth_render failed ({rc})")
+    return out
+
+
+def add_noise(img: np.ndarray, sigma: float, seed: int, stream: int) -> np.ndarray:
+    a = np.ascontiguousarray(img, np.float32).copy()
+    _load().synth_add_noise(a.ctypes.data, a.size, float(sigma), int(seed), int(stream))
+    return a
+
+
+def quantize_unit(img: np.ndarray) -> np.ndarray:
+    """Clip to [0, 255], round to 8 bit, divide by the maximum (float32)."""
+    a = np.ascontiguousarray(img, np.float32).copy()
+    _load().synth_quantize_unit(a.ctypes.data, a.size)
+    return a
+
+
+def pcg32_first(seed: int, stream: int) -> int:
+    return int(_load().synth_pcg32_first(int(seed), int(stream)))
+
+
+def poi_grid(width: int, height: int, step: int):
+    """POI grid of ReliabilityPropagation::propagate (reliability_propagation.cpp:154-176):
+    step = spacing + 1, grid_w = ceil(W / step), row-major, x = (i % grid_w) * step."""
+    gw = (width + step - 1) // step
+    gh = (height + step - 1) // step
+    i = np.arange(gw * gh)
+    return np.stack([(i % gw) * step, (i // gw) * step], axis=1).astype(np.float64)
+
+
+def solvable_mask(xy: np.ndarray, width: int, height: int, radius: int) -> np.ndarray:
+    """POIs whose circular subset stays inside the image (icgn.cpp:475-479)."""
+    cx = np.round(xy[:, 0]).astype(np.int64)
+    cy = np.round(xy[:, 1]).astype(np.int64)
+    return (cx - radius >= 0) & (cy - radius >= 0) & (cx + radius < width) & (cy + radius < height)
+
+
+# ---------------------------------------------------------------------------
+# Named workloads (the correlation configurations the benchmarks and tests use)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Workload:
+    name: str
+    reference: np.ndarray
+    deformed: np.ndarray
+    field: Field
+    poi_xy: np.ndarray
+    init: np.ndarray          # structured INITIAL_DTYPE
+    radius: int
+    shape_order: int
+    degree: int
+    max_iterations: int = 30
+    threshold: float = 1e-3
+
+    def truth(self):
+        return self.field.displacement(self.poi_xy[:, 0], self.poi_xy[:, 1])
+
+
+def _integer_init(field: Field, xy: np.ndarray):
+    from ._lib import INITIAL_DTYPE
+    u, v, _ = field.displacement(xy[:, 0], xy[:, 1])
+    ini = np.zeros(xy.shape[0], INITIAL_DTYPE)
+    ini["u"] = np.round(u)
+    ini["v"] = np.round(v)
+    return ini
+
+
+def plain_512(seed: int = 1) -> Workload:
+    """512^2 8-bit speckle (4000 blobs), affine u = 0.35 + 1e-3 (X - 255.5),
+    v = -0.20 - 5e-4 (Y - 255.5); POI step 10, radius 15, first order,
+    bicubic, ZNSSD, ||dp|| < 1e-3 or 30 iterations, init = integer truth."""
+    W = H = 512
+    f = affine(0.35, -0.20, 1e-3, 0.0, 0.0, -5e-4, 255.5, 255.5)
+    ref = quantize_unit(render(W, H, 4000, seed))
+    dfm = quantize_unit(render(W, H, 4000, seed, f))
+    xy = poi_grid(W, H, 10)
+    xy = xy[solvable_mask(xy, W, H, 15)]
+    return Workload("plain_512_r15_order1_cubic", ref, dfm, f, xy, _integer_init(f, xy), 15, 1, 3)
+
+
+def large_field_2048(seed: int = 2, step: int = 5, radius: int = 20, degree: int = 5) -> Workload:
+    """2048^2 float32 speckle (64k blobs), u = 0.8 sin(2 pi X / 256),
+    v = 0.5 cos(2 pi Y / 320), Gaussian noise sigma = 2 grey levels on both
+    frames; POI step 5, radius 20, second order, init = integer truth."""
+    W = H = 2048
+    f = sinusoid(0.8, 256.0, 0.5, 320.0)
+    ref = add_noise(render(W, H, 65536, seed), 2.0, seed, 2 * seed)
+    dfm = add_noise(render(W, H, 65536, seed, f), 2.0, seed, 2 * seed + 1)
+    xy = poi_grid(W, H, step)
+    xy = xy[solvable_mask(xy, W, H, radius)]
+    return Workload(f"large_field_2048_r{radius}_order2_deg{degree}", ref, dfm, f, xy, _integer_init(f, xy), radius,
+                    2, degree)
