@@ -123,6 +123,8 @@ void dic_image_free(dic_image *img);
  *   used by standalone ICGNSolver::solve, icgn.cpp:441-443). */
 int dic_ref_gradient(dic_ctx *ctx, const dic_image *ref, int mode, int degree, int border, int precision,
                      dic_grad **out);
+/* Recompute central-difference gradients of ref into an existing handle of the same size. */
+int dic_ref_gradient_into(dic_ctx *ctx, const dic_image *ref, dic_grad *out);
 int dic_grad_download(dic_ctx *ctx, const dic_grad *g, double *gx_host, double *gy_host);
 void dic_grad_free(dic_grad *g);
 
@@ -132,6 +134,11 @@ void dic_grad_free(dic_grad *g);
  * precision selects FP64 or FP32 storage (arithmetic is FP64 either way). ---- */
 int dic_bspline_prefilter(dic_ctx *ctx, const dic_image *img, int degree, int border, int exact, int precision,
                           dic_coeffs **out);
+/* Recompute the coefficients of img into an existing handle of the same size
+ * (its degree, border, exact flag and precision); no allocation after the
+ * first call on a context.  New boundary: lets a resident handle follow a
+ * sequence of deformed frames. */
+int dic_bspline_prefilter_into(dic_ctx *ctx, const dic_image *img, dic_coeffs *out);
 int dic_coeffs_info(const dic_coeffs *c, int *width, int *height, int *degree, int *border, int *precision);
 /* Copies the padded coefficient image to host as FP64, row-major (H+2b) x (W+2b). */
 int dic_coeffs_download(dic_ctx *ctx, const dic_coeffs *c, double *host);

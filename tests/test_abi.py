@@ -49,3 +49,9 @@ def test_config_from_dict_mirrors_binding(pkg):
         pkg.config_from_dict({"correlation": {"criterion": "zncc"}})
     with pytest.raises(RuntimeError, match="order"):
         pkg.config_from_dict({"shape_function": {"order": 3}})
+
+
+def test_cpp_host_layer_builds():
+    """include/dic_hip.hpp compiles with plain g++ into the C++ example."""
+    exe = os.path.join(ROOT, "traditional-dic_amd", "solve_pair")
+    assert os.path.exists(exe) and os.access(exe, os.X_OK)

@@ -266,4 +266,18 @@ def test_large_field_2048_fp64_and_mixed(pkg, ctx, orc, synth):
     rms = np.sqrt(np.mean((fp64["p"][ok, 0] - u[ok]) ** 2 + (fp64["p"][ok, 1] - v[ok]) ** 2))
     _report(f"large_field_2048: ok={ok.sum()}/{n} fp64-vs-oracle du={du:.2e} dv={dv:.2e} "
             f"mixed-vs-fp64 max={em:.2e} rms-vs-truth={rms:.3e}")
-    assert ok.mean() > 0.999 and em <= 1e-3 and rms < 0.05
+    # border POIs whose warped subset leaves the image exit InvalidInput, as in the reference
+    assert set(np.unique(fp64["status"])) <= {0, 2}
+    assert ok.mean() > 0.995 and em <= 1e-3 and rms < 0.05
+
+
+def test_cpp_host_layer_solves(pkg):
+    """The C++ mirror of ICGNSolver / BSplineInterpolator (include/dic_hip.hpp) end to end."""
+    import os
+    import subprocess
+
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "traditional-dic_amd", "solve_pair")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    _report(r.stdout.strip())
+    assert r.returncode == 0, r.stdout + r.stderr

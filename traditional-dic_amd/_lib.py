@@ -38,6 +38,7 @@ EXPORTED_SYMBOLS = (
     "dic_coeffs_download", "dic_coeffs_free", "dic_bspline_sample", "dic_icgn_solve_batch",
     "dic_icgn_solve_batch_device", "dic_buffer_alloc", "dic_buffer_free", "dic_buffer_upload",
     "dic_buffer_download", "dic_event_record", "dic_event_elapsed_ms", "dic_event_destroy",
+    "dic_bspline_prefilter_into", "dic_ref_gradient_into",
 )
 
 # Displacement2D + iterations (dic_result); InitialDisplacement (dic_initial)
@@ -94,6 +95,8 @@ _SIGNATURES = {
     "dic_grad_download": ([_vp, _vp, _vp, _vp], _i32),
     "dic_grad_free": ([_vp], None),
     "dic_bspline_prefilter": ([_vp, _vp, _i32, _i32, _i32, _i32, ctypes.POINTER(_vp)], _i32),
+    "dic_bspline_prefilter_into": ([_vp, _vp, _vp], _i32),
+    "dic_ref_gradient_into": ([_vp, _vp, _vp], _i32),
     "dic_coeffs_info": ([_vp] + [ctypes.POINTER(ctypes.c_int)] * 5, _i32),
     "dic_coeffs_download": ([_vp, _vp, _vp], _i32),
     "dic_coeffs_free": ([_vp], None),
@@ -234,6 +237,12 @@ class Context:
         self.check(self.lib.dic_bspline_prefilter(self.handle, img.handle, degree, border, int(exact), precision,
                                                   ctypes.byref(h)))
         return Coefficients(self, h)
+
+    def prefilter_into(self, img: "DeviceImage", coeffs: "Coefficients"):
+        self.check(self.lib.dic_bspline_prefilter_into(self.handle, img.handle, coeffs.handle))
+
+    def ref_gradient_into(self, img: "DeviceImage", grad: "Gradient"):
+        self.check(self.lib.dic_ref_gradient_into(self.handle, img.handle, grad.handle))
 
     def ref_gradient(self, img: "DeviceImage", mode: int = GRADIENT_CENTRAL, degree: int = 5, border: int = 3,
                      precision: int = PRECISION_FP64) -> "Gradient":

@@ -301,6 +301,27 @@ void dic_image_free(dic_image *img)
     delete img;
 }
 
+static int run_prefilter(dic_ctx *ctx, const dic_image *img, dic_coeffs *c)
+{
+    const size_t tmp = sizeof(double) * (size_t)c->pitch * c->rows;
+    int rc = ensure_staging(ctx, 2 * tmp);
+    if (rc != DIC_OK) return rc;
+    double *ta = (double *)ctx->staging;
+    double *tb = ta + (size_t)c->pitch * c->rows;
+    return check_hip(ctx,
+                     launch_prefilter(img->data, img->width, img->height, c->degree, c->border, c->exact,
+                                      c->precision, c->data, ta, tb, ctx->stream),
+                     "prefilter launch");
+}
+
+int dic_bspline_prefilter_into(dic_ctx *ctx, const dic_image *img, dic_coeffs *out)
+{
+    if (!ctx || !img || !out) return DIC_ERR_INVALID_ARGUMENT;
+    if (img->width != out->width || img->height != out->height)
+        return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "image and coefficient handle sizes differ");
+    return run_prefilter(ctx, img, out);
+}
+
 int dic_bspline_prefilter(dic_ctx *ctx, const dic_image *img, int degree, int border, int exact, int precision,
                           dic_coeffs **out)
 {
@@ -331,16 +352,8 @@ int dic_bspline_prefilter(dic_ctx *ctx, const dic_image *img, int degree, int bo
         delete c;
         return rc;
     }
-    const size_t tmp = sizeof(double) * (size_t)PW * PH;
-    rc = ensure_staging(ctx, 2 * tmp);
-    if (rc == DIC_OK) {
-        double *ta = (double *)ctx->staging;
-        double *tb = ta + (size_t)PW * PH;
-        rc = check_hip(ctx,
-                       launch_prefilter(img->data, img->width, img->height, degree, border, exact, precision, c->data,
-                                        ta, tb, ctx->stream),
-                       "prefilter launch");
-    }
+    c->exact = exact ? 1 : 0;
+    rc = run_prefilter(ctx, img, c);
     if (rc != DIC_OK) {
         ctx_free(ctx, c->data, c->bytes);
         delete c;
@@ -434,6 +447,16 @@ int dic_ref_gradient(dic_ctx *ctx, const dic_image *ref, int mode, int degree, i
     }
     *out = g;
     return DIC_OK;
+}
+
+int dic_ref_gradient_into(dic_ctx *ctx, const dic_image *ref, dic_grad *out)
+{
+    if (!ctx || !ref || !out) return DIC_ERR_INVALID_ARGUMENT;
+    if (ref->width != out->width || ref->height != out->height)
+        return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "image and gradient handle sizes differ");
+    return check_hip(ctx, launch_gradient_central(ref->data, ref->width, ref->height, out->precision, out->gx, out->gy,
+                                                  ctx->stream),
+                     "gradient launch");
 }
 
 int dic_grad_download(dic_ctx *ctx, const dic_grad *g, double *gx_host, double *gy_host)

@@ -45,7 +45,7 @@ struct dic_grad {
 
 struct dic_coeffs {
     dic_ctx *ctx = nullptr;
-    int width = 0, height = 0, degree = 5, border = 3, precision = 0;
+    int width = 0, height = 0, degree = 5, border = 3, precision = 0, exact = 1;
     int pitch = 0, rows = 0; // padded width and height
     void *data = nullptr;
     size_t bytes = 0;
