@@ -6,10 +6,18 @@
 // (sample table in dy-outer/dx-inner order, icgn.cpp:485-489); every sum the
 // reference forms serially is a per-lane partial followed by a butterfly
 // (xor-shuffle) reduction, after which every lane holds the identical total.
-// The Hessian is accumulated and LDL^T-factored once per POI (Eigen 3.4 LDLT
-// semantics, icgn.cpp:532-542); per iteration lane 0 solves for dp, applies
-// the inverse-compositional update (icgn.cpp:83-120 / :289-357) and
-// publishes the new parameters through LDS.
+//
+// Once per POI the Hessian is accumulated and LDL^T-factored with Eigen 3.4
+// semantics (icgn.cpp:532-542), then turned into the solve operator H+:
+// column j of H+ is the faithful LDLT solve of e_j (lane j computes it), so
+// per iteration dp = -H+ G (icgn.cpp:594) is a lane-parallel mat-vec equal to
+// the reference's solve up to rounding (the solve is linear in G, including
+// its zeroing of vanishing pivots).  The inverse-compositional update runs on
+// all lanes with uniform values: the affine one in closed form
+// (icgn.cpp:83-120), the second-order one (icgn.cpp:289-357) as a
+// wave-distributed full-pivot Gauss-Jordan elimination (one matrix entry per
+// lane) with FullPivLU's pivot order and rank test.  No barrier or LDS
+// traffic is needed inside the iteration loop.
 //
 // Precision variants
 //   R = double (plain drop-in): FP64 everywhere, the reference's two-pass
@@ -62,6 +70,12 @@ __device__ __forceinline__ int wave_sum_i(int v)
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
     return v;
 }
+__device__ __forceinline__ double uniform_f64(double v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
 
 template <int P, typename R> __device__ __forceinline__ void steepest_descent(R gx, R gy, R lx, R ly, R *sd)
 {
@@ -99,35 +113,21 @@ struct IcgnArgs {
 };
 
 struct IcgnShared {
-    double H[144];  // Hessian, then its LDL^T factor (lower triangle)
-    double lu[36];  // 6x6 scratch for the second-order update
-    double inv[36];
-    double gp[36];
-    double p[12];   // current warp parameters (FP64 accumulator)
-    double x[12];   // solve work vector
-    double dp[12];  // parameter increment
-    double pn[12];  // updated parameters
-    double g6[6];   // 6-vector work area of the 6x6 inverse
-    double red[32]; // wave-reduced sums of the current pass
+    double H[144];    // Hessian, then its LDL^T factor (lower triangle)
+    double Hinv[144]; // H+ = LDLT solve operator, row-major
+    double x[12];     // factorisation work vector
+    double p[12];     // current warp parameters (FP64 accumulator)
+    double dp[12];    // increment of the current iteration
+    double X[2][6];   // rows 0, 1 of G(p) G(dp)^-1 (second-order update)
+    double piv[6];    // pivot magnitudes of the 6 x 6 elimination
+    double red[16];   // G (P entries) and C of the current iteration
     double ssd[12], sfsd[12];
-    double fmean, fnorm, sfh, sfh2, corr;
+    double fmean, fnorm, sfh, sfh2;
     int tr[12];
-    int rt[6], ct[6];
     int ns, state;
 };
 
-__device__ __forceinline__ double uniform_f64(double v)
-{
-    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
-    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ float uniform_f32(float v)
-{
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-
-// ---- Eigen 3.4 LDLT (ldlt_inplace<Lower>::unblocked), lane 0 only ----
+// ---- Eigen 3.4 LDLT factorisation (ldlt_inplace<Lower>::unblocked), lane 0 ----
 template <int P> __device__ void ldlt_factor(double *a, int *tr, double *temp, int *ok, int *is_positive)
 {
     int sign = 2 /*zero*/, found_zero = 0, ret = 1;
@@ -199,6 +199,54 @@ template <int P> __device__ void ldlt_factor(double *a, int *tr, double *temp, i
     *is_positive = (sign == 0 || sign == 2);
 }
 
+// Column `lane` of the LDLT solve operator (Eigen LDLT::_solve_impl:
+// P b, L^-1, D^+ with |d| > DBL_MIN, L^-T, P^T), for lanes < P; each lane
+// works in place on its own column of sh.Hinv (once per POI).
+template <int P> __device__ __forceinline__ void ldlt_operator_column(IcgnShared &sh, int lane)
+{
+    if (lane >= P) return;
+    double *x = sh.Hinv + lane; // x[i] lives at x[i * P]
+#pragma unroll 1
+    for (int i = 0; i < P; ++i) x[i * P] = (i == lane) ? 1.0 : 0.0;
+#pragma unroll 1
+    for (int k = 0; k < P; ++k) {
+        const int t = sh.tr[k];
+        if (t != k) {
+            const double v = x[k * P];
+            x[k * P] = x[t * P];
+            x[t * P] = v;
+        }
+    }
+#pragma unroll 1
+    for (int i = 0; i < P; ++i) {
+        double s = x[i * P];
+#pragma unroll 1
+        for (int j = 0; j < i; ++j) s -= sh.H[i * P + j] * x[j * P];
+        x[i * P] = s;
+    }
+#pragma unroll 1
+    for (int i = 0; i < P; ++i) {
+        const double d = sh.H[i * P + i];
+        x[i * P] = fabs(d) > DBL_MIN ? x[i * P] / d : 0.0;
+    }
+#pragma unroll 1
+    for (int i = P - 1; i >= 0; --i) {
+        double s = x[i * P];
+#pragma unroll 1
+        for (int j = i + 1; j < P; ++j) s -= sh.H[j * P + i] * x[j * P];
+        x[i * P] = s;
+    }
+#pragma unroll 1
+    for (int k = P - 1; k >= 0; --k) {
+        const int t = sh.tr[k];
+        if (t != k) {
+            const double v = x[k * P];
+            x[k * P] = x[t * P];
+            x[t * P] = v;
+        }
+    }
+}
+
 // inverse_compositional_affine_update, icgn.cpp:83-120
 __device__ __forceinline__ void ic_affine_update(const double *p, const double *d, double *o)
 {
@@ -217,136 +265,108 @@ __device__ __forceinline__ void ic_affine_update(const double *p, const double *
     o[5] = ((dp2 + 1.0) * (dvdy + 1.0)) / den - (dp3 * dvdx) / den - 1.0;
 }
 
-// build_second_order_warp_matrix, icgn.cpp:196-261 (row-major 6x6)
-__device__ void warp_matrix(const double *p, double *G)
+// Entry (row, col) of build_second_order_warp_matrix (icgn.cpp:196-261).
+__device__ double warp_entry(const double *q, int row, int col)
 {
-    const double u = p[0], v = p[1], ux = p[2], uy = p[3], vx = p[4], vy = p[5];
-    const double uxx = p[6], uxy = p[7], uyy = p[8], vxx = p[9], vxy = p[10], vyy = p[11];
-    G[0] = u; G[1] = 1.0 + ux; G[2] = uy; G[3] = 0.5 * uxx; G[4] = uxy; G[5] = 0.5 * uyy;
-    G[6] = v; G[7] = vx; G[8] = 1.0 + vy; G[9] = 0.5 * vxx; G[10] = vxy; G[11] = 0.5 * vyy;
-    G[12] = 1.0; G[13] = 0.0; G[14] = 0.0; G[15] = 0.0; G[16] = 0.0; G[17] = 0.0;
-    G[18] = u * u;
-    G[19] = 2.0 * u * (1.0 + ux);
-    G[20] = 2.0 * u * uy;
-    G[21] = 1.0 + 2.0 * ux + ux * ux + u * uxx;
-    G[22] = 2.0 * (uy + ux * uy + u * uxy);
-    G[23] = uy * uy + u * uyy;
-    G[24] = u * v;
-    G[25] = v + u * vx + v * ux;
-    G[26] = u + u * vy + v * uy;
-    G[27] = vx + ux * vx + 0.5 * (u * vxx + v * uxx);
-    G[28] = 1.0 + vy + ux + ux * vy + uy * vx + u * vxy + v * uxy;
-    G[29] = uy + uy * vy + 0.5 * (u * vyy + v * uyy);
-    G[30] = v * v;
-    G[31] = 2.0 * v * vx;
-    G[32] = 2.0 * v * (1.0 + vy);
-    G[33] = vx * vx + v * vxx;
-    G[34] = 2.0 * (vx + vx * vy + v * vxy);
-    G[35] = 1.0 + 2.0 * vy + vy * vy + v * vyy;
+    const double u = q[0], v = q[1], ux = q[2], uy = q[3], vx = q[4], vy = q[5];
+    const double uxx = q[6], uxy = q[7], uyy = q[8], vxx = q[9], vxy = q[10], vyy = q[11];
+    switch (row * 6 + col) {
+    case 0: return u;
+    case 1: return 1.0 + ux;
+    case 2: return uy;
+    case 3: return 0.5 * uxx;
+    case 4: return uxy;
+    case 5: return 0.5 * uyy;
+    case 6: return v;
+    case 7: return vx;
+    case 8: return 1.0 + vy;
+    case 9: return 0.5 * vxx;
+    case 10: return vxy;
+    case 11: return 0.5 * vyy;
+    case 12: return 1.0;
+    case 13: case 14: case 15: case 16: case 17: return 0.0;
+    case 18: return u * u;
+    case 19: return 2.0 * u * (1.0 + ux);
+    case 20: return 2.0 * u * uy;
+    case 21: return 1.0 + 2.0 * ux + ux * ux + u * uxx;
+    case 22: return 2.0 * (uy + ux * uy + u * uxy);
+    case 23: return uy * uy + u * uyy;
+    case 24: return u * v;
+    case 25: return v + u * vx + v * ux;
+    case 26: return u + u * vy + v * uy;
+    case 27: return vx + ux * vx + 0.5 * (u * vxx + v * uxx);
+    case 28: return 1.0 + vy + ux + ux * vy + uy * vx + u * vxy + v * uxy;
+    case 29: return uy + uy * vy + 0.5 * (u * vyy + v * uyy);
+    case 30: return v * v;
+    case 31: return 2.0 * v * vx;
+    case 32: return 2.0 * v * (1.0 + vy);
+    case 33: return vx * vx + v * vxx;
+    case 34: return 2.0 * (vx + vx * vy + v * vxy);
+    default: return 1.0 + 2.0 * vy + vy * vy + v * vyy;
+    }
 }
 
-// Eigen 3.4 FullPivLU of a 6x6 (complete pivoting, first maximum in
-// column-major scan) + rank test + inverse; returns false if not invertible.
-__device__ bool fullpivlu_inverse6(double *lu, int *rt, int *ct, double *inv, double *c)
+// inverse_compositional_second_order_update (icgn.cpp:289-357) on the whole
+// wave.  G_new = G(p) G(dp)^-1 P^-1 and extract_from_warp_matrix (:265-287)
+// reads only rows 0 and 1 of G_new, so it suffices to solve X G(dp) = B with
+// B = rows 0, 1 of G(p): lane r*8+c holds entry (r, c) of [G(dp)^T | B^T]
+// (6 x 8, lanes 0..47) and a full-pivot Gauss-Jordan elimination runs with
+// FullPivLU's pivot choice (largest magnitude, first in G(dp)'s column-major
+// order) and rank test (|pivot| > max|pivot| * 6 eps).  Reads sh.p, sh.dp;
+// writes sh.X; returns whether G(dp) is invertible (wave-uniform).
+__device__ bool second_order_solve_wave(IcgnShared &sh, int lane)
 {
+    const int r = lane >> 3, c = lane & 7;
+    const bool active = lane < 48;
+    double a = 0.0;
+    if (active) a = (c < 6) ? warp_entry(sh.dp, c, r) : warp_entry(sh.p, c - 6, r);
+    int perm = c; // in column c < 6: which unknown it holds (tracked per lane)
+    double maxpiv = 0.0;
     int nonzero = 6;
-    double maxpivot = 0.0;
+#pragma unroll 1
     for (int k = 0; k < 6; ++k) {
-        int br = k, bc = k;
-        double best = -1.0;
-        for (int c = k; c < 6; ++c)
-            for (int r = k; r < 6; ++r) {
-                const double s = fabs(lu[r * 6 + c]);
-                if (s > best) {
-                    best = s;
-                    br = r;
-                    bc = c;
-                }
+        // pivot: argmax |a| over rows >= k, columns k..5; ties -> smallest row-major key
+        double best = (active && r >= k && c >= k && c < 6) ? fabs(a) : -1.0;
+        int key = r * 6 + c;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            const double ob = __shfl_xor(best, m, 64);
+            const int ok = __shfl_xor(key, m, 64);
+            if (ob > best || (ob == best && ok < key)) {
+                best = ob;
+                key = ok;
             }
+        }
         if (best == 0.0) {
             nonzero = k;
-            for (int i = k; i < 6; ++i) rt[i] = ct[i] = i;
             break;
         }
-        if (best > maxpivot) maxpivot = best;
-        rt[k] = br;
-        ct[k] = bc;
-        if (k != br)
-            for (int c = 0; c < 6; ++c) {
-                const double t = lu[k * 6 + c];
-                lu[k * 6 + c] = lu[br * 6 + c];
-                lu[br * 6 + c] = t;
-            }
-        if (k != bc)
-            for (int r = 0; r < 6; ++r) {
-                const double t = lu[r * 6 + k];
-                lu[r * 6 + k] = lu[r * 6 + bc];
-                lu[r * 6 + bc] = t;
-            }
-        const double piv = lu[k * 6 + k];
-        for (int r = k + 1; r < 6; ++r) lu[r * 6 + k] /= piv;
-        for (int r = k + 1; r < 6; ++r)
-            for (int c = k + 1; c < 6; ++c) lu[r * 6 + c] -= lu[r * 6 + k] * lu[k * 6 + c];
+        const int br = key / 6, bc = key % 6;
+        maxpiv = fmax(maxpiv, best);
+        if (lane == 0) sh.piv[k] = best;
+        // swap rows k <-> br (all 8 columns), then columns k <-> bc (the 6 x 6 part)
+        const int src_r = (r == k) ? br : ((r == br) ? k : r);
+        a = __shfl(a, src_r * 8 + c, 64);
+        const int src_c = (c < 6) ? ((c == k) ? bc : ((c == bc) ? k : c)) : c;
+        a = __shfl(a, r * 8 + src_c, 64);
+        perm = __shfl(perm, r * 8 + src_c, 64);
+        // eliminate column k from every other row, normalise row k
+        const double piv = __shfl(a, k * 8 + k, 64);
+        const double fr = __shfl(a, r * 8 + k, 64);
+        const double pr = __shfl(a, k * 8 + c, 64);
+        if (active) a = (r == k) ? a / piv : a - (fr / piv) * pr;
     }
-    const double thr = fabs(maxpivot) * (6.0 * DBL_EPSILON);
+    __syncthreads();
     int rank = 0;
-    for (int i = 0; i < nonzero; ++i) rank += fabs(lu[i * 6 + i]) > thr;
-    if (rank != 6) return false;
-    for (int col = 0; col < 6; ++col) {
-        for (int i = 0; i < 6; ++i) c[i] = (i == col) ? 1.0 : 0.0;
-        for (int k = 0; k < 6; ++k)
-            if (rt[k] != k) {
-                const double t = c[k];
-                c[k] = c[rt[k]];
-                c[rt[k]] = t;
-            }
-        for (int i = 0; i < 6; ++i)
-            for (int j = 0; j < i; ++j) c[i] -= lu[i * 6 + j] * c[j];
-        for (int i = 5; i >= 0; --i) {
-            for (int j = i + 1; j < 6; ++j) c[i] -= lu[i * 6 + j] * c[j];
-            c[i] /= lu[i * 6 + i];
-        }
-        for (int k = 5; k >= 0; --k)
-            if (ct[k] != k) {
-                const double t = c[k];
-                c[k] = c[ct[k]];
-                c[ct[k]] = t;
-            }
-        for (int i = 0; i < 6; ++i) inv[i * 6 + col] = c[i];
-    }
-    return true;
-}
-
-// inverse_compositional_second_order_update, icgn.cpp:289-357, lane 0 only,
-// 6x6 scratch in LDS.
-__device__ void ic_second_order_update(const double *p, const double *d, double *o, IcgnShared &sh)
-{
-    warp_matrix(p, sh.gp);
-    warp_matrix(d, sh.lu);
-    if (!fullpivlu_inverse6(sh.lu, sh.rt, sh.ct, sh.inv, sh.g6)) {
-        for (int i = 0; i < 12; ++i) o[i] = p[i] - d[i];
-        return;
-    }
-    // rows 0 and 1 of G(p) * G(dp)^-1 * P^-1 (extract_from_warp_matrix reads
-    // only those): column 0 <- product column 2, 1 <- 0, 2 <- 1, 3..5 identity
-    auto T = [&](int r, int c) {
-        double acc = 0.0;
 #pragma unroll 1
-        for (int k = 0; k < 6; ++k) acc += sh.gp[r * 6 + k] * sh.inv[k * 6 + c];
-        return acc;
-    };
-    o[0] = T(0, 2);
-    o[2] = T(0, 0) - 1.0;
-    o[3] = T(0, 1);
-    o[6] = 2.0 * T(0, 3);
-    o[7] = T(0, 4);
-    o[8] = 2.0 * T(0, 5);
-    o[1] = T(1, 2);
-    o[4] = T(1, 0);
-    o[5] = T(1, 1) - 1.0;
-    o[9] = 2.0 * T(1, 3);
-    o[10] = T(1, 4);
-    o[11] = 2.0 * T(1, 5);
+    for (int k = 0; k < nonzero; ++k)
+        if (sh.piv[k] > maxpiv * (6.0 * DBL_EPSILON)) ++rank;
+    if (rank != 6) return false;
+    // row k of the right-hand part holds unknown perm(column k) of X^T
+    const int unknown = __shfl(perm, r * 8 + r, 64);
+    if (active && c >= 6) sh.X[c - 6][unknown] = a;
+    __syncthreads();
+    return true;
 }
 
 template <int D, typename R, typename CS>
@@ -374,114 +394,6 @@ __device__ __forceinline__ void decode(int t, int &dx, int &dy)
     dy = t >> 16;
 }
 
-// Sum over the wave; lane 0 stores the total to LDS.
-__device__ __forceinline__ void reduce_to(double v, double *dst, int lane)
-{
-    v = wave_sum(v);
-    if (lane == 0) *dst = v;
-}
-
-// ---- per-iteration serial step, lane 0 only (icgn.cpp:584-610) ----
-// Reads the reduced sums of the sampling pass from sh.red, forms G and C,
-// solves dp = -H^-1 G with the stored LDL^T factor, checks finiteness,
-// applies the inverse-compositional update and sets sh.state:
-// 0 continue, 1 converged, 2 numerical failure.
-template <int P, int OBJ, bool TWOPASS>
-__device__ __forceinline__ void lane0_step(IcgnShared &sh, double thr)
-{
-    constexpr bool ZNSSD = OBJ == DIC_OBJECTIVE_ZNSSD;
-    double *x = sh.x;
-    double C;
-    if constexpr (TWOPASS) { // red = [G(P) unscaled, C]
-        const double sc = 2.0 / sh.fnorm;
-#pragma unroll 1
-        for (int j = 0; j < P; ++j) x[j] = sh.red[j] * sc;
-        C = sh.red[P];
-    } else if constexpr (ZNSSD) { // red = [S1, S2, SFG, S(g-K)sd (P)]
-        const double S1 = sh.red[0], S2 = sh.red[1], SFG = sh.red[2];
-        const double md = S1 / (double)sh.ns;
-        const double var = S2 - S1 * md;
-        if (!(var > 0.0) || sqrt(var) <= kEps) {
-            sh.state = 2;
-            return;
-        }
-        const double gnorm = sqrt(var), sc = 2.0 / sh.fnorm;
-#pragma unroll 1
-        for (int j = 0; j < P; ++j) x[j] = sc * (sh.sfsd[j] - (sh.red[3 + j] - md * sh.ssd[j]) / gnorm);
-        C = sh.sfh2 - 2.0 * (SFG - md * sh.sfh) / gnorm + var / (gnorm * gnorm);
-    } else { // SSD: red = [C, G(P)]
-        C = sh.red[0];
-#pragma unroll 1
-        for (int j = 0; j < P; ++j) x[j] = sh.red[1 + j];
-    }
-    sh.corr = C;
-    // Eigen LDLT::solve: P b, L^-1, D^+ (|d| > DBL_MIN), L^-T, P^T
-#pragma unroll 1
-    for (int k = 0; k < P; ++k) {
-        const int t = sh.tr[k];
-        if (t != k) {
-            const double v = x[k];
-            x[k] = x[t];
-            x[t] = v;
-        }
-    }
-#pragma unroll 1
-    for (int i = 0; i < P; ++i) {
-        double s = x[i];
-#pragma unroll 1
-        for (int j = 0; j < i; ++j) s -= sh.H[i * P + j] * x[j];
-        x[i] = s;
-    }
-#pragma unroll 1
-    for (int i = 0; i < P; ++i) {
-        const double d = sh.H[i * P + i];
-        x[i] = fabs(d) > DBL_MIN ? x[i] / d : 0.0;
-    }
-#pragma unroll 1
-    for (int i = P - 1; i >= 0; --i) {
-        double s = x[i];
-#pragma unroll 1
-        for (int j = i + 1; j < P; ++j) s -= sh.H[j * P + i] * x[j];
-        x[i] = s;
-    }
-#pragma unroll 1
-    for (int k = P - 1; k >= 0; --k) {
-        const int t = sh.tr[k];
-        if (t != k) {
-            const double v = x[k];
-            x[k] = x[t];
-            x[t] = v;
-        }
-    }
-    bool fin = true;
-    double nrm = 0.0;
-#pragma unroll 1
-    for (int j = 0; j < P; ++j) {
-        const double d = -x[j];
-        sh.dp[j] = d;
-        fin = fin && isfinite(d);
-        nrm += d * d;
-    }
-    if (!fin) {
-        sh.state = 2;
-        return;
-    }
-    nrm = sqrt(nrm);
-    if constexpr (P == 6) {
-        ic_affine_update(sh.p, sh.dp, sh.pn);
-    } else {
-        ic_second_order_update(sh.p, sh.dp, sh.pn, sh);
-    }
-    bool pf = true;
-#pragma unroll 1
-    for (int j = 0; j < P; ++j) {
-        const double v = sh.pn[j];
-        sh.p[j] = v;
-        pf = pf && isfinite(v);
-    }
-    sh.state = !pf ? 2 : (nrm < thr ? 1 : 0);
-}
-
 template <int P, int OBJ, int D, typename R, bool MASKED>
 __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
 {
@@ -501,7 +413,7 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
     const double px = a.xy[2 * poi], py = a.xy[2 * poi + 1];
     const int cx = (int)round(px), cy = (int)round(py);
 
-    // sample s of the full circle is used by this POI (masked: inside the image and the ROI)
+    // sample of the full circle used by this POI (masked: inside the image and the ROI)
     auto usable = [&](int x, int y) -> bool {
         if constexpr (MASKED) return x >= 0 && y >= 0 && x < W && y < Hh && a.roi[(size_t)y * W + x];
         return true;
@@ -617,37 +529,49 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
             }
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-                reduce_to(a_sd[j].value(), &sh.ssd[j], lane);
-                reduce_to(a_fsd[j].value(), &sh.sfsd[j], lane);
+                const double v1 = wave_sum(a_sd[j].value()), v2 = wave_sum(a_fsd[j].value());
+                if (lane == 0) {
+                    sh.ssd[j] = v1;
+                    sh.sfsd[j] = v2;
+                }
             }
-            reduce_to(a_f.value(), &sh.sfh, lane);
-            reduce_to(a_f2.value(), &sh.sfh2, lane);
+            const double v1 = wave_sum(a_f.value()), v2 = wave_sum(a_f2.value());
+            if (lane == 0) {
+                sh.sfh = v1;
+                sh.sfh2 = v2;
+            }
         }
         __syncthreads();
         if (lane == 0) {
             int ok, pos;
             ldlt_factor<P>(sh.H, sh.tr, sh.x, &ok, &pos);
             sh.state = (ok && pos) ? 0 : 1;
+        }
+        __syncthreads();
+        if (sh.state != 0) {
+            early = true;
+            status = DIC_NUMERICAL_FAILURE;
+        } else {
+            ldlt_operator_column<P>(sh, lane);
+            __syncthreads();
+        }
+    }
+
+    double corr = __builtin_inf();
+    bool solved = false;
+    if (!early) {
+        if (lane == 0) {
             const dic_initial ini = a.init[poi];
-#pragma unroll
-            for (int j = 0; j < 12; ++j) sh.p[j] = 0.0;
             sh.p[0] = ini.u;
             sh.p[1] = ini.v;
             sh.p[2] = ini.du_dx;
             sh.p[3] = ini.du_dy;
             sh.p[4] = ini.dv_dx;
             sh.p[5] = ini.dv_dy;
-            sh.corr = __builtin_inf();
+#pragma unroll
+            for (int j = 6; j < 12; ++j) sh.p[j] = 0.0;
         }
         __syncthreads();
-        if (sh.state != 0) {
-            early = true;
-            status = DIC_NUMERICAL_FAILURE;
-        }
-    }
-
-    bool solved = false;
-    if (!early) {
         const double Wm1 = (double)(W - 1), Hm1 = (double)(Hh - 1);
         const int ns = sh.ns;
         const double fmean = uniform_f64(sh.fmean), fnorm = uniform_f64(sh.fnorm);
@@ -655,7 +579,6 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
         bool converged = false, failed = false;
         for (int it = 0; it < a.max_it; ++it) {
             iterations = it + 1;
-            // current parameters, uniform across the wave
             double p[12];
 #pragma unroll
             for (int j = 0; j < 12; ++j) p[j] = (j < P) ? uniform_f64(sh.p[j]) : 0.0;
@@ -746,6 +669,7 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
                 failed = true;
                 break;
             }
+            // ---- G and C into sh.red (lane 0 stores each wave total) ----
             if constexpr (TWOPASS) {
                 // two-pass mean / norm / residual, icgn.cpp:577-592
                 const double gmean = wave_sum(s1.value()) / (double)ns;
@@ -783,37 +707,109 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
                     for (int j = 0; j < P; ++j) gl[j] += e * sd[j];
                     cl += e * e;
                 }
+                const double sc = 2.0 / fnorm;
 #pragma unroll
-                for (int j = 0; j < P; ++j) reduce_to(gl[j], &sh.red[j], lane);
-                reduce_to(cl, &sh.red[P], lane);
+                for (int j = 0; j < P; ++j) {
+                    const double v = wave_sum(gl[j]) * sc;
+                    if (lane == 0) sh.red[j] = v;
+                }
+                const double v = wave_sum(cl);
+                if (lane == 0) sh.red[P] = v;
             } else if constexpr (ZNSSD) {
-                reduce_to(s1.value(), &sh.red[0], lane);
-                reduce_to(s2.value(), &sh.red[1], lane);
-                reduce_to(sfg.value(), &sh.red[2], lane);
+                const double S1 = wave_sum(s1.value()), S2 = wave_sum(s2.value()), SFG = wave_sum(sfg.value());
+                const double md = S1 / (double)ns;
+                const double var = S2 - S1 * md;
+                if (!(var > 0.0) || sqrt(var) <= kEps) {
+                    status = DIC_NUMERICAL_FAILURE;
+                    failed = true;
+                    break;
+                }
+                const double gnorm = sqrt(var), sc = 2.0 / fnorm;
 #pragma unroll
-                for (int j = 0; j < P; ++j) reduce_to(sg[j].value(), &sh.red[3 + j], lane);
+                for (int j = 0; j < P; ++j) {
+                    const double v = wave_sum(sg[j].value());
+                    if (lane == 0) sh.red[j] = sc * (sh.sfsd[j] - (v - md * sh.ssd[j]) / gnorm);
+                }
+                if (lane == 0) sh.red[P] = sh.sfh2 - 2.0 * (SFG - md * sh.sfh) / gnorm + var / (gnorm * gnorm);
             } else {
-                reduce_to(s2.value(), &sh.red[0], lane);
 #pragma unroll
-                for (int j = 0; j < P; ++j) reduce_to(sg[j].value(), &sh.red[1 + j], lane);
+                for (int j = 0; j < P; ++j) {
+                    const double v = wave_sum(sg[j].value());
+                    if (lane == 0) sh.red[j] = v;
+                }
+                const double v = wave_sum(s2.value());
+                if (lane == 0) sh.red[P] = v;
             }
-            if (lane == 0) lane0_step<P, OBJ, TWOPASS>(sh, a.thr);
             __syncthreads();
-            const int st = sh.state;
+            corr = sh.red[P];
+            // ---- dp = -H+ G (icgn.cpp:594): lane i forms row i ----
+            if (lane < P) {
+                double d = 0.0;
+#pragma unroll
+                for (int j = 0; j < P; ++j) d += sh.Hinv[lane * P + j] * sh.red[j];
+                sh.dp[lane] = -d;
+            }
             __syncthreads();
-            if (st == 2) {
+            bool fin = true;
+            double nrm = 0.0;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const double d = sh.dp[j];
+                fin = fin && isfinite(d);
+                nrm += d * d;
+            }
+            if (!fin) {
                 status = DIC_NUMERICAL_FAILURE;
                 failed = true;
                 break;
             }
-            if (st == 1) {
+            nrm = sqrt(nrm);
+            bool invertible = true;
+            if constexpr (P == 12) invertible = second_order_solve_wave(sh, lane);
+            if (lane == 0) {
+                double pn[12];
+                if constexpr (P == 6) {
+                    ic_affine_update(sh.p, sh.dp, pn);
+                } else if (invertible) {
+                    // (X * P^-1): column 0 <- X(., 2), column 1 <- X(., 0), column 2 <- X(., 1)
+                    pn[0] = sh.X[0][2];
+                    pn[2] = sh.X[0][0] - 1.0;
+                    pn[3] = sh.X[0][1];
+                    pn[6] = 2.0 * sh.X[0][3];
+                    pn[7] = sh.X[0][4];
+                    pn[8] = 2.0 * sh.X[0][5];
+                    pn[1] = sh.X[1][2];
+                    pn[4] = sh.X[1][0];
+                    pn[5] = sh.X[1][1] - 1.0;
+                    pn[9] = 2.0 * sh.X[1][3];
+                    pn[10] = sh.X[1][4];
+                    pn[11] = 2.0 * sh.X[1][5];
+                } else { // additive fallback, icgn.cpp:323-337
+#pragma unroll
+                    for (int j = 0; j < 12; ++j) pn[j] = sh.p[j] - sh.dp[j];
+                }
+                int ok = 1;
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    sh.p[j] = pn[j];
+                    ok = ok && isfinite(pn[j]);
+                }
+                sh.state = ok;
+            }
+            __syncthreads();
+            if (!sh.state) {
+                status = DIC_NUMERICAL_FAILURE;
+                failed = true;
+                break;
+            }
+            if (nrm < a.thr) { // icgn.cpp:607-610
                 converged = true;
                 break;
             }
         }
         if (!failed) {
             // running out of iterations still reports Success (icgn.cpp:620-622)
-            if (!converged && isfinite(sh.corr)) converged = true;
+            if (!converged && isfinite(corr)) converged = true;
             status = converged ? DIC_SUCCESS : DIC_NOT_CONVERGED;
             solved = true;
         }
@@ -829,7 +825,7 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
         if (solved) {
 #pragma unroll
             for (int j = 0; j < P; ++j) r.p[j] = sh.p[j];
-            r.correlation = sh.corr;
+            r.correlation = corr;
         } else { // failure exits keep the initial u, v (icgn.cpp:455-462)
             r.p[0] = ini.u;
             r.p[1] = ini.v;
