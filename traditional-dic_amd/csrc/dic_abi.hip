@@ -336,6 +336,8 @@ int dic_bspline_prefilter(dic_ctx *ctx, const dic_image *img, int degree, int bo
     const int b = border < 3 ? 3 : border;
     const int PW = img->width + 2 * b, PH = img->height + 2 * b;
     if (PW > 16384) return set_error(ctx, DIC_ERR_UNSUPPORTED, "padded width above 16384");
+    if ((size_t)PW * PH * sizeof(double) >= ((size_t)1 << 31))
+        return set_error(ctx, DIC_ERR_UNSUPPORTED, "padded coefficient image of 2 GiB or more");
     dic_coeffs *c = new (std::nothrow) dic_coeffs();
     if (!c) return set_error(ctx, DIC_ERR_HIP, "host allocation failed");
     c->ctx = ctx;
@@ -556,6 +558,7 @@ static int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_imag
     L.gx = grad->gx;
     L.gy = grad->gy;
     L.coeffs = def->data;
+    L.coeff_bytes = def->bytes;
     L.pitch = def->pitch;
     L.border = def->border;
     L.degree = def->degree;

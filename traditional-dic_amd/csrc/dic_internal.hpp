@@ -76,6 +76,7 @@ struct IcgnLaunch {
     const float *ref;
     const void *gx, *gy;
     const void *coeffs;
+    size_t coeff_bytes;
     int pitch, border, degree, precision;
     const uint8_t *roi;
     const int *table;

@@ -7,8 +7,11 @@
 // reference forms serially is a per-lane partial followed by a butterfly
 // (xor-shuffle) reduction, after which every lane holds the identical total.
 //
-// Once per POI the Hessian is accumulated and LDL^T-factored with Eigen 3.4
-// semantics (icgn.cpp:532-542), then turned into the solve operator H+:
+// Once per POI the Hessian is accumulated (second order: from 3 x 15
+// gradient-weighted monomial moments, since every entry is a coefficient
+// times sum g_T dx^p dy^q with g_T in {gx^2, gx gy, gy^2}, p + q <= 4) and
+// LDL^T-factored with Eigen 3.4 semantics (icgn.cpp:532-542; 12 x 12: one
+// row per lane, 6 x 6: one lane), then turned into the solve operator H+:
 // column j of H+ is the faithful LDLT solve of e_j (lane j computes it), so
 // per iteration dp = -H+ G (icgn.cpp:594) is a lane-parallel mat-vec equal to
 // the reference's solve up to rounding (the solve is linear in G, including
@@ -16,17 +19,19 @@
 // all lanes with uniform values: the affine one in closed form
 // (icgn.cpp:83-120), the second-order one (icgn.cpp:289-357) as a
 // wave-distributed full-pivot Gauss-Jordan elimination (one matrix entry per
-// lane) with FullPivLU's pivot order and rank test.  No barrier or LDS
-// traffic is needed inside the iteration loop.
+// lane) with FullPivLU's pivot order and rank test.  Inside the iteration
+// loop the reduced sums, dp and p pass through LDS; the workgroup is a
+// single wave, so its barriers cost no waiting.
 //
 // Precision variants
 //   R = double (plain drop-in): FP64 everywhere, the reference's two-pass
 //     mean / norm / residual structure (icgn.cpp:577-592), deformed samples
 //     kept in LDS between the passes.
 //   R = float (mixed): FP32 coefficients, gradients, warp offsets (relative to
-//     the integer part of u, v), sampling and per-lane Kahan-compensated sums;
-//     FP64 cross-lane reductions, Hessian factorisation, solve, warp update
-//     and parameter accumulator.  ZNSSD sums are taken in one pass about the
+//     the integer part of u, v), sampling and products; per-lane sums blocked
+//     in FP32 over 8 samples and folded into FP64; FP64 cross-lane
+//     reductions, Hessian factorisation, solve, warp update and parameter
+//     accumulator.  ZNSSD sums are taken in one pass about the
 //     shift K = mean(f):  sum(g-K), sum((g-K)^2), sum(fhat (g-K)),
 //     sum((g-K) sd); mean, norm, G and C follow algebraically.
 #include <cfloat>
@@ -40,22 +45,29 @@ namespace {
 
 constexpr double kEps = 1e-12; // icgn.cpp:13
 
+// Per-lane accumulator.  FP64: plain sum.  FP32 (mixed precision): values
+// and products are FP32, summed in FP32 over a block of kBlock samples and
+// folded into an FP64 per-lane total (blocked pairwise summation; an FP64
+// add costs the same issue slot as an FP32 add on this GPU, and the result
+// is tighter than Kahan-compensated FP32).
+constexpr int kBlock = 8;
 template <typename R> struct Acc;
 template <> struct Acc<double> {
     double s = 0.0;
     __device__ __forceinline__ void add(double v) { s += v; }
+    __device__ __forceinline__ void fold() {}
     __device__ __forceinline__ double value() const { return s; }
 };
-template <> struct Acc<float> { // Kahan-compensated FP32 partial
-    float s = 0.0f, c = 0.0f;
-    __device__ __forceinline__ void add(float v)
+template <> struct Acc<float> {
+    double s = 0.0;
+    float b = 0.0f;
+    __device__ __forceinline__ void add(float v) { b += v; }
+    __device__ __forceinline__ void fold()
     {
-        const float y = v - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
+        s += (double)b;
+        b = 0.0f;
     }
-    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+    __device__ __forceinline__ double value() const { return s + (double)b; }
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -103,6 +115,7 @@ struct IcgnArgs {
     const uint8_t *roi;
     const int *table;
     int nfull;
+    int coeff_bytes; // size of the padded coefficient image (< 2^31)
     int radius, max_it;
     double thr;
     int min_needed;
@@ -123,6 +136,7 @@ struct IcgnShared {
     double red[16];   // G (P entries) and C of the current iteration
     double ssd[12], sfsd[12];
     double fmean, fnorm, sfh, sfh2;
+    double mom[3][15]; // second-order Hessian moments (gx^2, gx gy, gy^2) x dx^p dy^q
     int tr[12];
     int ns, state;
 };
@@ -197,6 +211,96 @@ template <int P> __device__ void ldlt_factor(double *a, int *tr, double *temp, i
     }
     *ok = ret;
     *is_positive = (sign == 0 || sign == 2);
+}
+
+// ---- Eigen 3.4 LDLT factorisation (ldlt_inplace<Lower>::unblocked) on the wave ----
+// Lane i < P holds row i of the symmetric matrix in registers.  Step k picks
+// the largest |diagonal| among rows k..P-1 (the not-yet-updated diagonal,
+// first maximum), applies the symmetric transposition (exchange rows k and
+// big between lanes, swap columns k and big in every row), updates column k
+// left-looking (A(i,k) -= sum_j L(i,j) D(j) L(k,j)), and scales it by the
+// pivot.  info / isPositive follow the same rules as the reference's Eigen
+// (zero pivot handling, sign tracking).  Writes L (strict lower), D
+// (diagonal) to sh.H and the transpositions to sh.tr.
+template <int P> __device__ void ldlt_factor_wave(IcgnShared &sh, int lane, int &ok_out, int &pos_out)
+{
+    const bool own = lane < P;
+    double row[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) row[j] = own ? sh.H[lane * P + j] : 0.0;
+    double dg = own ? sh.H[lane * P + lane] : 0.0;
+    int sign = 2 /*zero*/, found_zero = 0, ret = 1;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        double best = (own && lane >= k) ? fabs(dg) : -1.0;
+        int key = lane;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            const double ob = __shfl_xor(best, m, 64);
+            const int ok = __shfl_xor(key, m, 64);
+            if (ob > best || (ob == best && ok < key)) {
+                best = ob;
+                key = ok;
+            }
+        }
+        const int big = key;
+        if (lane == 0) sh.tr[k] = big;
+        if (big != k) {
+            const int src = (lane == k) ? big : ((lane == big) ? k : lane);
+#pragma unroll
+            for (int j = 0; j < P; ++j) row[j] = __shfl(row[j], src, 64);
+            dg = __shfl(dg, src, 64);
+#pragma unroll
+            for (int m = k + 1; m < P; ++m)
+                if (big == m) {
+                    const double t = row[k];
+                    row[k] = row[m];
+                    row[m] = t;
+                }
+        }
+        if (k > 0) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < k; ++j) acc += row[j] * (__shfl(dg, j, 64) * __shfl(row[j], k, 64));
+            row[k] -= acc;
+        }
+        const double akk = __shfl(row[k], k, 64);
+        const bool valid = fabs(akk) > 0.0;
+        if (k == 0 && !valid) {
+            sign = 2;
+            ret = 0;
+            if (lane < P) sh.tr[lane] = lane;
+            break;
+        }
+        if (k + 1 < P) {
+            if (valid) {
+                if (lane > k) row[k] /= akk;
+            } else {
+                ret = ret && !__any(own && lane > k && row[k] != 0.0);
+            }
+        }
+        if (found_zero && valid) ret = 0;
+        else if (!valid) found_zero = 1;
+        if (sign == 0) {
+            if (akk < 0.0) sign = 3;
+        } else if (sign == 1) {
+            if (akk > 0.0) sign = 3;
+        } else if (sign == 2) {
+            if (akk > 0.0) sign = 0;
+            else if (akk < 0.0) sign = 1;
+        }
+        if (lane == k) dg = akk;
+    }
+    __syncthreads();
+    if (own) {
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+            if (j < lane) sh.H[lane * P + j] = row[j];
+        sh.H[lane * P + lane] = dg;
+    }
+    ok_out = ret;
+    pos_out = (sign == 0 || sign == 2);
+    __syncthreads();
 }
 
 // Column `lane` of the LDLT solve operator (Eigen LDLT::_solve_impl:
@@ -369,20 +473,54 @@ __device__ bool second_order_solve_wave(IcgnShared &sh, int lane)
     return true;
 }
 
-template <int D, typename R, typename CS>
-__device__ __forceinline__ R sample_at(const CS *__restrict__ coeffs, int pitch, int b, int ix, int iy, R tx, R ty)
+// Raw buffer view of the padded coefficient image: every row of the n x n
+// block is loaded with the same per-sample byte offset and a wave-uniform
+// row offset (j * pitch * sizeof(R)) in a scalar register, so a sample costs
+// one 32-bit offset computation instead of n 64-bit addresses.  Loads past
+// num_records return 0 (never taken: the warp bounds test runs first).
+// Dword buffer loads of consecutive elements; the backend merges them into
+// dwordx4 / dwordx2 loads that share the per-sample offset.
+template <typename R> struct RowLoad;
+template <> struct RowLoad<float> {
+    template <int N> __device__ __forceinline__ static void row(__amdgpu_buffer_rsrc_t rs, int voff, int soff, float *c)
+    {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            c[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff + 4 * k, soff, 0));
+    }
+};
+template <> struct RowLoad<double> {
+    template <int N> __device__ __forceinline__ static void row(__amdgpu_buffer_rsrc_t rs, int voff, int soff, double *c)
+    {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int lo = (int)__builtin_amdgcn_raw_buffer_load_b32(rs, voff + 8 * k, soff, 0);
+            const int hi = (int)__builtin_amdgcn_raw_buffer_load_b32(rs, voff + 8 * k + 4, soff, 0);
+            c[k] = __hiloint2double(hi, lo);
+        }
+    }
+};
+
+// B-spline value at (ix + tx, iy + ty) from the padded coefficients (row
+// pitch in elements, border b): all n*n loads are issued before the weights.
+template <int D, typename R>
+__device__ __forceinline__ R sample_at(__amdgpu_buffer_rsrc_t rs, int pitch, int b, int ix, int iy, R tx, R ty)
 {
     using S = Spline<D>;
+    const int voff = ((iy + b - S::off) * pitch + (ix + b - S::off)) * (int)sizeof(R);
+    const int rowb = pitch * (int)sizeof(R);
+    R c[S::n][S::n];
+#pragma unroll
+    for (int j = 0; j < S::n; ++j) RowLoad<R>::template row<S::n>(rs, voff, j * rowb, c[j]);
     R wx[S::n], wy[S::n];
     S::weights(tx, wx);
     S::weights(ty, wy);
-    const CS *base = coeffs + (size_t)(iy + b - S::off) * pitch + (ix + b - S::off);
     R v = R(0);
 #pragma unroll
     for (int j = 0; j < S::n; ++j) {
         R rv = R(0);
 #pragma unroll
-        for (int k = 0; k < S::n; ++k) rv += wx[k] * (R)base[(size_t)j * pitch + k];
+        for (int k = 0; k < S::n; ++k) rv += wx[k] * c[j][k];
         v += wy[j] * rv;
     }
     return v;
@@ -394,8 +532,11 @@ __device__ __forceinline__ void decode(int t, int &dx, int &dy)
     dy = t >> 16;
 }
 
+// The second-order mixed kernel asks for 4 waves per SIMD (128 VGPRs): left
+// alone the compiler takes ~160 VGPRs (3 waves) and runs ~15% slower; the
+// spill it needs lies outside the sample loops.
 template <int P, int OBJ, int D, typename R, bool MASKED>
-__global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
+__global__ __launch_bounds__(64, (P == 12 && sizeof(R) == 4) ? 4 : 1) void icgn_kernel(IcgnArgs a)
 {
     constexpr bool ZNSSD = OBJ == DIC_OBJECTIVE_ZNSSD;
     constexpr bool FP64 = sizeof(R) == 8;
@@ -408,7 +549,8 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
     if (poi >= a.n) return;
     const R *__restrict__ gxp = (const R *)a.gx;
     const R *__restrict__ gyp = (const R *)a.gy;
-    const R *__restrict__ cf = (const R *)a.coeffs;
+    const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<void *>(a.coeffs), (short)0, a.coeff_bytes, 0x00020000);
     const int W = a.W, Hh = a.H;
     const double px = a.xy[2 * poi], py = a.xy[2 * poi + 1];
     const int cx = (int)round(px), cy = (int)round(py);
@@ -481,36 +623,89 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
         __syncthreads();
         const double fmean = sh.fmean, fnorm = sh.fnorm;
         const double hscale = ZNSSD ? 2.0 / (fnorm * fnorm) : 1.0;
-        constexpr int RG = (P == 6) ? 6 : 2;
+        if constexpr (P == 6) {
 #pragma unroll
-        for (int r0 = 0; r0 < P; r0 += RG) {
-            Acc<R> acc[RG][P];
-            for (int s = lane; s < a.nfull; s += 64) {
-                int dx, dy;
-                decode(a.table[s], dx, dy);
-                const int x = cx + dx, y = cy + dy;
-                if (!usable(x, y)) continue;
-                R sd[P];
-                steepest_descent<P, R>(gxp[(size_t)y * W + x], gyp[(size_t)y * W + x], (R)dx, (R)dy, sd);
+            for (int r0 = 0; r0 < P; r0 += 6) {
+                Acc<R> acc[6][P];
+                int blk = 0;
+                for (int s = lane; s < a.nfull; s += 64) {
+                    int dx, dy;
+                    decode(a.table[s], dx, dy);
+                    const int x = cx + dx, y = cy + dy;
+                    if (!usable(x, y)) continue;
+                    R sd[P];
+                    steepest_descent<P, R>(gxp[(size_t)y * W + x], gyp[(size_t)y * W + x], (R)dx, (R)dy, sd);
 #pragma unroll
-                for (int i = 0; i < RG; ++i)
+                    for (int i = 0; i < 6; ++i)
 #pragma unroll
-                    for (int j = r0 + i; j < P; ++j) acc[i][j].add(sd[r0 + i] * sd[j]);
-            }
+                        for (int j = i; j < P; ++j) acc[i][j].add(sd[i] * sd[j]);
+                    if (++blk == kBlock) {
+                        blk = 0;
 #pragma unroll
-            for (int i = 0; i < RG; ++i)
+                        for (int i = 0; i < 6; ++i)
 #pragma unroll
-                for (int j = r0 + i; j < P; ++j) {
-                    const double h = wave_sum(acc[i][j].value()) * hscale;
-                    if (lane == 0) {
-                        sh.H[(r0 + i) * P + j] = h;
-                        sh.H[j * P + (r0 + i)] = h;
+                            for (int j = i; j < P; ++j) acc[i][j].fold();
                     }
                 }
+#pragma unroll
+                for (int i = 0; i < 6; ++i)
+#pragma unroll
+                    for (int j = i; j < P; ++j) {
+                        const double h = wave_sum(acc[i][j].value()) * hscale;
+                        if (lane == 0) {
+                            sh.H[i * P + j] = h;
+                            sh.H[j * P + i] = h;
+                        }
+                    }
+            }
+        } else {
+            // Second order: sd_a = g_t(a) * c_a dx^p_a dy^q_a, so every Hessian
+            // entry is c_a c_b * sum G_T dx^p dy^q with T in {gx^2, gx gy, gy^2}
+            // and p + q <= 4: three passes over 15 moments each.
+#pragma unroll
+            for (int T = 0; T < 3; ++T) {
+                Acc<R> m[15];
+                int blk = 0;
+                for (int s = lane; s < a.nfull; s += 64) {
+                    int dx, dy;
+                    decode(a.table[s], dx, dy);
+                    const int x = cx + dx, y = cy + dy;
+                    if (!usable(x, y)) continue;
+                    const R gx = gxp[(size_t)y * W + x], gy = gyp[(size_t)y * W + x];
+                    const R g = (T == 0) ? gx * gx : ((T == 1) ? gx * gy : gy * gy);
+                    const R X = (R)dx, Y = (R)dy, X2 = X * X, Y2 = Y * Y, XY = X * Y;
+                    const R mono[15] = {R(1), X, Y, X2, XY, Y2, X2 * X, X2 * Y, X * Y2, Y2 * Y,
+                                        X2 * X2, X2 * XY, X2 * Y2, XY * Y2, Y2 * Y2};
+#pragma unroll
+                    for (int k = 0; k < 15; ++k) m[k].add(g * mono[k]);
+                    if (++blk == kBlock) {
+                        blk = 0;
+#pragma unroll
+                        for (int k = 0; k < 15; ++k) m[k].fold();
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 15; ++k) {
+                    const double v = wave_sum(m[k].value());
+                    if (lane == 0) sh.mom[T][k] = v;
+                }
+            }
+            __syncthreads();
+            // parameter a: gradient type (0 gx, 1 gy), monomial exponents, coefficient
+            constexpr int ty[12] = {0, 1, 0, 0, 1, 1, 0, 0, 0, 1, 1, 1};
+            constexpr int pe[12] = {0, 0, 1, 0, 1, 0, 2, 1, 0, 2, 1, 0};
+            constexpr int qe[12] = {0, 0, 0, 1, 0, 1, 0, 1, 2, 0, 1, 2};
+            constexpr double ce[12] = {1, 1, 1, 1, 1, 1, 0.5, 1, 0.5, 0.5, 1, 0.5};
+            for (int e = lane; e < 144; e += 64) {
+                const int ia = e / 12, ib = e % 12;
+                const int pp = pe[ia] + pe[ib], qq = qe[ia] + qe[ib], d = pp + qq;
+                sh.H[e] = hscale * ce[ia] * ce[ib] * sh.mom[ty[ia] + ty[ib]][d * (d + 1) / 2 + qq];
+            }
         }
         if constexpr (!TWOPASS && ZNSSD) {
             Acc<R> a_sd[P], a_fsd[P], a_f, a_f2;
             const R fm = (R)fmean, fin = (R)(1.0 / fnorm);
+            int blk = 0;
             for (int s = lane; s < a.nfull; s += 64) {
                 int dx, dy;
                 decode(a.table[s], dx, dy);
@@ -525,6 +720,16 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
                 for (int j = 0; j < P; ++j) {
                     a_sd[j].add(sd[j]);
                     a_fsd[j].add(fh * sd[j]);
+                }
+                if (++blk == kBlock) {
+                    blk = 0;
+                    a_f.fold();
+                    a_f2.fold();
+#pragma unroll
+                    for (int j = 0; j < P; ++j) {
+                        a_sd[j].fold();
+                        a_fsd[j].fold();
+                    }
                 }
             }
 #pragma unroll
@@ -542,13 +747,18 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
             }
         }
         __syncthreads();
-        if (lane == 0) {
-            int ok, pos;
-            ldlt_factor<P>(sh.H, sh.tr, sh.x, &ok, &pos);
-            sh.state = (ok && pos) ? 0 : 1;
+        int ok, pos;
+        if constexpr (P == 12) {
+            ldlt_factor_wave<P>(sh, lane, ok, pos);
+        } else { // 6 x 6: the serial form on one lane is as fast
+            if (lane == 0) {
+                ldlt_factor<P>(sh.H, sh.tr, sh.x, &ok, &pos);
+                sh.state = ok && pos;
+            }
+            __syncthreads();
+            ok = pos = sh.state;
         }
-        __syncthreads();
-        if (sh.state != 0) {
+        if (!(ok && pos)) {
             early = true;
             status = DIC_NUMERICAL_FAILURE;
         } else {
@@ -592,8 +802,24 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
 #pragma unroll
             for (int j = 2; j < 12; ++j) q[j] = (R)p[j];
 
+            // whole warped subset provably inside [0, W-1] x [0, H-1]: the
+            // per-sample bounds test can be skipped (the extent bounds the
+            // linear part over the disc and the quadratic part by r^2 / 2)
+            bool inside;
+            {
+                const double rr = (double)a.radius;
+                double ex = rr * sqrt((1.0 + p[2]) * (1.0 + p[2]) + p[3] * p[3]);
+                double ey = rr * sqrt(p[4] * p[4] + (1.0 + p[5]) * (1.0 + p[5]));
+                if constexpr (P == 12) {
+                    ex += 0.5 * rr * rr * (fabs(p[6]) + fabs(p[7]) + fabs(p[8]));
+                    ey += 0.5 * rr * rr * (fabs(p[9]) + fabs(p[10]) + fabs(p[11]));
+                }
+                const double x0 = (double)cx + p[0], y0 = (double)cy + p[1], m = 0.01;
+                inside = !wild && x0 - ex - m >= 0.0 && x0 + ex + m <= Wm1 && y0 - ey - m >= 0.0 && y0 + ey + m <= Hm1;
+            }
             bool oob = wild;
             Acc<R> s1, s2, sfg, sg[P];
+            int blk = 0;
             for (int s = wild ? a.nfull : lane; s < a.nfull; s += 64) {
                 int dx, dy;
                 decode(a.table[s], dx, dy);
@@ -610,7 +836,7 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
                         wx += p[6] * dx22 + p[7] * dxy + p[8] * dy22;
                         wy += p[9] * dx22 + p[10] * dxy + p[11] * dy22;
                     }
-                    if (!(wx >= 0.0 && wy >= 0.0 && wx <= Wm1 && wy <= Hm1)) {
+                    if (!inside && !(wx >= 0.0 && wy >= 0.0 && wx <= Wm1 && wy <= Hm1)) {
                         oob = true;
                         continue;
                     }
@@ -635,21 +861,27 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
                     tx = xl - fx;
                     ty = yl - fy;
                     // x' = ix + tx must lie in [0, W-1] (tx in [0, 1))
-                    if (ix < 0 || iy < 0 || ix > W - 1 || iy > Hh - 1 || (ix == W - 1 && tx > R(0)) ||
-                        (iy == Hh - 1 && ty > R(0))) {
+                    if (!inside && (ix < 0 || iy < 0 || ix > W - 1 || iy > Hh - 1 || (ix == W - 1 && tx > R(0)) ||
+                                    (iy == Hh - 1 && ty > R(0)))) {
                         oob = true;
                         continue;
                     }
                 }
-                const R g = sample_at<D, R, R>(cf, a.pitch, a.b, ix, iy, tx, ty);
+                R gxv = R(0), gyv = R(0), fv = R(0);
+                if constexpr (!TWOPASS) { // independent loads, issued before the gathers
+                    gxv = gxp[(size_t)y * W + x];
+                    gyv = gyp[(size_t)y * W + x];
+                    fv = (R)a.ref[(size_t)y * W + x];
+                }
+                const R g = sample_at<D, R>(crs, a.pitch, a.b, ix, iy, tx, ty);
                 if constexpr (TWOPASS) {
                     gbuf[s] = g;
                     s1.add(g);
                 } else {
                     R sd[P];
-                    steepest_descent<P, R>(gxp[(size_t)y * W + x], gyp[(size_t)y * W + x], (R)dx, (R)dy, sd);
+                    steepest_descent<P, R>(gxv, gyv, (R)dx, (R)dy, sd);
                     if constexpr (ZNSSD) {
-                        const R fh = ((R)a.ref[(size_t)y * W + x] - fm) * fin;
+                        const R fh = (fv - fm) * fin;
                         const R d = g - fm;
                         s1.add(d);
                         s2.add(d * d);
@@ -657,11 +889,19 @@ __global__ __launch_bounds__(64) void icgn_kernel(IcgnArgs a)
 #pragma unroll
                         for (int j = 0; j < P; ++j) sg[j].add(d * sd[j]);
                     } else { // SSD residual e = f - g, icgn.cpp:945-950
-                        const R e = (R)a.ref[(size_t)y * W + x] - g;
+                        const R e = fv - g;
                         s2.add(e * e);
 #pragma unroll
                         for (int j = 0; j < P; ++j) sg[j].add(e * sd[j]);
                     }
+                }
+                if (++blk == kBlock) {
+                    blk = 0;
+                    s1.fold();
+                    s2.fold();
+                    sfg.fold();
+#pragma unroll
+                    for (int j = 0; j < P; ++j) sg[j].fold();
                 }
             }
             if (__any(oob)) {
@@ -896,6 +1136,7 @@ hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s)
     a.roi = L.roi;
     a.table = L.table;
     a.nfull = L.table_count;
+    a.coeff_bytes = (int)L.coeff_bytes;
     a.radius = pr.subset_radius;
     a.max_it = pr.max_iterations;
     a.thr = pr.convergence_threshold;
