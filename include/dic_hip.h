@@ -162,6 +162,66 @@ int dic_icgn_solve_batch_device(dic_ctx *ctx, const dic_icgn_params *params, con
                                 const double *poi_xy_device, const dic_initial *init_device, int n,
                                 dic_result *out_device, void *stream);
 
+/* ---- Batched solving: POIs of many image pairs, with uneven subset radii and
+ * shape orders, in one launch captured in a hipGraph.  No counterpart in the
+ * reference, which solves pair after pair and POI after POI (the callers
+ * ReliabilityPropagation::propagate, reliability_propagation.cpp:233-354, and
+ * the per-pair workflow loop); each POI's arithmetic is the same as
+ * dic_icgn_solve_batch_device's, bit for bit.  All coefficient images of a
+ * batch share one B-spline degree and one precision; every array of an item
+ * is a device pointer.  The item arrays must stay valid while the batch is
+ * used; a batch is replayed with dic_batch_run as often as wanted (the inputs
+ * may change between runs, the array addresses may not). ---- */
+typedef struct dic_batch_item {
+    const dic_image *reference;
+    const dic_grad *gradient;
+    const dic_coeffs *deformed;
+    const double *poi_xy;         /* 2n doubles (device) */
+    const dic_initial *init;      /* n (device) */
+    dic_result *out;              /* n (device) */
+    int32_t n;
+    int32_t subset_radius;
+    int32_t shape_order;          /* 1 or 2 */
+    int32_t max_iterations;
+    double convergence_threshold;
+} dic_batch_item;
+
+enum { DIC_BATCH_GRID = 0, DIC_BATCH_PERSISTENT = 1 };
+enum { DIC_BATCH_ORDER_NATURAL = 0, DIC_BATCH_ORDER_COST = 1, DIC_BATCH_ORDER_INTERLEAVE = 2, DIC_BATCH_ORDER_XCD = 3 };
+
+/* Launch mapping chosen by dic_batch_autotune (or set by hand).  A batch is
+ * one launch per shape order present (second order first), replayed as one
+ * hipGraph.
+ *   mapping DIC_BATCH_GRID: one workgroup (one 64-lane wave) per POI;
+ *   mapping DIC_BATCH_PERSISTENT: waves_per_cu x CUs workgroups take `chunk`
+ *     POIs at a time from a queue counter until the queue is empty.
+ *   order: NATURAL = item after item; COST = items by descending estimated
+ *     cost per POI (subset samples x shape order); INTERLEAVE = round robin
+ *     over the items in runs of 256 POIs; XCD = natural order dealt so that
+ *     each of the 8 XCDs gets one contiguous eighth (grid mapping). */
+typedef struct dic_batch_tuning {
+    int32_t mapping;
+    int32_t waves_per_cu;
+    int32_t chunk;
+    int32_t order;
+} dic_batch_tuning;
+
+typedef struct dic_batch dic_batch;
+
+int dic_batch_create(dic_ctx *ctx, const dic_batch_item *items, int n_items, int objective, dic_batch **out);
+int dic_batch_set_tuning(dic_batch *b, const dic_batch_tuning *t);
+int dic_batch_get_tuning(const dic_batch *b, dic_batch_tuning *t);
+/* Total POIs and the hipGraph's launch count so far. */
+int dic_batch_info(const dic_batch *b, int64_t *total_pois, int64_t *runs);
+/* Replays the batch's hipGraph on `stream` (NULL = the context's stream). */
+int dic_batch_run(dic_batch *b, void *stream);
+/* Times every candidate (NULL = the built-in sweep) `reps` times on the
+ * context's stream, keeps the fastest (median) and reports it; the outputs
+ * hold the last run's (identical) results. */
+int dic_batch_autotune(dic_batch *b, const dic_batch_tuning *candidates, int n_candidates, int reps,
+                       dic_batch_tuning *best, float *best_ms);
+void dic_batch_destroy(dic_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

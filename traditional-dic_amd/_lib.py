@@ -39,6 +39,8 @@ EXPORTED_SYMBOLS = (
     "dic_icgn_solve_batch_device", "dic_buffer_alloc", "dic_buffer_free", "dic_buffer_upload",
     "dic_buffer_download", "dic_event_record", "dic_event_elapsed_ms", "dic_event_destroy",
     "dic_bspline_prefilter_into", "dic_ref_gradient_into",
+    "dic_batch_create", "dic_batch_set_tuning", "dic_batch_get_tuning", "dic_batch_info", "dic_batch_run",
+    "dic_batch_autotune", "dic_batch_destroy",
 )
 
 # Displacement2D + iterations (dic_result); InitialDisplacement (dic_initial)
@@ -66,6 +68,40 @@ class IcgnParams(ctypes.Structure):
         ("min_valid_samples", ctypes.c_int32),
         ("min_valid_sample_ratio", ctypes.c_double),
     ]
+
+
+BATCH_GRID, BATCH_PERSISTENT = 0, 1
+BATCH_ORDER_NATURAL, BATCH_ORDER_COST, BATCH_ORDER_INTERLEAVE, BATCH_ORDER_XCD = 0, 1, 2, 3
+
+
+class BatchItem(ctypes.Structure):
+    """dic_batch_item: one (pair, parameter set) of a batched launch (device arrays)."""
+    _fields_ = [
+        ("reference", ctypes.c_void_p),
+        ("gradient", ctypes.c_void_p),
+        ("deformed", ctypes.c_void_p),
+        ("poi_xy", ctypes.c_void_p),
+        ("init", ctypes.c_void_p),
+        ("out", ctypes.c_void_p),
+        ("n", ctypes.c_int32),
+        ("subset_radius", ctypes.c_int32),
+        ("shape_order", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int32),
+        ("convergence_threshold", ctypes.c_double),
+    ]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [
+        ("mapping", ctypes.c_int32),
+        ("waves_per_cu", ctypes.c_int32),
+        ("chunk", ctypes.c_int32),
+        ("order", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {"mapping": ("grid", "persistent")[self.mapping], "waves_per_cu": self.waves_per_cu,
+                "chunk": self.chunk, "order": ("natural", "cost", "interleave", "xcd")[self.order]}
 
 
 class DicError(RuntimeError):
@@ -111,6 +147,14 @@ _SIGNATURES = {
     "dic_event_record": ([_vp, ctypes.POINTER(_vp)], _i32),
     "dic_event_elapsed_ms": ([_vp, _vp, ctypes.POINTER(ctypes.c_float)], _i32),
     "dic_event_destroy": ([_vp], None),
+    "dic_batch_create": ([_vp, ctypes.POINTER(BatchItem), _i32, _i32, ctypes.POINTER(_vp)], _i32),
+    "dic_batch_set_tuning": ([_vp, ctypes.POINTER(BatchTuning)], _i32),
+    "dic_batch_get_tuning": ([_vp, ctypes.POINTER(BatchTuning)], _i32),
+    "dic_batch_info": ([_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)], _i32),
+    "dic_batch_run": ([_vp, _vp], _i32),
+    "dic_batch_autotune": ([_vp, ctypes.POINTER(BatchTuning), _i32, _i32, ctypes.POINTER(BatchTuning),
+                            ctypes.POINTER(ctypes.c_float)], _i32),
+    "dic_batch_destroy": ([_vp], None),
 }
 
 _lib = None
@@ -281,6 +325,76 @@ class Context:
         self.check(self.lib.dic_icgn_solve_batch_device(self.handle, ctypes.byref(params), ref.handle, grad.handle,
                                                         coeffs.handle, roi_ptr, xy_ptr, init_ptr, int(n), out_ptr,
                                                         stream))
+
+
+    def batch(self, items, objective: int = OBJECTIVE_ZNSSD) -> "Batch":
+        """items: sequence of dicts with keys reference, gradient, deformed (handles),
+        poi_xy, init, out (DeviceBuffer), n, subset_radius, shape_order,
+        max_iterations, convergence_threshold."""
+        arr = (BatchItem * max(1, len(items)))()
+        keep = []
+        for k, it in enumerate(items):
+            arr[k].reference = it["reference"].handle
+            arr[k].gradient = it["gradient"].handle
+            arr[k].deformed = it["deformed"].handle
+            arr[k].poi_xy = it["poi_xy"].ptr
+            arr[k].init = it["init"].ptr
+            arr[k].out = it["out"].ptr
+            arr[k].n = int(it["n"])
+            arr[k].subset_radius = int(it["subset_radius"])
+            arr[k].shape_order = int(it["shape_order"])
+            arr[k].max_iterations = int(it.get("max_iterations", 30))
+            arr[k].convergence_threshold = float(it.get("convergence_threshold", 1e-3))
+            keep.append(it)
+        h = _vp()
+        self.check(self.lib.dic_batch_create(self.handle, arr, len(items), int(objective), ctypes.byref(h)))
+        return Batch(self, h, keep)
+
+
+class Batch:
+    """dic_batch: one hipGraph-replayed launch over the POIs of many items."""
+
+    def __init__(self, ctx: Context, handle, items):
+        self.ctx, self.handle, self.items = ctx, handle, items
+
+    def set_tuning(self, mapping: int, waves_per_cu: int = 8, chunk: int = 1, order: int = BATCH_ORDER_COST):
+        t = BatchTuning(mapping, waves_per_cu, chunk, order)
+        self.ctx.check(self.ctx.lib.dic_batch_set_tuning(self.handle, ctypes.byref(t)))
+
+    def tuning(self) -> BatchTuning:
+        t = BatchTuning()
+        self.ctx.check(self.ctx.lib.dic_batch_get_tuning(self.handle, ctypes.byref(t)))
+        return t
+
+    def info(self):
+        n, r = ctypes.c_int64(), ctypes.c_int64()
+        self.ctx.check(self.ctx.lib.dic_batch_info(self.handle, ctypes.byref(n), ctypes.byref(r)))
+        return n.value, r.value
+
+    def run(self, stream: int | None = None):
+        self.ctx.check(self.ctx.lib.dic_batch_run(self.handle, stream))
+
+    def autotune(self, candidates=None, reps: int = 3):
+        best, ms = BatchTuning(), ctypes.c_float()
+        if candidates:
+            arr = (BatchTuning * len(candidates))(*[BatchTuning(*c) for c in candidates])
+            n = len(candidates)
+        else:
+            arr, n = None, 0
+        self.ctx.check(self.ctx.lib.dic_batch_autotune(self.handle, arr, n, int(reps), ctypes.byref(best),
+                                                       ctypes.byref(ms)))
+        return best, ms.value
+
+    def destroy(self):
+        if self.handle:
+            self.ctx.lib.dic_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:

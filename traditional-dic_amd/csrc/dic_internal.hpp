@@ -70,6 +70,54 @@ hipError_t launch_gradient_bspline(const dic_coeffs *c, int precision, void *gx,
 hipError_t launch_sample(const dic_coeffs *c, const double *xy, int n, double *val, double *grad, hipStream_t s);
 
 // icgn.hip
+// Kernel arguments of one (pair, parameter set): images, planes, table, POIs.
+struct IcgnArgs {
+    int W, H, pitch, b;
+    const float *ref;
+    const void *gx, *gy, *coeffs;
+    const uint8_t *roi;
+    const int *table;
+    int nfull;
+    int coeff_bytes; // size of the padded coefficient image (< 2^31)
+    int radius, max_it;
+    double thr;
+    int min_needed;
+    const double *xy;
+    const dic_initial *init;
+    dic_result *out;
+    int n;
+};
+
+// One descriptor of the persistent batch kernel.
+struct BatchPair {
+    IcgnArgs a;
+    int shape_order;
+    int reserved;
+};
+
+// Descriptors launched together: same shape order (and, for FP64 ZNSSD, the
+// same LDS size class); their items are items[first, first + count).
+struct BatchGroup {
+    int shape_order;
+    int max_nfull;
+    int first, count;
+};
+constexpr int kMaxBatchGroups = 64; // queue counters in the batch's 256-byte counter buffer
+
+struct BatchLaunch {
+    int objective, degree, precision;
+    const BatchPair *pairs;
+    const int2 *items; // (descriptor, POI), group after group
+    const BatchGroup *groups;
+    int ngroups;
+    bool persistent;   // work-queue kernels on a fixed grid, else one workgroup per item
+    int grid;          // persistent workgroups
+    int chunk;         // items per queue grab
+    unsigned *counter; // one queue counter per group (persistent)
+};
+hipError_t launch_batch(const BatchLaunch &L, hipStream_t s);
+hipError_t add_batch_nodes(const BatchLaunch &L, hipGraph_t g);
+
 struct IcgnLaunch {
     const dic_icgn_params *params;
     int width, height;

@@ -108,22 +108,6 @@ template <int P, typename R> __device__ __forceinline__ void steepest_descent(R 
     }
 }
 
-struct IcgnArgs {
-    int W, H, pitch, b;
-    const float *ref;
-    const void *gx, *gy, *coeffs;
-    const uint8_t *roi;
-    const int *table;
-    int nfull;
-    int coeff_bytes; // size of the padded coefficient image (< 2^31)
-    int radius, max_it;
-    double thr;
-    int min_needed;
-    const double *xy;
-    const dic_initial *init;
-    dic_result *out;
-    int n;
-};
 
 struct IcgnShared {
     double H[144];    // Hessian, then its LDL^T factor (lower triangle)
@@ -535,18 +519,16 @@ __device__ __forceinline__ void decode(int t, int &dx, int &dy)
 // The second-order mixed kernel asks for 4 waves per SIMD (128 VGPRs): left
 // alone the compiler takes ~160 VGPRs (3 waves) and runs ~15% slower; the
 // spill it needs lies outside the sample loops.
+// One POI on the calling wave (64 lanes, the whole workgroup): sh is the
+// workgroup's LDS record, gbuf (FP64 ZNSSD only) holds nfull doubles.
 template <int P, int OBJ, int D, typename R, bool MASKED>
-__global__ __launch_bounds__(64, (P == 12 && sizeof(R) == 4) ? 4 : 1) void icgn_kernel(IcgnArgs a)
+__device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, IcgnShared &sh, double *gbuf)
 {
     constexpr bool ZNSSD = OBJ == DIC_OBJECTIVE_ZNSSD;
     constexpr bool FP64 = sizeof(R) == 8;
     constexpr bool TWOPASS = ZNSSD && FP64;
-    __shared__ IcgnShared sh;
-    extern __shared__ double gbuf[];
 
     const int lane = threadIdx.x;
-    const int poi = blockIdx.x;
-    if (poi >= a.n) return;
     const R *__restrict__ gxp = (const R *)a.gx;
     const R *__restrict__ gyp = (const R *)a.gy;
     const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
@@ -1080,6 +1062,64 @@ __global__ __launch_bounds__(64, (P == 12 && sizeof(R) == 4) ? 4 : 1) void icgn_
 }
 
 template <int P, int OBJ, int D, typename R, bool MASKED>
+__global__ __launch_bounds__(64, (P == 12 && sizeof(R) == 4) ? 4 : 1) void icgn_kernel(IcgnArgs a)
+{
+    __shared__ IcgnShared sh;
+    extern __shared__ double gbuf[];
+    if ((int)blockIdx.x >= a.n) return;
+    solve_poi<P, OBJ, D, R, MASKED>(a, (int)blockIdx.x, sh, gbuf);
+}
+
+// Batched solving over many (pair, radius) descriptors: one launch per group
+// of descriptors with the same shape order (each compiled like the per-pair
+// kernel of that order; FP64 ZNSSD groups also share an LDS size class).
+// items[] lists (descriptor, POI) in the order the host chose.
+//   PERSISTENT = false: one workgroup (one wave) per item; the hardware
+//     dispatcher hands out items in workgroup order as slots free up.
+//   PERSISTENT = true: a fixed grid of workgroups takes `chunk` items at a
+//     time from a queue counter (one atomic per chunk) until none are left.
+// Each POI's result depends on nothing but its own inputs, so every mapping
+// and order gives the results of the one-launch-per-pair kernels.
+template <int P, int OBJ, int D, typename R>
+__device__ __forceinline__ void run_item(const BatchPair *__restrict__ pairs, const int2 *__restrict__ items,
+                                         unsigned u, IcgnShared &sh, double *gbuf)
+{
+    const int2 it = items[u];
+    const int pid = __builtin_amdgcn_readfirstlane(it.x);
+    const int poi = __builtin_amdgcn_readfirstlane(it.y);
+    solve_poi<P, OBJ, D, R, false>(pairs[pid].a, poi, sh, gbuf);
+}
+
+template <int P, int OBJ, int D, typename R, bool PERSISTENT>
+__global__ __launch_bounds__(64, (P == 12 && sizeof(R) == 4) ? 4 : 1) void icgn_batch_kernel(
+    const BatchPair *__restrict__ pairs, const int2 *__restrict__ items, int nitems, unsigned *counter, int chunk)
+{
+    __shared__ IcgnShared sh;
+    extern __shared__ double gbuf[];
+    if constexpr (!PERSISTENT) {
+        if ((int)blockIdx.x < nitems) run_item<P, OBJ, D, R>(pairs, items, blockIdx.x, sh, gbuf);
+    } else {
+        for (;;) {
+            unsigned t = 0;
+            if (threadIdx.x == 0) t = atomicAdd(counter, (unsigned)chunk);
+            t = (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl(t, 0, 64));
+            if (t >= (unsigned)nitems) break;
+            const unsigned end = min(t + (unsigned)chunk, (unsigned)nitems);
+            for (unsigned u = t; u < end; ++u) {
+                __syncthreads();
+                run_item<P, OBJ, D, R>(pairs, items, u, sh, gbuf);
+            }
+        }
+    }
+}
+
+__global__ void fill_u32_kernel(unsigned *p, int n, unsigned v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <int P, int OBJ, int D, typename R, bool MASKED>
 hipError_t launch_variant(const IcgnArgs &args, hipStream_t s)
 {
     constexpr bool TWOPASS = OBJ == DIC_OBJECTIVE_ZNSSD && sizeof(R) == 8;
@@ -1158,6 +1198,131 @@ hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s)
                    : dispatch_degree<6, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, s);
     return ssd ? dispatch_degree<12, DIC_OBJECTIVE_SSD>(a, L.degree, L.precision, masked, s)
                : dispatch_degree<12, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, s);
+}
+
+
+namespace {
+template <int P, int OBJ, int D, bool PERS>
+const void *batch_func_prec(int precision)
+{
+    return precision == DIC_PRECISION_FP64 ? (const void *)icgn_batch_kernel<P, OBJ, D, double, PERS>
+                                           : (const void *)icgn_batch_kernel<P, OBJ, D, float, PERS>;
+}
+template <int P, int OBJ, bool PERS>
+const void *batch_func_deg(int degree, int precision)
+{
+    switch (degree) {
+    case 1: return batch_func_prec<P, OBJ, 1, PERS>(precision);
+    case 3: return batch_func_prec<P, OBJ, 3, PERS>(precision);
+    case 5: return batch_func_prec<P, OBJ, 5, PERS>(precision);
+    default: return nullptr;
+    }
+}
+template <int P, bool PERS>
+const void *batch_func_obj(int objective, int degree, int precision)
+{
+    return objective == DIC_OBJECTIVE_SSD ? batch_func_deg<P, DIC_OBJECTIVE_SSD, PERS>(degree, precision)
+                                          : batch_func_deg<P, DIC_OBJECTIVE_ZNSSD, PERS>(degree, precision);
+}
+const void *batch_func(int order, bool persistent, int objective, int degree, int precision)
+{
+    if (order == 2)
+        return persistent ? batch_func_obj<12, true>(objective, degree, precision)
+                          : batch_func_obj<12, false>(objective, degree, precision);
+    return persistent ? batch_func_obj<6, true>(objective, degree, precision)
+                      : batch_func_obj<6, false>(objective, degree, precision);
+}
+
+struct Launch1 { // one group's launch
+    const void *func;
+    size_t lds;
+    const int2 *items;
+    int nitems;
+    unsigned *counter;
+};
+
+hipError_t batch_launch_of(const BatchLaunch &L, int g, Launch1 &o)
+{
+    const BatchGroup &G = L.groups[g];
+    const bool twopass = L.objective == DIC_OBJECTIVE_ZNSSD && L.precision == DIC_PRECISION_FP64;
+    o.func = batch_func(G.shape_order, L.persistent, L.objective, L.degree, L.precision);
+    if (!o.func) return hipErrorInvalidValue;
+    o.lds = twopass ? sizeof(double) * (size_t)G.max_nfull : 0;
+    o.items = L.items + G.first;
+    o.nitems = G.count;
+    o.counter = L.counter + g;
+    if (o.lds > 48 * 1024) return hipFuncSetAttribute(o.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)o.lds);
+    return hipSuccess;
+}
+} // namespace
+
+hipError_t launch_batch(const BatchLaunch &L, hipStream_t s)
+{
+    hipError_t e;
+    if (L.persistent) {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, L.counter, L.ngroups, 0u);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    for (int g = 0; g < L.ngroups; ++g) {
+        Launch1 l;
+        e = batch_launch_of(L, g, l);
+        if (e != hipSuccess) return e;
+        if (l.nitems <= 0) continue;
+        const BatchPair *pairs = L.pairs;
+        const int2 *items = l.items;
+        int nitems = l.nitems, chunk = L.chunk;
+        unsigned *counter = l.counter;
+        void *args[] = {(void *)&pairs, (void *)&items, (void *)&nitems, (void *)&counter, (void *)&chunk};
+        const unsigned grid = L.persistent ? (unsigned)L.grid : (unsigned)nitems;
+        e = hipLaunchKernel(l.func, dim3(grid), dim3(64), args, l.lds, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t add_batch_nodes(const BatchLaunch &L, hipGraph_t g)
+{
+    hipError_t e;
+    hipGraphNode_t prev = nullptr;
+    if (L.persistent) { // the queue counters are cleared by a kernel node, not a memset node
+        unsigned *counter = L.counter;
+        int n = L.ngroups;
+        unsigned zero = 0;
+        void *rargs[] = {(void *)&counter, (void *)&n, (void *)&zero};
+        hipKernelNodeParams rp{};
+        rp.func = (void *)fill_u32_kernel;
+        rp.gridDim = dim3(1);
+        rp.blockDim = dim3(64);
+        rp.sharedMemBytes = 0;
+        rp.kernelParams = rargs;
+        rp.extra = nullptr;
+        e = hipGraphAddKernelNode(&prev, g, nullptr, 0, &rp);
+        if (e != hipSuccess) return e;
+    }
+    for (int k = 0; k < L.ngroups; ++k) { // groups in the host's order (heavier first)
+        Launch1 l;
+        e = batch_launch_of(L, k, l);
+        if (e != hipSuccess) return e;
+        if (l.nitems <= 0) continue;
+        const BatchPair *pairs = L.pairs;
+        const int2 *items = l.items;
+        int nitems = l.nitems, chunk = L.chunk;
+        unsigned *counter = l.counter;
+        void *kargs[] = {(void *)&pairs, (void *)&items, (void *)&nitems, (void *)&counter, (void *)&chunk};
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void *>(l.func);
+        kp.gridDim = dim3(L.persistent ? (unsigned)L.grid : (unsigned)nitems);
+        kp.blockDim = dim3(64);
+        kp.sharedMemBytes = (unsigned)l.lds;
+        kp.kernelParams = kargs;
+        kp.extra = nullptr;
+        hipGraphNode_t node;
+        e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
+        if (e != hipSuccess) return e;
+        prev = node;
+    }
+    return hipSuccess;
 }
 
 } // namespace dic
