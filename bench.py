@@ -135,6 +135,67 @@ def batched_leg(pkg, ctx, synth, reps=3):
     return out
 
 
+def serving_leg(pkg, ctx, synth, duration_s=6.0, rate=20.0, seed=4000):
+    """Poisson request stream (rate req/s for duration_s): each request is an
+    incremental uniaxial-strain sequence of U{8..64} frames at 1024^2 or
+    2448x2048 (50/50) against its resident reference frame; radius 15, first
+    order, POI step 7, quintic, mixed precision; 200 GB admission budget.
+    Requests of one size share one rendered, pinned frame pool (64 frames,
+    strain 2e-4 per frame, noise sigma 1.5)."""
+    t0 = time.time()
+    pools = {}
+    for (w, h) in ((1024, 1024), (2448, 2048)):
+        bufs = [ctx.host_array((h, w), np.float32) for _ in range(65)]
+        ref, frames = synth.strain_sequence(w, h, 64, seed=seed + w, out=bufs)
+        xy = synth.strain_pois(w, h)
+        pools[(w, h)] = (ref, frames, xy, np.zeros(xy.shape[0], pkg.INITIAL_DTYPE))
+    log(f"serving frame pools rendered in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(seed)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration_s:
+            break
+        size = (1024, 1024) if rng.random() < 0.5 else (2448, 2048)
+        arrivals.append((t, size, int(rng.integers(8, 65))))
+    srv = ctx.server(radius=15, shape_order=1, degree=5, precision=pkg.PRECISION_MIXED, max_active=256,
+                     max_width=2448, max_height=2048, hbm_budget=200 * 10**9)
+    # warm the kernels and allocations with one short request
+    ref, frames, xy, ini = pools[(1024, 1024)]
+    srv.submit(ref, frames[:2], xy, ini).wait(120.0)
+    reqs = []
+    start = time.perf_counter()
+    for (ta, size, nf) in arrivals:
+        delay = ta - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        ref, frames, xy, ini = pools[size]
+        reqs.append(srv.submit(ref, frames[:nf], xy, ini))
+    for r in reqs:
+        r.wait(600.0)
+    st = srv.stats()
+    frame_lat = np.concatenate([r.frame_done_s - r.arrival for r in reqs])
+    req_lat = np.array([r.frame_done_s[-1] - r.arrival for r in reqs])
+    first = min(r.arrival for r in reqs)
+    last = max(r.frame_done_s[-1] for r in reqs)
+    pois = int(sum(r.results.size for r in reqs))
+    ok = int(sum(int(((r.results["status"] == pkg.SUCCESS) & (r.results["valid"] == 1)).sum()) for r in reqs))
+    srv.close()
+    out = {"workload": (f"Poisson {rate:g} req/s for {duration_s:g} s, {len(reqs)} requests of 8..64 frames at "
+                        f"1024^2 / 2448x2048, radius 15, first order, step 7, quintic, mixed, 200 GB budget"),
+           "requests": len(reqs), "frames": int(frame_lat.size), "pois": pois, "converged": ok,
+           "span_s": round(last - first, 3), "converged_pois_per_s": round(ok / (last - first), 1),
+           "offered_pois_per_s": round(pois / duration_s, 1),
+           "frame_latency_ms": {"p50": round(float(np.percentile(frame_lat, 50)) * 1e3, 2),
+                                "p99": round(float(np.percentile(frame_lat, 99)) * 1e3, 2)},
+           "request_latency_ms": {"p50": round(float(np.percentile(req_lat, 50)) * 1e3, 2),
+                                  "p99": round(float(np.percentile(req_lat, 99)) * 1e3, 2)},
+           "ticks": st["ticks"], "max_active_seen": st["max_active_seen"],
+           "peak_hbm_gb": round(st["peak_bytes_held"] / 1e9, 2)}
+    log(f"serving: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +204,8 @@ def main():
     ap.add_argument("--cpu-fraction", type=int, default=8, help="CPU baseline solves every n-th POI")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-pair batched leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the request-stream serving leg")
+    ap.add_argument("--serving-seconds", type=float, default=6.0)
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)")
@@ -264,6 +327,7 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     batched = None if args.no_batched else batched_leg(pkg, ctx, synth)
+    serving = None if args.no_serving else serving_leg(pkg, ctx, synth, duration_s=args.serving_seconds)
 
     line = {
         "metric": "converged POIs/s (FP32 vs FP64 ICGN, |du|<1e-3 px); p50/p99 request latency",
@@ -293,6 +357,7 @@ def main():
         "mixed_vs_fp64_max_px": float(np.max(dmix[ok64], initial=0.0)),
         "cpu_baseline": cpu,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -222,6 +222,79 @@ int dic_batch_autotune(dic_batch *b, const dic_batch_tuning *candidates, int n_c
                        dic_batch_tuning *best, float *best_ms);
 void dic_batch_destroy(dic_batch *b);
 
+/* ---- Pinned host memory: frames handed to the server from pinned memory are
+ * uploaded by the copy engines while the GPU solves earlier frames. ---- */
+int dic_host_alloc(dic_ctx *ctx, size_t bytes, void **ptr);
+void dic_host_free(void *ptr);
+
+/* ---- Serving: a request loop with continuous batching.  No counterpart in the
+ * reference (whose workflows correlate one frame pair at a time,
+ * python/traditional_dic/workflows/subset_2d.py); every frame's arithmetic is
+ * the IC-GN of dic_icgn_solve_batch_device.
+ *
+ * A request is an incremental frame sequence against one reference frame:
+ * frame f is correlated with the reference at the request's POIs, starting
+ * from the request's initial guesses (f = 0) or from frame f-1's results
+ * (u, v, gradients; confidence = correlation).  dic_server_submit returns at
+ * once; a worker thread admits pending requests in arrival order while the
+ * number of active requests stays below max_active and the device bytes they
+ * hold (reference, gradients, two frame and coefficient buffers, POI,
+ * initial-guess and result arrays) stay within hbm_budget.  Each tick is one
+ * launch over the next frame of every active request; a request that arrives
+ * while a tick runs joins the next tick, and a finished request leaves and
+ * frees its slot.  Two ticks are in flight: uploads and prefilters of tick
+ * k+1 (copy engines, prep stream) overlap the solve of tick k.  Each POI's
+ * result depends only on its own inputs, so a request's results are the same
+ * served alone or in any mix.  The caller keeps every array of a request
+ * valid until dic_server_wait reports it done. ---- */
+typedef struct dic_server_config {
+    uint64_t hbm_budget;          /* device bytes held by admitted requests (0 = no cap) */
+    int32_t max_active;           /* requests served at once */
+    int32_t max_width, max_height; /* largest frame accepted */
+    int32_t subset_radius;
+    int32_t shape_order;          /* 1 or 2 */
+    int32_t max_iterations;
+    double convergence_threshold;
+    int32_t degree;               /* B-spline degree 1, 3 or 5 */
+    int32_t precision;            /* DIC_PRECISION_* */
+    int32_t objective;            /* DIC_OBJECTIVE_* */
+    int32_t reserved;
+} dic_server_config;
+
+typedef struct dic_request {
+    const float *reference;       /* width x height, host */
+    int32_t width, height;
+    int32_t n_frames;
+    int32_t n;                    /* POIs */
+    const float *const *frames;   /* n_frames host pointers, width x height each */
+    const double *poi_xy;         /* 2n, host */
+    const dic_initial *init;      /* n, host: guesses for frame 0 */
+    dic_result *results;          /* n_frames x n, host, written as frames finish */
+    double *frame_done_s;         /* n_frames completion times (dic_server_now clock), or NULL */
+} dic_request;
+
+typedef struct dic_server_stats {
+    int64_t submitted, completed, frames_done, pois_done, ticks;
+    int32_t active, pending, max_active_seen, reserved;
+    uint64_t bytes_held, peak_bytes_held;
+} dic_server_stats;
+
+typedef struct dic_server dic_server;
+
+int dic_server_create(dic_ctx *ctx, const dic_server_config *cfg, dic_server **out);
+/* Queues a request; *id identifies it; *arrival_s (may be NULL) is its arrival time. */
+int dic_server_submit(dic_server *s, const dic_request *req, int64_t *id, double *arrival_s);
+/* Blocks until request id is done (DIC_OK), the server failed, or timeout_s
+ * (< 0: no limit) passed (DIC_ERR_UNSUPPORTED). */
+int dic_server_wait(dic_server *s, int64_t id, double timeout_s);
+int dic_server_frames_done(dic_server *s, int64_t id, int *frames);
+int dic_server_stats_get(dic_server *s, dic_server_stats *st);
+/* Seconds since dic_server_create on the clock of arrival and completion times. */
+double dic_server_now(const dic_server *s);
+const char *dic_server_error(const dic_server *s);
+/* Finishes all submitted requests, then stops the worker and frees everything. */
+void dic_server_destroy(dic_server *s);
+
 #ifdef __cplusplus
 }
 #endif

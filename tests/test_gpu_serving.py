@@ -1,0 +1,117 @@
+"""Serving loop (dic_server_*): continuous batching of incremental frame
+sequences with HBM-budget admission.
+
+Each POI's result depends only on its own inputs, so a request's results
+must be bit-identical served alone and served in a mix of requests that
+arrive while it runs.  Frame f's guesses are frame f-1's results, so the
+served FP64 results are checked against the same chain of direct
+dic_icgn_solve_batch calls (identical statuses, |du|, |dv| <= 1e-9 px) and
+frame 0 against the CPU oracle (|du|, |dv| <= 1e-5 px).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _report(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def seqs(synth, pkg):
+    out = []
+    for k, (w, h, nf) in enumerate([(256, 256, 5), (320, 256, 3), (256, 256, 4), (288, 320, 6)]):
+        ref, frames = synth.strain_sequence(w, h, nf, seed=4100 + k)
+        xy = synth.strain_pois(w, h)
+        ini = np.zeros(xy.shape[0], pkg.INITIAL_DTYPE)
+        out.append((ref, frames, xy, ini))
+    return out
+
+
+def _direct_chain(pkg, ctx, seq, precision):
+    ref, frames, xy, ini = seq
+    cfg = pkg.SubsetConfig(subset_radius=15)
+    params = pkg.make_params(cfg, precision)
+    dref = ctx.upload_image(ref)
+    grad = ctx.ref_gradient(dref, 0, 5, 3, precision)
+    res, cur = [], ini.copy()
+    for f in frames:
+        co = ctx.prefilter(ctx.upload_image(f), 5, 3, True, precision)
+        r = ctx.solve(params, dref, grad, co, xy, cur)
+        res.append(r)
+        cur = np.zeros(xy.shape[0], pkg.INITIAL_DTYPE)
+        for k, name in enumerate(("u", "v", "du_dx", "du_dy", "dv_dx", "dv_dy")):
+            cur[name] = r["p"][:, k]
+        cur["confidence"] = r["correlation"]
+    return np.stack(res)
+
+
+def test_served_fp64_matches_direct_chain_and_oracle(pkg, ctx, orc, seqs):
+    srv = ctx.server(radius=15, shape_order=1, degree=5, precision=pkg.PRECISION_FP64, max_active=8,
+                     max_width=512, max_height=512)
+    req = srv.submit(*seqs[0])
+    got = req.wait(60.0)
+    srv.close()
+    want = _direct_chain(pkg, ctx, seqs[0], pkg.PRECISION_FP64)
+    assert np.array_equal(got["status"], want["status"])
+    ok = want["status"] == 0
+    d = np.abs(got["p"][..., :2] - want["p"][..., :2])[ok].max(initial=0)
+    _report(f"served vs direct chain: {ok.mean():.4f} ok, max |du| {d:.2e}")
+    assert d <= 1e-9 and ok.mean() > 0.98
+    ref, frames, xy, ini = seqs[0]
+    pre = orc.Precomputed(frames[0], degree=5)
+    gx, gy = orc.reference_gradient_cache(ref)
+    cpu = orc.icgn_solve(ref, pre, gx, gy, xy, ini, radius=15, max_iterations=30, threshold=1e-3, shape_order=1)
+    assert np.array_equal(got["status"][0], cpu["status"])
+    okc = cpu["status"] == 0
+    assert np.abs(got["p"][0, okc, :2] - cpu["p"][okc, :2]).max(initial=0) <= 1e-5
+    # strain truth: u = f * 2e-4 * (x - xc)
+    xc = 0.5 * (ref.shape[1] - 1)
+    for f in range(len(frames)):
+        u = (f + 1) * 2e-4 * (xy[:, 0] - xc)
+        assert np.abs(got["p"][f, ok[f], 0] - u[ok[f]]).max() < 0.05
+
+
+def test_alone_and_in_mix_are_identical(pkg, ctx, seqs):
+    srv = ctx.server(radius=15, shape_order=1, degree=5, precision=pkg.PRECISION_MIXED, max_active=8,
+                     max_width=512, max_height=512)
+    alone = srv.submit(*seqs[0]).wait(60.0).copy()
+    # the same request again, in a mix: others are submitted before and after it
+    r1 = srv.submit(*seqs[1])
+    r2 = srv.submit(*seqs[2])
+    again = srv.submit(*seqs[0])
+    r3 = srv.submit(*seqs[3])
+    for r in (r1, r2, again, r3):
+        r.wait(60.0)
+    st = srv.stats()
+    _report(f"mix stats: {st}")
+    assert st["max_active_seen"] >= 2 and st["completed"] == 5
+    assert again.results.tobytes() == alone.tobytes()
+    assert np.all(np.isfinite(again.frame_done_s)) and np.all(np.diff(again.frame_done_s) >= 0)
+    srv.close()
+
+
+def test_budget_admission_and_refusal(pkg, ctx, seqs):
+    # room for one 256 x 256 request at a time: requests are served one after another
+    one = 256 * 256 * (3 * 4 + 2 * 4) + 2 * 262 * 262 * 4 + 1000 * (16 + 56 + 2 * 136) + 64 * 1024
+    srv = ctx.server(radius=15, precision=pkg.PRECISION_MIXED, max_active=8, max_width=512, max_height=512,
+                     hbm_budget=one + 64 * 1024)
+    reqs = [srv.submit(*seqs[0]), srv.submit(*seqs[2])]
+    big = srv.submit(*seqs[3])  # 288 x 320: never fits
+    for r in reqs:
+        r.wait(60.0)
+    with pytest.raises(pkg.DicError):
+        big.wait(60.0)
+    st = srv.stats()
+    _report(f"budget stats: {st}")
+    assert st["max_active_seen"] == 1 and st["peak_bytes_held"] <= one + 64 * 1024
+    assert st["completed"] == 2
+    srv.close()
+
+
+def test_server_rejects_bad_config(pkg, ctx):
+    with pytest.raises(pkg.DicError):
+        ctx.server(shape_order=3)
+    with pytest.raises(pkg.DicError):
+        ctx.server(max_active=0)

@@ -41,6 +41,8 @@ EXPORTED_SYMBOLS = (
     "dic_bspline_prefilter_into", "dic_ref_gradient_into",
     "dic_batch_create", "dic_batch_set_tuning", "dic_batch_get_tuning", "dic_batch_info", "dic_batch_run",
     "dic_batch_autotune", "dic_batch_destroy",
+    "dic_host_alloc", "dic_host_free", "dic_server_create", "dic_server_submit", "dic_server_wait",
+    "dic_server_frames_done", "dic_server_stats_get", "dic_server_now", "dic_server_error", "dic_server_destroy",
 )
 
 # Displacement2D + iterations (dic_result); InitialDisplacement (dic_initial)
@@ -104,6 +106,51 @@ class BatchTuning(ctypes.Structure):
                 "chunk": self.chunk, "order": ("natural", "cost", "interleave", "xcd")[self.order]}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget", ctypes.c_uint64),
+        ("max_active", ctypes.c_int32),
+        ("max_width", ctypes.c_int32),
+        ("max_height", ctypes.c_int32),
+        ("subset_radius", ctypes.c_int32),
+        ("shape_order", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int32),
+        ("convergence_threshold", ctypes.c_double),
+        ("degree", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("objective", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("reference", ctypes.c_void_p),
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("n_frames", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("frames", ctypes.c_void_p),
+        ("poi_xy", ctypes.c_void_p),
+        ("init", ctypes.c_void_p),
+        ("results", ctypes.c_void_p),
+        ("frame_done_s", ctypes.c_void_p),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("frames_done", ctypes.c_int64),
+        ("pois_done", ctypes.c_int64), ("ticks", ctypes.c_int64),
+        ("active", ctypes.c_int32), ("pending", ctypes.c_int32), ("max_active_seen", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("bytes_held", ctypes.c_uint64), ("peak_bytes_held", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class DicError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"[dic_hip error {code}] {message}")
@@ -155,6 +202,17 @@ _SIGNATURES = {
     "dic_batch_autotune": ([_vp, ctypes.POINTER(BatchTuning), _i32, _i32, ctypes.POINTER(BatchTuning),
                             ctypes.POINTER(ctypes.c_float)], _i32),
     "dic_batch_destroy": ([_vp], None),
+    "dic_host_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], _i32),
+    "dic_host_free": ([_vp], None),
+    "dic_server_create": ([_vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)], _i32),
+    "dic_server_submit": ([_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_int64),
+                           ctypes.POINTER(ctypes.c_double)], _i32),
+    "dic_server_wait": ([_vp, ctypes.c_int64, ctypes.c_double], _i32),
+    "dic_server_frames_done": ([_vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int)], _i32),
+    "dic_server_stats_get": ([_vp, ctypes.POINTER(ServerStats)], _i32),
+    "dic_server_now": ([_vp], ctypes.c_double),
+    "dic_server_error": ([_vp], ctypes.c_char_p),
+    "dic_server_destroy": ([_vp], None),
 }
 
 _lib = None
@@ -349,6 +407,110 @@ class Context:
         h = _vp()
         self.check(self.lib.dic_batch_create(self.handle, arr, len(items), int(objective), ctypes.byref(h)))
         return Batch(self, h, keep)
+
+
+    def host_array(self, shape, dtype=np.float32) -> np.ndarray:
+        """A numpy array in pinned host memory (dic_host_alloc), freed with the array."""
+        dt = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dt.itemsize
+        p = _vp()
+        self.check(self.lib.dic_host_alloc(self.handle, max(1, nbytes), ctypes.byref(p)))
+        buf = (ctypes.c_char * max(1, nbytes)).from_address(p.value)
+        arr = np.frombuffer(buf, dtype=np.uint8, count=nbytes).view(dt).reshape(shape)
+        _PINNED[id(buf)] = p.value
+        import weakref
+        weakref.finalize(buf, _free_pinned, self.lib, id(buf))
+        return arr
+
+    def server(self, radius=15, shape_order=1, degree=5, precision=PRECISION_MIXED, objective=OBJECTIVE_ZNSSD,
+               max_active=64, max_width=4096, max_height=4096, hbm_budget=0, max_iterations=30,
+               convergence_threshold=1e-3) -> "Server":
+        cfg = ServerConfig(int(hbm_budget), int(max_active), int(max_width), int(max_height), int(radius),
+                           int(shape_order), int(max_iterations), float(convergence_threshold), int(degree),
+                           int(precision), int(objective), 0)
+        h = _vp()
+        self.check(self.lib.dic_server_create(self.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        return Server(self, h)
+
+
+_PINNED = {}
+
+
+def _free_pinned(lib, key):
+    p = _PINNED.pop(key, None)
+    if p:
+        lib.dic_host_free(p)
+
+
+class ServedRequest:
+    """A submitted request: results[f, i] fill in as frames finish."""
+
+    def __init__(self, server, rid, arrival, results, done_s, keep):
+        self.server, self.id, self.arrival, self.results, self.frame_done_s = server, rid, arrival, results, done_s
+        self._keep = keep
+
+    def wait(self, timeout_s: float = -1.0):
+        rc = self.server.ctx.lib.dic_server_wait(self.server.handle, self.id, float(timeout_s))
+        if rc != DIC_OK:
+            raise DicError(rc, self.server.error())
+        return self.results
+
+    def frames_done(self) -> int:
+        f = ctypes.c_int()
+        self.server.ctx.check(self.server.ctx.lib.dic_server_frames_done(self.server.handle, self.id, ctypes.byref(f)))
+        return f.value
+
+
+class Server:
+    """dic_server: continuous-batching request loop on one device."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.handle = ctx, handle
+
+    def submit(self, reference: np.ndarray, frames, poi_xy: np.ndarray, init: np.ndarray) -> ServedRequest:
+        ref = np.ascontiguousarray(reference, np.float32)
+        fr = [np.ascontiguousarray(f, np.float32) for f in frames]
+        for f in fr:
+            if f.shape != ref.shape:
+                raise ValueError("every frame must have the reference's shape")
+        xy = np.ascontiguousarray(poi_xy, np.float64).reshape(-1, 2)
+        ini = np.ascontiguousarray(init, INITIAL_DTYPE)
+        n = xy.shape[0]
+        if ini.shape[0] != n:
+            raise ValueError("one initial guess per POI is required")
+        ptrs = (ctypes.c_void_p * len(fr))(*[f.ctypes.data for f in fr])
+        results = np.zeros((len(fr), n), RESULT_DTYPE)
+        done = np.full(len(fr), np.nan)
+        req = Request(ref.ctypes.data, ref.shape[1], ref.shape[0], len(fr), n, ctypes.cast(ptrs, ctypes.c_void_p),
+                      xy.ctypes.data, ini.ctypes.data, results.ctypes.data, done.ctypes.data)
+        rid, arr = ctypes.c_int64(), ctypes.c_double()
+        rc = self.ctx.lib.dic_server_submit(self.handle, ctypes.byref(req), ctypes.byref(rid), ctypes.byref(arr))
+        if rc != DIC_OK:
+            raise DicError(rc, self.error() or "request refused")
+        return ServedRequest(self, rid.value, arr.value, results, done, (ref, fr, ptrs, xy, ini))
+
+    def now(self) -> float:
+        return self.ctx.lib.dic_server_now(self.handle)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self.ctx.check(self.ctx.lib.dic_server_stats_get(self.handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def error(self) -> str:
+        e = self.ctx.lib.dic_server_error(self.handle)
+        return e.decode() if e else ""
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.dic_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

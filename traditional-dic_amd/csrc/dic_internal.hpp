@@ -116,6 +116,19 @@ struct BatchLaunch {
     unsigned *counter; // one queue counter per group (persistent)
 };
 hipError_t launch_batch(const BatchLaunch &L, hipStream_t s);
+
+// One launch over descriptors of one shape order with contiguous POI ranges
+// [offsets[k], offsets[k+1]) (serving ticks).
+struct RangeLaunch {
+    int objective, degree, precision, shape_order;
+    const BatchPair *pairs; // device
+    const int *offsets;     // device, npairs + 1 prefix sums
+    int npairs;
+    int total;              // offsets[npairs]
+    int max_nfull;
+};
+hipError_t launch_range(const RangeLaunch &L, hipStream_t s);
+hipError_t launch_init_from_results(const dic_result *prev, dic_initial *init, int n, hipStream_t s);
 hipError_t add_batch_nodes(const BatchLaunch &L, hipGraph_t g);
 
 struct IcgnLaunch {

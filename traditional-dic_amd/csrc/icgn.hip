@@ -1113,6 +1113,45 @@ __global__ __launch_bounds__(64, (P == 12 && sizeof(R) == 4) ? 4 : 1) void icgn_
     }
 }
 
+// Serving ticks: descriptors own contiguous POI ranges [offsets[k],
+// offsets[k+1]); workgroup g finds its descriptor by a binary search over
+// the (npairs + 1) prefix offsets, so a tick needs no per-POI item table.
+template <int P, int OBJ, int D, typename R>
+__global__ __launch_bounds__(64, (P == 12 && sizeof(R) == 4) ? 4 : 1) void icgn_range_kernel(
+    const BatchPair *__restrict__ pairs, const int *__restrict__ offsets, int npairs)
+{
+    __shared__ IcgnShared sh;
+    extern __shared__ double gbuf[];
+    const int g = (int)blockIdx.x;
+    if (g >= offsets[npairs]) return;
+    int lo = 0, hi = npairs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (offsets[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const int pid = __builtin_amdgcn_readfirstlane(lo);
+    solve_poi<P, OBJ, D, R, false>(pairs[pid].a, g - offsets[pid], sh, gbuf);
+}
+
+// Next frame's initial guesses from this frame's results (u, v, the four
+// displacement gradients and the correlation as confidence).
+__global__ void init_from_results_kernel(const dic_result *__restrict__ prev, dic_initial *__restrict__ init, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const dic_result r = prev[i];
+    dic_initial o;
+    o.u = r.p[0];
+    o.v = r.p[1];
+    o.du_dx = r.p[2];
+    o.du_dy = r.p[3];
+    o.dv_dx = r.p[4];
+    o.dv_dy = r.p[5];
+    o.confidence = r.correlation;
+    init[i] = o;
+}
+
 __global__ void fill_u32_kernel(unsigned *p, int n, unsigned v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1323,6 +1362,58 @@ hipError_t add_batch_nodes(const BatchLaunch &L, hipGraph_t g)
         prev = node;
     }
     return hipSuccess;
+}
+
+
+namespace {
+template <int P, int OBJ, int D>
+const void *range_func_prec(int precision)
+{
+    return precision == DIC_PRECISION_FP64 ? (const void *)icgn_range_kernel<P, OBJ, D, double>
+                                           : (const void *)icgn_range_kernel<P, OBJ, D, float>;
+}
+template <int P, int OBJ>
+const void *range_func_deg(int degree, int precision)
+{
+    switch (degree) {
+    case 1: return range_func_prec<P, OBJ, 1>(precision);
+    case 3: return range_func_prec<P, OBJ, 3>(precision);
+    case 5: return range_func_prec<P, OBJ, 5>(precision);
+    default: return nullptr;
+    }
+}
+template <int P>
+const void *range_func_obj(int objective, int degree, int precision)
+{
+    return objective == DIC_OBJECTIVE_SSD ? range_func_deg<P, DIC_OBJECTIVE_SSD>(degree, precision)
+                                          : range_func_deg<P, DIC_OBJECTIVE_ZNSSD>(degree, precision);
+}
+} // namespace
+
+hipError_t launch_range(const RangeLaunch &L, hipStream_t s)
+{
+    if (L.total <= 0) return hipSuccess;
+    const void *func = L.shape_order == 2 ? range_func_obj<12>(L.objective, L.degree, L.precision)
+                                          : range_func_obj<6>(L.objective, L.degree, L.precision);
+    if (!func) return hipErrorInvalidValue;
+    const bool twopass = L.objective == DIC_OBJECTIVE_ZNSSD && L.precision == DIC_PRECISION_FP64;
+    const size_t lds = twopass ? sizeof(double) * (size_t)L.max_nfull : 0;
+    if (lds > 48 * 1024) {
+        hipError_t e = hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const BatchPair *pairs = L.pairs;
+    const int *offsets = L.offsets;
+    int npairs = L.npairs;
+    void *args[] = {(void *)&pairs, (void *)&offsets, (void *)&npairs};
+    return hipLaunchKernel(func, dim3((unsigned)L.total), dim3(64), args, lds, s);
+}
+
+hipError_t launch_init_from_results(const dic_result *prev, dic_initial *init, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(init_from_results_kernel, dim3((n + 255) / 256), dim3(256), 0, s, prev, init, n);
+    return hipGetLastError();
 }
 
 } // namespace dic

@@ -253,3 +253,28 @@ def batched_pairs(n_pairs: int = 64, sizes=(256, 512, 1024, 2048, 4096), seed0: 
         xy = xy[solvable_mask(xy, n, n, radius)]
         cases.append(PairCase(ref, dfm, f, xy, _integer_init(f, xy), radius, order, step, s))
     return cases
+
+
+def strain_sequence(width: int, height: int, n_frames: int, seed: int, step_strain: float = 2e-4,
+                    sigma: float = 1.5, density: float = 4000.0 / (512 * 512), out=None):
+    """Incremental uniaxial-strain sequence: frame f (1-based) is the reference
+    speckle under u = f * step_strain * (X - xc), v = 0 (xc the image centre),
+    plus Gaussian noise of `sigma` grey levels (stream 2f + 1); the reference
+    gets stream 0.  float32, unquantised.  `out`, if given, is a list of
+    n_frames + 1 preallocated arrays (reference first) to render into."""
+    nb = max(1, int(round(density * width * height)))
+    xc, yc = 0.5 * (width - 1), 0.5 * (height - 1)
+    imgs = []
+    for f in range(n_frames + 1):
+        field = None if f == 0 else affine(0.0, 0.0, f * step_strain, 0.0, 0.0, 0.0, xc, yc)
+        img = add_noise(render(width, height, nb, seed, field), sigma, seed, 2 * f + (1 if f else 0))
+        if out is not None:
+            out[f][...] = img
+            img = out[f]
+        imgs.append(img)
+    return imgs[0], imgs[1:]
+
+
+def strain_pois(width: int, height: int, step: int = 7, radius: int = 15):
+    xy = poi_grid(width, height, step)
+    return xy[solvable_mask(xy, width, height, radius)]
