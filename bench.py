@@ -41,6 +41,21 @@ C_ITER = {(1, 6): 0, (3, 6): 121, (5, 6): 237, (1, 12): 0, (3, 12): 145, (5, 12)
 C_UPD = {6: 40, 12: 1400}
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "icgn_large_field_2048_hbm_traffic.md")
+
+
+def measured_traffic(kind: str):
+    """HBM bytes per launch of the IC-GN kernel ("mixed" / "fp64") from the
+    committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE passes,
+    each scaled by its reading for a cold 1 GiB device copy); None if absent."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+        blob = text.split("```json", 1)[1].split("```", 1)[0]
+        return int(json.loads(blob)[kind]["hbm_bytes_per_launch"])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def algorithmic_flops(ns: int, P: int, degree: int, iterations: np.ndarray) -> float:
     it = iterations.astype(np.float64)
     per = ns * C_PRE[P] + it * ns * C_ITER[(degree, P)] + P ** 3 + it * (2 * P * P + C_UPD[P])
@@ -196,6 +211,45 @@ def serving_leg(pkg, ctx, synth, duration_s=6.0, rate=20.0, seed=4000):
     return out
 
 
+def stereo_leg(pkg, ctx, synth, steps=5):
+    """Stereo cross-camera matching: 2448x2048 left/right views, plane at 1 m
+    with a 0.5 mm bulge, second order, radius 25, POI step 8, guesses from the
+    flat-plane epipolar prediction; IC-GN over all POIs, inputs in HBM."""
+    t0 = time.time()
+    c = synth.stereo_pair()
+    n = c.poi_xy.shape[0]
+    log(f"stereo pair rendered in {time.time() - t0:.1f}s: {n} POIs")
+    dref, ddef = ctx.upload_image(c.left), ctx.upload_image(c.right)
+    dxy, dini = ctx.to_device(np.ascontiguousarray(c.poi_xy)), ctx.to_device(c.init)
+    cfg = pkg.SubsetConfig(subset_radius=c.radius, shape_function="second_order")
+    res, ms = {}, {}
+    for name, prec in (("fp64", pkg.PRECISION_FP64), ("mixed", pkg.PRECISION_MIXED)):
+        grad = ctx.ref_gradient(dref, pkg.GRADIENT_CENTRAL, 5, 3, prec)
+        co = ctx.prefilter(ddef, 5, 3, True, prec)
+        out = ctx.alloc(n * pkg.RESULT_DTYPE.itemsize)
+        params = pkg.make_params(cfg, prec)
+        ms[name] = _median_ms(ctx, lambda: ctx.solve_device(params, dref, grad, co, dxy.ptr, dini.ptr, n, out.ptr),
+                              steps)
+        res[name] = out.download(pkg.RESULT_DTYPE)
+    fp, mx = res["fp64"], res["mixed"]
+    ok64 = (fp["status"] == pkg.SUCCESS) & (fp["valid"] == 1)
+    d = np.maximum(np.abs(mx["p"][:, 0] - fp["p"][:, 0]), np.abs(mx["p"][:, 1] - fp["p"][:, 1]))
+    okmx = (mx["status"] == pkg.SUCCESS) & (mx["valid"] == 1) & ok64 & (d <= 1e-3)
+    err = np.hypot(mx["p"][okmx, 0] + c.poi_xy[okmx, 0] - c.truth_xy[okmx, 0],
+                   mx["p"][okmx, 1] + c.poi_xy[okmx, 1] - c.truth_xy[okmx, 1])
+    out = {"workload": "stereo 2448x2048, f 8000 px, baseline 200 mm, 15 deg, plane 1 m + 0.5 mm bulge, "
+                       "second order, radius 25, step 8, epipolar guess",
+           "pois": int(n), "fp64_ms": round(ms["fp64"], 3), "mixed_ms": round(ms["mixed"], 3),
+           "converged": int(okmx.sum()), "converged_fp64": int(ok64.sum()),
+           "value": round(okmx.sum() / (ms["mixed"] * 1e-3), 1),
+           "fp64_value": round(ok64.sum() / (ms["fp64"] * 1e-3), 1),
+           "mixed_error_vs_truth_px": {"median": round(float(np.median(err)), 5),
+                                       "p99": round(float(np.percentile(err, 99)), 5)},
+           "mixed_vs_fp64_max_px": float(np.max(d[ok64], initial=0.0))}
+    log(f"stereo: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +260,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-pair batched leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream serving leg")
     ap.add_argument("--serving-seconds", type=float, default=6.0)
+    ap.add_argument("--no-stereo", action="store_true", help="skip the stereo matching leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy a cold 1 GiB device buffer (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)")
@@ -222,6 +279,11 @@ def main():
     log(f"workload generated in {time.time() - t0:.1f}s: {n} POIs, {ns} samples/subset")
 
     ctx = pkg.Context(0)
+    if args.calibrate_copy:
+        a, b = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
+        ctx.copy(b, a)
+        ctx.synchronize()
+        del a, b
     cfg = pkg.SubsetConfig(subset_radius=radius, max_iterations=30, convergence_threshold=1e-3,
                            shape_function="second_order")
     cfg.image_precompute.degree = degree
@@ -285,7 +347,7 @@ def main():
         achieved = fl / t / 1e12
         peak = PEAK_FP32_TFLOPS if name == "mixed" else PEAK_FP64_TFLOPS
         return {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None,
+                "frac": round(achieved / peak, 4), "traffic": measured_traffic(name),
                 "kernel": f"icgn_kernel<P=12,ZNSSD,quintic,{'f32' if name == 'mixed' else 'f64'}>",
                 "algorithmic_gflop_per_launch": round(fl / 1e9, 3), "kernel_ms": round(timing[name]["kernel_ms"], 4)}
 
@@ -328,6 +390,7 @@ def main():
 
     batched = None if args.no_batched else batched_leg(pkg, ctx, synth)
     serving = None if args.no_serving else serving_leg(pkg, ctx, synth, duration_s=args.serving_seconds)
+    stereo = None if args.no_stereo else stereo_leg(pkg, ctx, synth)
 
     line = {
         "metric": "converged POIs/s (FP32 vs FP64 ICGN, |du|<1e-3 px); p50/p99 request latency",
@@ -358,6 +421,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,
+        "stereo": stereo,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -104,6 +104,8 @@ int dic_buffer_alloc(dic_ctx *ctx, size_t bytes, void **device_ptr);
 void dic_buffer_free(dic_ctx *ctx, void *device_ptr);
 int dic_buffer_upload(dic_ctx *ctx, void *device_dst, const void *host_src, size_t bytes);
 int dic_buffer_download(dic_ctx *ctx, void *host_dst, const void *device_src, size_t bytes);
+/* Device-to-device copy on the context's stream (not synchronised). */
+int dic_buffer_copy(dic_ctx *ctx, void *device_dst, const void *device_src, size_t bytes);
 /* Records a new event on the context's stream; free with dic_event_destroy. */
 int dic_event_record(dic_ctx *ctx, void **event);
 int dic_event_elapsed_ms(void *start, void *end, float *ms);
@@ -197,8 +199,9 @@ enum { DIC_BATCH_ORDER_NATURAL = 0, DIC_BATCH_ORDER_COST = 1, DIC_BATCH_ORDER_IN
  *     POIs at a time from a queue counter until the queue is empty.
  *   order: NATURAL = item after item; COST = items by descending estimated
  *     cost per POI (subset samples x shape order); INTERLEAVE = round robin
- *     over the items in runs of 256 POIs; XCD = natural order dealt so that
- *     each of the 8 XCDs gets one contiguous eighth (grid mapping). */
+ *     over the items in runs of 256 POIs; XCD = natural order with each
+ *     item's POIs dealt so that each of the 8 XCDs gets one contiguous eighth
+ *     of them (grid mapping). */
 typedef struct dic_batch_tuning {
     int32_t mapping;
     int32_t waves_per_cu;

@@ -37,7 +37,7 @@ EXPORTED_SYMBOLS = (
     "dic_grad_download", "dic_grad_free", "dic_bspline_prefilter", "dic_coeffs_info",
     "dic_coeffs_download", "dic_coeffs_free", "dic_bspline_sample", "dic_icgn_solve_batch",
     "dic_icgn_solve_batch_device", "dic_buffer_alloc", "dic_buffer_free", "dic_buffer_upload",
-    "dic_buffer_download", "dic_event_record", "dic_event_elapsed_ms", "dic_event_destroy",
+    "dic_buffer_download", "dic_buffer_copy", "dic_event_record", "dic_event_elapsed_ms", "dic_event_destroy",
     "dic_bspline_prefilter_into", "dic_ref_gradient_into",
     "dic_batch_create", "dic_batch_set_tuning", "dic_batch_get_tuning", "dic_batch_info", "dic_batch_run",
     "dic_batch_autotune", "dic_batch_destroy",
@@ -191,6 +191,7 @@ _SIGNATURES = {
     "dic_buffer_free": ([_vp, _vp], None),
     "dic_buffer_upload": ([_vp, _vp, _vp, ctypes.c_size_t], _i32),
     "dic_buffer_download": ([_vp, _vp, _vp, ctypes.c_size_t], _i32),
+    "dic_buffer_copy": ([_vp, _vp, _vp, ctypes.c_size_t], _i32),
     "dic_event_record": ([_vp, ctypes.POINTER(_vp)], _i32),
     "dic_event_elapsed_ms": ([_vp, _vp, ctypes.POINTER(ctypes.c_float)], _i32),
     "dic_event_destroy": ([_vp], None),
@@ -303,6 +304,10 @@ class Context:
         buf = self.alloc(a.nbytes)
         self.check(self.lib.dic_buffer_upload(self.handle, buf.ptr, _ptr(a), a.nbytes))
         return buf
+
+    def copy(self, dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int | None = None):
+        n = min(dst.nbytes, src.nbytes) if nbytes is None else int(nbytes)
+        self.check(self.lib.dic_buffer_copy(self.handle, dst.ptr, src.ptr, n))
 
     def record_event(self) -> int:
         e = _vp()

@@ -106,18 +106,22 @@ std::vector<int2> make_items(const dic_batch *b, int order, std::vector<BatchGro
                 for (int i = 0; i < b->pairs[p].a.n; ++i) items.push_back(make_int2(p, i));
         }
         if (order == DIC_BATCH_ORDER_XCD) {
-            // workgroup k runs on XCD k % 8: give each XCD one contiguous
-            // eighth of the group, so neighbouring POIs share that XCD's L2
-            const size_t m = items.size() - start;
-            const size_t blk = (m + kXcds - 1) / kXcds;
-            std::vector<int2> sw;
-            sw.reserve(m);
-            for (size_t j = 0; j < blk; ++j)
-                for (size_t x = 0; x < (size_t)kXcds; ++x) {
-                    const size_t src = x * blk + j;
-                    if (src < m) sw.push_back(items[start + src]);
-                }
-            std::copy(sw.begin(), sw.end(), items.begin() + start);
+            // workgroup k runs on XCD k % 8: deal each descriptor's POIs so that
+            // every XCD takes one contiguous eighth of them (neighbouring
+            // subsets then share that XCD's L2), descriptors in natural order
+            size_t at = start;
+            for (int p : ids) {
+                const int n = b->pairs[p].a.n, per = (n + kXcds - 1) / kXcds;
+                std::vector<int2> sw;
+                sw.reserve((size_t)n);
+                for (int j = 0; j < per; ++j)
+                    for (int x = 0; x < kXcds; ++x) {
+                        const int poi = x * per + j;
+                        if (poi < n) sw.push_back(make_int2(p, poi));
+                    }
+                std::copy(sw.begin(), sw.end(), items.begin() + at);
+                at += sw.size();
+            }
         }
         G.count = (int)(items.size() - start);
         if (G.count > 0) groups.push_back(G);

@@ -220,6 +220,13 @@ int dic_buffer_download(dic_ctx *ctx, void *host_dst, const void *device_src, si
     return rc;
 }
 
+int dic_buffer_copy(dic_ctx *ctx, void *device_dst, const void *device_src, size_t bytes)
+{
+    if (!ctx || (bytes && (!device_src || !device_dst))) return DIC_ERR_INVALID_ARGUMENT;
+    return check_hip(ctx, hipMemcpyAsync(device_dst, device_src, bytes, hipMemcpyDeviceToDevice, ctx->stream),
+                     "buffer copy");
+}
+
 int dic_event_record(dic_ctx *ctx, void **event)
 {
     if (!ctx || !event) return DIC_ERR_INVALID_ARGUMENT;

@@ -28,11 +28,10 @@
 //     mean / norm / residual structure (icgn.cpp:577-592), deformed samples
 //     kept in LDS between the passes.
 //   R = float (mixed): FP32 coefficients, gradients, warp offsets (relative to
-//     the integer part of u, v), sampling and products; per-lane sums blocked
-//     in FP32 over 8 samples and folded into FP64; FP64 cross-lane
-//     reductions, Hessian factorisation, solve, warp update and parameter
-//     accumulator.  ZNSSD sums are taken in one pass about the
-//     shift K = mean(f):  sum(g-K), sum((g-K)^2), sum(fhat (g-K)),
+//     the integer part of u, v), sampling, products and per-lane sums; FP64
+//     cross-lane (pairwise) reductions, Hessian factorisation, solve, warp
+//     update and parameter accumulator.  ZNSSD sums are taken in one pass
+//     about the shift K = mean(f):  sum(g-K), sum((g-K)^2), sum(fhat (g-K)),
 //     sum((g-K) sd); mean, norm, G and C follow algebraically.
 #include <cfloat>
 #include <cmath>
@@ -45,29 +44,16 @@ namespace {
 
 constexpr double kEps = 1e-12; // icgn.cpp:13
 
-// Per-lane accumulator.  FP64: plain sum.  FP32 (mixed precision): values
-// and products are FP32, summed in FP32 over a block of kBlock samples and
-// folded into an FP64 per-lane total (blocked pairwise summation; an FP64
-// add costs the same issue slot as an FP32 add on this GPU, and the result
-// is tighter than Kahan-compensated FP32).
-constexpr int kBlock = 8;
-template <typename R> struct Acc;
-template <> struct Acc<double> {
-    double s = 0.0;
-    __device__ __forceinline__ void add(double v) { s += v; }
-    __device__ __forceinline__ void fold() {}
-    __device__ __forceinline__ double value() const { return s; }
-};
-template <> struct Acc<float> {
-    double s = 0.0;
-    float b = 0.0f;
-    __device__ __forceinline__ void add(float v) { b += v; }
-    __device__ __forceinline__ void fold()
-    {
-        s += (double)b;
-        b = 0.0f;
-    }
-    __device__ __forceinline__ double value() const { return s + (double)b; }
+// Per-lane accumulator in the working precision.  Each lane sums at most
+// ceil(Ns / 64) terms (20 for r = 20, 45 for r = 30) and the 64 lane totals
+// are then added pairwise in FP64 by the butterfly reduction: pairwise
+// summation with short FP32 leaves in the mixed path (error below
+// (leaf + 6) * eps32 * sum |x|).  An FP64 per-lane fold would cost 2 VGPRs
+// per accumulator and, at 12 + 3 accumulators, a wave per SIMD.
+template <typename R> struct Acc {
+    R s = R(0);
+    __device__ __forceinline__ void add(R v) { s += v; }
+    __device__ __forceinline__ double value() const { return (double)s; }
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -486,26 +472,42 @@ template <> struct RowLoad<double> {
 };
 
 // B-spline value at (ix + tx, iy + ty) from the padded coefficients (row
-// pitch in elements, border b): all n*n loads are issued before the weights.
+// pitch in elements, border b).  FP64: all n*n loads are issued before the
+// weights (latency hiding at 4 waves per SIMD); FP32: row by row, which keeps
+// fewer load results live and lets the mixed kernels run 5 waves per SIMD.
 template <int D, typename R>
 __device__ __forceinline__ R sample_at(__amdgpu_buffer_rsrc_t rs, int pitch, int b, int ix, int iy, R tx, R ty)
 {
     using S = Spline<D>;
     const int voff = ((iy + b - S::off) * pitch + (ix + b - S::off)) * (int)sizeof(R);
     const int rowb = pitch * (int)sizeof(R);
-    R c[S::n][S::n];
-#pragma unroll
-    for (int j = 0; j < S::n; ++j) RowLoad<R>::template row<S::n>(rs, voff, j * rowb, c[j]);
     R wx[S::n], wy[S::n];
-    S::weights(tx, wx);
-    S::weights(ty, wy);
     R v = R(0);
+    if constexpr (sizeof(R) == 8) {
+        R c[S::n][S::n];
 #pragma unroll
-    for (int j = 0; j < S::n; ++j) {
-        R rv = R(0);
+        for (int j = 0; j < S::n; ++j) RowLoad<R>::template row<S::n>(rs, voff, j * rowb, c[j]);
+        S::weights(tx, wx);
+        S::weights(ty, wy);
 #pragma unroll
-        for (int k = 0; k < S::n; ++k) rv += wx[k] * c[j][k];
-        v += wy[j] * rv;
+        for (int j = 0; j < S::n; ++j) {
+            R rv = R(0);
+#pragma unroll
+            for (int k = 0; k < S::n; ++k) rv += wx[k] * c[j][k];
+            v += wy[j] * rv;
+        }
+    } else {
+        S::weights(tx, wx);
+        S::weights(ty, wy);
+#pragma unroll
+        for (int j = 0; j < S::n; ++j) {
+            R c[S::n];
+            RowLoad<R>::template row<S::n>(rs, voff, j * rowb, c);
+            R rv = R(0);
+#pragma unroll
+            for (int k = 0; k < S::n; ++k) rv += wx[k] * c[k];
+            v += wy[j] * rv;
+        }
     }
     return v;
 }
@@ -516,9 +518,6 @@ __device__ __forceinline__ void decode(int t, int &dx, int &dy)
     dy = t >> 16;
 }
 
-// The second-order mixed kernel asks for 4 waves per SIMD (128 VGPRs): left
-// alone the compiler takes ~160 VGPRs (3 waves) and runs ~15% slower; the
-// spill it needs lies outside the sample loops.
 // One POI on the calling wave (64 lanes, the whole workgroup): sh is the
 // workgroup's LDS record, gbuf (FP64 ZNSSD only) holds nfull doubles.
 template <int P, int OBJ, int D, typename R, bool MASKED>
@@ -609,7 +608,6 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
 #pragma unroll
             for (int r0 = 0; r0 < P; r0 += 6) {
                 Acc<R> acc[6][P];
-                int blk = 0;
                 for (int s = lane; s < a.nfull; s += 64) {
                     int dx, dy;
                     decode(a.table[s], dx, dy);
@@ -621,13 +619,6 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                     for (int i = 0; i < 6; ++i)
 #pragma unroll
                         for (int j = i; j < P; ++j) acc[i][j].add(sd[i] * sd[j]);
-                    if (++blk == kBlock) {
-                        blk = 0;
-#pragma unroll
-                        for (int i = 0; i < 6; ++i)
-#pragma unroll
-                            for (int j = i; j < P; ++j) acc[i][j].fold();
-                    }
                 }
 #pragma unroll
                 for (int i = 0; i < 6; ++i)
@@ -647,7 +638,6 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
 #pragma unroll
             for (int T = 0; T < 3; ++T) {
                 Acc<R> m[15];
-                int blk = 0;
                 for (int s = lane; s < a.nfull; s += 64) {
                     int dx, dy;
                     decode(a.table[s], dx, dy);
@@ -660,11 +650,6 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                                         X2 * X2, X2 * XY, X2 * Y2, XY * Y2, Y2 * Y2};
 #pragma unroll
                     for (int k = 0; k < 15; ++k) m[k].add(g * mono[k]);
-                    if (++blk == kBlock) {
-                        blk = 0;
-#pragma unroll
-                        for (int k = 0; k < 15; ++k) m[k].fold();
-                    }
                 }
 #pragma unroll
                 for (int k = 0; k < 15; ++k) {
@@ -687,7 +672,6 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
         if constexpr (!TWOPASS && ZNSSD) {
             Acc<R> a_sd[P], a_fsd[P], a_f, a_f2;
             const R fm = (R)fmean, fin = (R)(1.0 / fnorm);
-            int blk = 0;
             for (int s = lane; s < a.nfull; s += 64) {
                 int dx, dy;
                 decode(a.table[s], dx, dy);
@@ -702,16 +686,6 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 for (int j = 0; j < P; ++j) {
                     a_sd[j].add(sd[j]);
                     a_fsd[j].add(fh * sd[j]);
-                }
-                if (++blk == kBlock) {
-                    blk = 0;
-                    a_f.fold();
-                    a_f2.fold();
-#pragma unroll
-                    for (int j = 0; j < P; ++j) {
-                        a_sd[j].fold();
-                        a_fsd[j].fold();
-                    }
                 }
             }
 #pragma unroll
@@ -801,7 +775,6 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
             }
             bool oob = wild;
             Acc<R> s1, s2, sfg, sg[P];
-            int blk = 0;
             for (int s = wild ? a.nfull : lane; s < a.nfull; s += 64) {
                 int dx, dy;
                 decode(a.table[s], dx, dy);
@@ -876,14 +849,6 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
 #pragma unroll
                         for (int j = 0; j < P; ++j) sg[j].add(e * sd[j]);
                     }
-                }
-                if (++blk == kBlock) {
-                    blk = 0;
-                    s1.fold();
-                    s2.fold();
-                    sfg.fold();
-#pragma unroll
-                    for (int j = 0; j < P; ++j) sg[j].fold();
                 }
             }
             if (__any(oob)) {
@@ -1061,13 +1026,23 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
     }
 }
 
+// Occupancy targets (waves per SIMD): mixed kernels 5 (at most 102 VGPRs;
+// the second-order one spills 64 bytes per lane outside the sample loops and
+// still runs ~4% faster than spill-free at 4 waves); FP64 second order 4
+// (<= 128 VGPRs); FP64 first order takes what it needs.
+template <int P, typename R> constexpr int kMinWaves = sizeof(R) == 4 ? 5 : (P == 12 ? 4 : 1);
+
 template <int P, int OBJ, int D, typename R, bool MASKED>
-__global__ __launch_bounds__(64, (P == 12 && sizeof(R) == 4) ? 4 : 1) void icgn_kernel(IcgnArgs a)
+__global__ __launch_bounds__(64, (kMinWaves<P, R>)) void icgn_kernel(IcgnArgs a)
 {
     __shared__ IcgnShared sh;
     extern __shared__ double gbuf[];
-    if ((int)blockIdx.x >= a.n) return;
-    solve_poi<P, OBJ, D, R, MASKED>(a, (int)blockIdx.x, sh, gbuf);
+    // workgroup k runs on XCD k % 8 (round-robin dispatch): each XCD takes one
+    // contiguous eighth of the POIs, so neighbouring subsets share its L2
+    const int per = (a.n + 7) >> 3;
+    const int poi = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    if (poi >= a.n) return;
+    solve_poi<P, OBJ, D, R, MASKED>(a, poi, sh, gbuf);
 }
 
 // Batched solving over many (pair, radius) descriptors: one launch per group
@@ -1091,7 +1066,7 @@ __device__ __forceinline__ void run_item(const BatchPair *__restrict__ pairs, co
 }
 
 template <int P, int OBJ, int D, typename R, bool PERSISTENT>
-__global__ __launch_bounds__(64, (P == 12 && sizeof(R) == 4) ? 4 : 1) void icgn_batch_kernel(
+__global__ __launch_bounds__(64, (kMinWaves<P, R>)) void icgn_batch_kernel(
     const BatchPair *__restrict__ pairs, const int2 *__restrict__ items, int nitems, unsigned *counter, int chunk)
 {
     __shared__ IcgnShared sh;
@@ -1117,7 +1092,7 @@ __global__ __launch_bounds__(64, (P == 12 && sizeof(R) == 4) ? 4 : 1) void icgn_
 // offsets[k+1]); workgroup g finds its descriptor by a binary search over
 // the (npairs + 1) prefix offsets, so a tick needs no per-POI item table.
 template <int P, int OBJ, int D, typename R>
-__global__ __launch_bounds__(64, (P == 12 && sizeof(R) == 4) ? 4 : 1) void icgn_range_kernel(
+__global__ __launch_bounds__(64, (kMinWaves<P, R>)) void icgn_range_kernel(
     const BatchPair *__restrict__ pairs, const int *__restrict__ offsets, int npairs)
 {
     __shared__ IcgnShared sh;
@@ -1168,7 +1143,7 @@ hipError_t launch_variant(const IcgnArgs &args, hipStream_t s)
         hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(args.n), dim3(64), lds, s, args);
+    hipLaunchKernelGGL(kern, dim3(8 * ((args.n + 7) / 8)), dim3(64), lds, s, args);
     return hipGetLastError();
 }
 

@@ -251,3 +251,135 @@ uint32_t synth_pcg32_first(uint64_t seed, uint64_t stream)
     pcg32_seed(&r, seed, stream);
     return pcg32_next(&r);
 }
+
+/* ---- stereo pair of a speckled surface (pinhole cameras) ---- */
+
+static void stereo_pose(const synth_stereo *st, int cam, double C[3], double R[3][3])
+{
+    const double th = 0.5 * st->converge_deg * 3.141592653589793 / 180.0;
+    const double phi = cam == 0 ? -th : th; /* R = R_y(phi) maps the optical axis to +z */
+    C[0] = cam == 0 ? -0.5 * st->baseline : 0.5 * st->baseline;
+    C[1] = 0.0;
+    C[2] = 0.0;
+    const double c = cos(phi), s = sin(phi);
+    R[0][0] = c, R[0][1] = 0.0, R[0][2] = s;
+    R[1][0] = 0.0, R[1][1] = 1.0, R[1][2] = 0.0;
+    R[2][0] = -s, R[2][1] = 0.0, R[2][2] = c;
+}
+
+static double stereo_height(const synth_stereo *st, double x, double y)
+{
+    const double s2 = st->bulge_sigma * st->bulge_sigma;
+    return st->bulge_amp * exp(-(x * x + y * y) / (2.0 * s2));
+}
+
+void synth_stereo_pixel_to_world(const synth_stereo *st, int cam, double u, double v, int with_bulge, double *X)
+{
+    double C[3], R[3][3];
+    stereo_pose(st, cam, C, R);
+    const double dc[3] = {(u - st->cx) / st->f, (v - st->cy) / st->f, 1.0};
+    double d[3]; /* R^T dc */
+    for (int i = 0; i < 3; ++i) d[i] = R[0][i] * dc[0] + R[1][i] * dc[1] + R[2][i] * dc[2];
+    double h = 0.0, t = 0.0;
+    for (int it = 0; it < (with_bulge ? 8 : 1); ++it) {
+        t = (st->distance - h - C[2]) / d[2];
+        if (with_bulge) h = stereo_height(st, C[0] + t * d[0], C[1] + t * d[1]);
+    }
+    t = (st->distance - h - C[2]) / d[2];
+    for (int i = 0; i < 3; ++i) X[i] = C[i] + t * d[i];
+}
+
+void synth_stereo_world_to_pixel(const synth_stereo *st, int cam, const double *X, double *u, double *v)
+{
+    double C[3], R[3][3];
+    stereo_pose(st, cam, C, R);
+    double q[3];
+    for (int i = 0; i < 3; ++i) q[i] = R[i][0] * (X[0] - C[0]) + R[i][1] * (X[1] - C[1]) + R[i][2] * (X[2] - C[2]);
+    *u = st->f * q[0] / q[2] + st->cx;
+    *v = st->f * q[1] / q[2] + st->cy;
+}
+
+int synth_render_stereo(int W, int H, const synth_stereo *st, int cam, float *out)
+{
+    if (W <= 0 || H <= 0 || st == NULL || out == NULL || st->nblobs < 0) return -1;
+    const size_t npx = (size_t)W * H;
+    double *sx = (double *)malloc(sizeof(double) * npx), *sy = (double *)malloc(sizeof(double) * npx);
+    blob *bl = (blob *)malloc(sizeof(blob) * (size_t)(st->nblobs > 0 ? st->nblobs : 1));
+    if (!sx || !sy || !bl) {
+        free(sx);
+        free(sy);
+        free(bl);
+        return -2;
+    }
+    pcg32 r;
+    pcg32_seed(&r, st->seed, 0x57e0ULL);
+    for (int k = 0; k < st->nblobs; ++k) {
+        bl[k].cx = st->x0 + (st->x1 - st->x0) * pcg32_double(&r);
+        bl[k].cy = st->y0 + (st->y1 - st->y0) * pcg32_double(&r);
+        bl[k].s = st->smin + (st->smax - st->smin) * pcg32_double(&r);
+        bl[k].a = st->amin + (st->amax - st->amin) * pcg32_double(&r);
+    }
+    /* surface coordinates seen by every pixel */
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double X[3];
+            synth_stereo_pixel_to_world(st, cam, (double)x, (double)y, 1, X);
+            sx[(size_t)y * W + x] = X[0];
+            sy[(size_t)y * W + x] = X[1];
+        }
+    /* blobs binned on the surface grid; each pixel sums the blobs of the
+     * 3 x 3 cells around its surface point, in blob-index order */
+    const double cell = 5.0 * st->smax;
+    const int gw = (int)ceil((st->x1 - st->x0) / cell) + 1, gh = (int)ceil((st->y1 - st->y0) / cell) + 1;
+    int *count = (int *)calloc((size_t)gw * gh + 1, sizeof(int));
+    int *idx = (int *)malloc(sizeof(int) * (size_t)(st->nblobs > 0 ? st->nblobs : 1));
+    if (!count || !idx) {
+        free(sx);
+        free(sy);
+        free(bl);
+        free(count);
+        free(idx);
+        return -2;
+    }
+    for (int k = 0; k < st->nblobs; ++k) {
+        const int gx = (int)((bl[k].cx - st->x0) / cell), gy = (int)((bl[k].cy - st->y0) / cell);
+        count[gy * gw + gx + 1]++;
+    }
+    for (int i = 0; i < gw * gh; ++i) count[i + 1] += count[i];
+    int *fill = (int *)malloc(sizeof(int) * (size_t)gw * gh);
+    memcpy(fill, count, sizeof(int) * (size_t)gw * gh);
+    for (int k = 0; k < st->nblobs; ++k) {
+        const int gx = (int)((bl[k].cx - st->x0) / cell), gy = (int)((bl[k].cy - st->y0) / cell);
+        idx[fill[gy * gw + gx]++] = k;
+    }
+    free(fill);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t i = (size_t)y * W + x;
+            const double px = sx[i], py = sy[i];
+            const int gx = (int)floor((px - st->x0) / cell), gy = (int)floor((py - st->y0) / cell);
+            double acc = st->background;
+            for (int cy = gy - 1; cy <= gy + 1; ++cy) {
+                if (cy < 0 || cy >= gh) continue;
+                for (int cx = gx - 1; cx <= gx + 1; ++cx) {
+                    if (cx < 0 || cx >= gw) continue;
+                    const int c = cy * gw + cx;
+                    for (int j = count[c]; j < count[c + 1]; ++j) {
+                        const blob b = bl[idx[j]];
+                        const double ddx = px - b.cx, ddy = py - b.cy;
+                        const double q = (ddx * ddx + ddy * ddy) / (b.s * b.s);
+                        if (q <= 25.0) acc += b.a * exp(-q);
+                    }
+                }
+            }
+            out[i] = (float)acc;
+        }
+    free(count);
+    free(idx);
+    free(sx);
+    free(sy);
+    free(bl);
+    return 0;
+}

@@ -35,6 +35,30 @@ void synth_quantize_unit(float *img, size_t n);
 void synth_scale(float *img, size_t n, float s);
 uint32_t synth_pcg32_first(uint64_t seed, uint64_t stream);
 
+/* Stereo pair of a speckled surface seen by two pinhole cameras.  World frame
+ * in mm: cameras at (-baseline/2, 0, 0) (camera 0, left) and (+baseline/2, 0, 0)
+ * (camera 1, right), each turned by converge_deg/2 about the y axis towards
+ * the other's side; the surface is z = distance - bulge_amp exp(-(x^2+y^2) /
+ * (2 bulge_sigma^2)) (the bulge comes towards the cameras).  Its texture is a
+ * speckle pattern in surface (x, y) coordinates: blobs with 1/e radius
+ * U(smin, smax) mm and peak U(amin, amax) over `background`, centres uniform in
+ * [x0, x1] x [y0, y1] mm.  Pixel (u, v) of a camera sees the surface point hit
+ * by its ray (pinhole: f, cx, cy in px; camera y = world y). */
+typedef struct synth_stereo {
+    double f, cx, cy;
+    double baseline, converge_deg, distance;
+    double bulge_amp, bulge_sigma;
+    double x0, x1, y0, y1;
+    int32_t nblobs, reserved;
+    uint64_t seed;
+    double background, smin, smax, amin, amax;
+} synth_stereo;
+
+int synth_render_stereo(int W, int H, const synth_stereo *st, int camera, float *out);
+/* Surface point (with or without the bulge) seen by pixel (u, v) of a camera. */
+void synth_stereo_pixel_to_world(const synth_stereo *st, int camera, double u, double v, int with_bulge, double *X);
+void synth_stereo_world_to_pixel(const synth_stereo *st, int camera, const double *X, double *u, double *v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,13 @@ class _Field(ctypes.Structure):
         (n, ctypes.c_double) for n in ("u0", "v0", "ux", "uy", "vx", "vy", "xc", "yc", "au", "lu", "av", "lv")]
 
 
+class _Stereo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("f", "cx", "cy", "baseline", "converge_deg", "distance", "bulge_amp",
+                                               "bulge_sigma", "x0", "x1", "y0", "y1")] + [
+        ("nblobs", ctypes.c_int32), ("reserved", ctypes.c_int32), ("seed", ctypes.c_uint64)] + [
+        (n, ctypes.c_double) for n in ("background", "smin", "smax", "amin", "amax")]
+
+
 _slib = None
 
 
@@ -45,6 +52,13 @@ def _load():
         lib.synth_quantize_unit.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
         lib.synth_pcg32_first.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         lib.synth_pcg32_first.restype = ctypes.c_uint32
+        lib.synth_render_stereo.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Stereo), ctypes.c_int,
+                                            ctypes.c_void_p]
+        lib.synth_render_stereo.restype = ctypes.c_int
+        lib.synth_stereo_pixel_to_world.argtypes = [ctypes.POINTER(_Stereo), ctypes.c_int, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+        lib.synth_stereo_world_to_pixel.argtypes = [ctypes.POINTER(_Stereo), ctypes.c_int, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_void_p]
         _slib = lib
     return _slib
 
@@ -278,3 +292,75 @@ def strain_sequence(width: int, height: int, n_frames: int, seed: int, step_stra
 def strain_pois(width: int, height: int, step: int = 7, radius: int = 15):
     xy = poi_grid(width, height, step)
     return xy[solvable_mask(xy, width, height, radius)]
+
+
+@dataclass
+class StereoCase:
+    """Left/right views of a speckled plane with a Gaussian bulge; POIs on the
+    left view, guesses from the flat-plane (epipolar) prediction, truth from
+    the bulged surface."""
+    left: np.ndarray
+    right: np.ndarray
+    poi_xy: np.ndarray
+    init: np.ndarray
+    truth_xy: np.ndarray      # right-view position of every POI's surface point
+    radius: int
+    shape_order: int
+    params: object
+
+
+def _stereo_map(st, xy, with_bulge):
+    lib = _load()
+    out = np.empty_like(xy)
+    X = np.empty(3)
+    u, v = ctypes.c_double(), ctypes.c_double()
+    for i, (x, y) in enumerate(xy):
+        lib.synth_stereo_pixel_to_world(ctypes.byref(st), 0, float(x), float(y), int(with_bulge), X.ctypes.data)
+        lib.synth_stereo_world_to_pixel(ctypes.byref(st), 1, X.ctypes.data, ctypes.byref(u), ctypes.byref(v))
+        out[i] = (u.value, v.value)
+    return out
+
+
+def stereo_pair(seed: int = 5, width: int = 2448, height: int = 2048, step: int = 8, radius: int = 25,
+                shape_order: int = 2) -> StereoCase:
+    """Two pinhole cameras (f = 8000 px, principal point at the image centre),
+    baseline 200 mm, 15 deg convergence, a plane at 1 m carrying a 0.5 mm
+    Gaussian bulge (sigma 40 mm) towards the cameras; speckle blobs of 1/e
+    radius 0.25..0.5 mm (2..4 px at 1 m) at the 512^2 workload's density per
+    pixel.  POIs: left-view grid of `step` px whose subset fits the left view
+    and whose predicted right-view subset fits the right view.  Guess: the
+    flat-plane mapping left -> right (u, v and its four derivatives by central
+    differences); truth: the same mapping through the bulged surface."""
+    from ._lib import INITIAL_DTYPE
+    st = _Stereo()
+    st.f, st.cx, st.cy = 8000.0, 0.5 * (width - 1), 0.5 * (height - 1)
+    st.baseline, st.converge_deg, st.distance = 200.0, 15.0, 1000.0
+    st.bulge_amp, st.bulge_sigma = 0.5, 40.0
+    st.x0, st.x1, st.y0, st.y1 = -260.0, 260.0, -180.0, 180.0
+    mm_per_px = st.distance / st.f
+    area_px = (st.x1 - st.x0) * (st.y1 - st.y0) / mm_per_px ** 2
+    st.nblobs = int(round(4000.0 / (512 * 512) * area_px))
+    st.seed = seed
+    st.background, st.smin, st.smax, st.amin, st.amax = 30.0, 2.0 * mm_per_px, 4.0 * mm_per_px, 60.0, 200.0
+    lib = _load()
+    views = []
+    for cam in (0, 1):
+        img = np.empty((height, width), np.float32)
+        rc = lib.synth_render_stereo(width, height, ctypes.byref(st), cam, img.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"synth_render_stereo failed ({rc})")
+        views.append(img)
+    xy = poi_grid(width, height, step)
+    xy = xy[solvable_mask(xy, width, height, radius)]
+    pred = _stereo_map(st, xy, False)
+    keep = solvable_mask(np.round(pred), width, height, radius + 2)
+    xy, pred = xy[keep], pred[keep]
+    h = 1.0
+    px = _stereo_map(st, xy + [h, 0.0], False) - _stereo_map(st, xy - [h, 0.0], False)
+    py = _stereo_map(st, xy + [0.0, h], False) - _stereo_map(st, xy - [0.0, h], False)
+    ini = np.zeros(xy.shape[0], INITIAL_DTYPE)
+    ini["u"], ini["v"] = pred[:, 0] - xy[:, 0], pred[:, 1] - xy[:, 1]
+    ini["du_dx"], ini["dv_dx"] = px[:, 0] / (2 * h) - 1.0, px[:, 1] / (2 * h)
+    ini["du_dy"], ini["dv_dy"] = py[:, 0] / (2 * h), py[:, 1] / (2 * h) - 1.0
+    truth = _stereo_map(st, xy, True)
+    return StereoCase(views[0], views[1], xy, ini, truth, radius, shape_order, st)
