@@ -60,7 +60,7 @@ typedef struct dic_icgn_params {
     double convergence_threshold; /* convergence_threshold on ||dp||_2 */
     int32_t shape_order;          /* 1 = FirstOrder, 2 = SecondOrder (or use_second_order) */
     int32_t objective;            /* DIC_OBJECTIVE_* */
-    int32_t optimizer;            /* DIC_OPTIMIZER_* (FGN: DIC_ERR_UNSUPPORTED) */
+    int32_t optimizer;            /* DIC_OPTIMIZER_*: IC-GN, or forward Gauss-Newton (ForwardGaussNewtonSolver) */
     int32_t precision;            /* DIC_PRECISION_*; must match the coefficient/gradient handles */
     int32_t masked;               /* 1: solve_with_mask semantics (roi required) */
     int32_t min_valid_samples;    /* masked only */
@@ -153,6 +153,10 @@ int dic_bspline_sample(dic_ctx *ctx, const dic_coeffs *c, const double *xy, int 
 
 /* ---- IC-GN: replaces ICGNSolver::solve_with_interpolators / solve_with_mask
  * (src/subset/solver/icgn.cpp:384-432 -> :446-1912), one POI per entry.
+ * With params->optimizer = DIC_OPTIMIZER_FGN the same entry points replace
+ * ForwardGaussNewtonSolver::solve_with_interpolators / solve_with_mask
+ * (src/subset/solver/forward_gauss_newton.cpp:117-175 -> :188-1941); grad is
+ * then unused and may be NULL.
  * Host-pointer form: copies in, runs, copies out, synchronises. ---- */
 int dic_icgn_solve_batch(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
                          const dic_coeffs *def, const uint8_t *roi_host, const double *poi_xy,

@@ -2,6 +2,7 @@
 // surface Traditional-DIC's C++ core calls on this path:
 //   dic::SubsetSolver / ICGNSolver      include/dic/subset/solver/subset_solver.hpp:37-67,
 //                                       include/dic/subset/solver/icgn.hpp:35-124
+//   ForwardGaussNewtonSolver            include/dic/subset/solver/forward_gauss_newton.hpp:11-107
 //   BSplineImagePreprocessor::compute   include/dic/interpolation/bspline.hpp:84-100
 //   BSplineInterpolator::value/gradient include/dic/interpolation/bspline.hpp:102-130
 //   Displacement2D, InitialDisplacement, SolverStatus, SubsetConfig
@@ -53,6 +54,7 @@ struct SubsetConfig {
     bool use_second_order{false};
     bool second_order_shape{false}; // shape_function == SecondOrder
     bool ssd{false};                // objective == SSD (else ZNSSD)
+    bool forward_gauss_newton{false}; // optimization == FGN (else IC-GN)
     double min_valid_sample_ratio{0.5};
     int min_valid_samples{12};
     BSplinePrecomputeConfig image_precompute{};
@@ -184,7 +186,7 @@ public:
         p.convergence_threshold = cfg_.convergence_threshold;
         p.shape_order = (cfg_.second_order_shape || cfg_.use_second_order) ? 2 : 1;
         p.objective = cfg_.ssd ? DIC_OBJECTIVE_SSD : DIC_OBJECTIVE_ZNSSD;
-        p.optimizer = DIC_OPTIMIZER_ICGN;
+        p.optimizer = cfg_.forward_gauss_newton ? DIC_OPTIMIZER_FGN : DIC_OPTIMIZER_ICGN;
         p.precision = (int)precision_;
         p.masked = roi ? 1 : 0;
         p.min_valid_samples = cfg_.min_valid_samples;
@@ -251,6 +253,23 @@ private:
     const Image *ref_ = nullptr;
     dic_grad *grad_ = nullptr;
     dic_coeffs *coeffs_ = nullptr;
+};
+
+// ForwardGaussNewtonSolver: the same surface; the deformed image's B-spline
+// gradient drives the steepest descent (the reference gradients go unused).
+class ForwardGaussNewtonSolver : public ICGNSolver {
+public:
+    explicit ForwardGaussNewtonSolver(Context &ctx, SubsetConfig cfg = {}, Precision p = Precision::FP64)
+        : ICGNSolver(ctx, with_fgn(cfg), p)
+    {
+    }
+
+private:
+    static SubsetConfig with_fgn(SubsetConfig c)
+    {
+        c.forward_gauss_newton = true;
+        return c;
+    }
 };
 
 } // namespace dic_hip

@@ -886,6 +886,262 @@ void orc_icgn_solve_batch(const orc_config *cfg, const float *ref, int W, int H,
         orc_icgn_solve(cfg, ref, NULL, W, H, gx, gy, dpre, roi, xy[2 * i], xy[2 * i + 1], &init[i], &out[i]);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Forward-compositional GN: forward_gauss_newton.cpp:188-399 (1st ZNSSD),    */
+/* :401-608 (1st ZNSSD masked), :622-794 (1st SSD), :796-966 (1st SSD masked), */
+/* :1069-1297 (2nd ZNSSD), :1299-1526 (2nd ZNSSD masked), :1540-1739 (2nd     */
+/* SSD), :1741-1941 (2nd SSD masked).  Per iteration: warp and sample g, then  */
+/* the steepest-descent vector from the deformed image's B-spline gradient at  */
+/* the warped point through dW/dz of the current warp (:44-60, :986-1019), H   */
+/* and G rebuilt, LDLT, dp = +H^-1 G, step cap, W(p) o W(dp).                 */
+/* Variant quirks restated as they are:                                        */
+/*   step cap 0.5 (1st ZNSSD, both), 0.35 (1st SSD, both; the unmasked one    */
+/*   tests convergence on the norm before the cap, :763-767), 0.5 (2nd ZNSSD), */
+/*   0.3 (2nd ZNSSD masked, norm before the cap, :1485-1489), 0.35 (2nd SSD,   */
+/*   both);  Levenberg-Marquardt H_ii += 1e-4 max_i H_ii on 2nd ZNSSD, 2nd SSD */
+/*   and 2nd SSD masked (:1242-1245, :1682-1685, :1886-1889), not on 2nd      */
+/*   ZNSSD masked.                                                             */
+/* ------------------------------------------------------------------------ */
+void orc_fgn_compose(const double *p, const double *dp, int P, double *pn)
+{
+    pn[0] = dp[0] + p[0] + p[2] * dp[0] + p[3] * dp[1];
+    pn[1] = dp[1] + p[1] + p[4] * dp[0] + p[5] * dp[1];
+    if (P == 6) {
+        pn[2] = dp[2] + p[2] * (1.0 + dp[2]) + p[3] * dp[4];
+        pn[3] = dp[3] + p[2] * dp[3] + p[3] * (1.0 + dp[5]);
+        pn[4] = dp[4] + p[4] * (1.0 + dp[2]) + p[5] * dp[4];
+        pn[5] = dp[5] + p[4] * dp[3] + p[5] * (1.0 + dp[5]);
+        return;
+    }
+    pn[2] = dp[2] + p[2] * (1.0 + dp[2]) + p[3] * dp[4] + p[6] * dp[0] + p[7] * dp[1];
+    pn[3] = dp[3] + p[2] * dp[3] + p[3] * (1.0 + dp[5]) + p[7] * dp[0] + p[8] * dp[1];
+    pn[4] = dp[4] + p[4] * (1.0 + dp[2]) + p[5] * dp[4] + p[9] * dp[0] + p[10] * dp[1];
+    pn[5] = dp[5] + p[4] * dp[3] + p[5] * (1.0 + dp[5]) + p[10] * dp[0] + p[11] * dp[1];
+    pn[6] = dp[6] + p[2] * dp[6] + p[3] * dp[9] + p[6] * (1.0 + 2.0 * dp[2]) + 2.0 * p[7] * dp[4];
+    pn[7] = dp[7] + p[2] * dp[7] + p[3] * dp[10] + p[6] * dp[3] + p[7] * (1.0 + dp[2] + dp[5]) + p[8] * dp[4];
+    pn[8] = dp[8] + p[2] * dp[8] + p[3] * dp[11] + p[7] * dp[3] + p[8] * (1.0 + 2.0 * dp[5]);
+    pn[9] = dp[9] + p[4] * dp[6] + p[5] * dp[9] + p[9] * (1.0 + 2.0 * dp[2]) + 2.0 * p[10] * dp[4];
+    pn[10] = dp[10] + p[4] * dp[7] + p[5] * dp[10] + p[9] * dp[3] + p[10] * (1.0 + dp[2] + dp[5]) + p[11] * dp[4];
+    pn[11] = dp[11] + p[4] * dp[8] + p[5] * dp[11] + p[10] * dp[3] + p[11] * (1.0 + 2.0 * dp[5]);
+}
+
+void orc_fgn_solve(const orc_config *cfg, const float *ref, int W, int H, const orc_precomputed *dpre,
+                   const uint8_t *roi, double px, double py, const orc_initial *init, orc_result *res)
+{
+    const int P = cfg->second_order ? 12 : 6;
+    const int znssd = cfg->objective == 0;
+    const int masked = roi != NULL;
+    memset(res, 0, sizeof(*res));
+    res->x = px;
+    res->y = py;
+    res->p[0] = init->u;
+    res->p[1] = init->v;
+    res->correlation = init->confidence;
+    res->status = ORC_INVALID_INPUT;
+    if (W <= 0 || H <= 0 || dpre == NULL || dpre->width != W || dpre->height != H || cfg->subset_radius < 1 ||
+        cfg->max_iterations < 0)
+        return;
+    const int cx = (int)round(px), cy = (int)round(py);
+    const int r = cfg->subset_radius;
+    if (!masked) {
+        if (cx - r < 0 || cy - r < 0 || cx + r >= W || cy + r >= H) return;
+    } else {
+        if (!(cx >= 0 && cy >= 0 && cx < W && cy < H && roi[(size_t)cy * W + cx])) return;
+    }
+    /* step cap and its convergence norm, Levenberg-Marquardt */
+    double cap = 0.5;
+    int precap_norm = 0, lm = 0;
+    if (P == 6) {
+        cap = znssd ? 0.5 : 0.35;
+        precap_norm = !znssd && !masked;
+    } else if (znssd) {
+        cap = masked ? 0.3 : 0.5;
+        precap_norm = masked;
+        lm = !masked;
+    } else {
+        cap = 0.35;
+        lm = 1;
+    }
+    const int cnt = (2 * r + 1) * (2 * r + 1);
+    double *lx = (double *)malloc(sizeof(double) * cnt);
+    double *ly = (double *)malloc(sizeof(double) * cnt);
+    double *fv = (double *)malloc(sizeof(double) * cnt);
+    double *fn = (double *)malloc(sizeof(double) * cnt);
+    double *gv = (double *)malloc(sizeof(double) * cnt);
+    int ns = 0, full = 0;
+    double fmean = 0.0, fnorm = 1.0;
+    for (int dy = -r; dy <= r; ++dy)
+        for (int dx = -r; dx <= r; ++dx) {
+            if (dx * dx + dy * dy > r * r) continue;
+            ++full;
+            const int x = cx + dx, y = cy + dy;
+            if (masked && !(x >= 0 && y >= 0 && x < W && y < H && roi[(size_t)y * W + x])) continue;
+            lx[ns] = (double)dx;
+            ly[ns] = (double)dy;
+            fv[ns] = (double)ref[(size_t)y * W + x];
+            fmean += fv[ns];
+            ++ns;
+        }
+    int min_needed = P;
+    if (masked) {
+        int ms = (int)ceil(cfg->min_valid_sample_ratio * (double)full);
+        if (cfg->min_valid_samples > ms) ms = cfg->min_valid_samples;
+        if (ms > min_needed) min_needed = ms;
+    }
+    if (ns < min_needed) goto done;
+    if (znssd) {
+        fmean /= (double)ns;
+        double acc = 0.0;
+        for (int i = 0; i < ns; ++i) acc += (fv[i] - fmean) * (fv[i] - fmean);
+        fnorm = sqrt(acc);
+        if (fnorm <= 1e-12) {
+            res->status = ORC_NUMERICAL_FAILURE;
+            goto done;
+        }
+        for (int i = 0; i < ns; ++i) fn[i] = (fv[i] - fmean) / fnorm;
+    }
+    {
+        double p[12] = {0};
+        p[0] = init->u;
+        p[1] = init->v;
+        p[2] = init->du_dx;
+        p[3] = init->du_dy;
+        p[4] = init->dv_dx;
+        p[5] = init->dv_dy;
+        double corr = INFINITY;
+        int converged = 0;
+        const double Wm1 = (double)(W - 1), Hm1 = (double)(H - 1);
+        for (int it = 0; it < cfg->max_iterations; ++it) {
+            res->iterations = it + 1;
+            double gmean = 0.0;
+            for (int i = 0; i < ns; ++i) {
+                double wx = (double)cx + lx[i] + p[0] + p[2] * lx[i] + p[3] * ly[i];
+                double wy = (double)cy + ly[i] + p[1] + p[4] * lx[i] + p[5] * ly[i];
+                if (P == 12) {
+                    const double dx22 = lx[i] * lx[i] * 0.5, dy22 = ly[i] * ly[i] * 0.5, dxy = lx[i] * ly[i];
+                    wx += p[6] * dx22 + p[7] * dxy + p[8] * dy22;
+                    wy += p[9] * dx22 + p[10] * dxy + p[11] * dy22;
+                }
+                if (!(wx >= 0.0 && wy >= 0.0 && wx <= Wm1 && wy <= Hm1)) {
+                    res->status = ORC_INVALID_INPUT;
+                    goto done;
+                }
+                gv[i] = orc_value(dpre, wx, wy);
+                gmean += gv[i];
+            }
+            double gnorm = 1.0;
+            if (znssd) {
+                gmean /= (double)ns;
+                double acc = 0.0;
+                for (int i = 0; i < ns; ++i) acc += (gv[i] - gmean) * (gv[i] - gmean);
+                gnorm = sqrt(acc);
+                if (gnorm <= 1e-12) {
+                    res->status = ORC_NUMERICAL_FAILURE;
+                    goto done;
+                }
+            }
+            double Hm[144], G[12];
+            memset(Hm, 0, sizeof(Hm));
+            memset(G, 0, sizeof(G));
+            corr = 0.0;
+            for (int i = 0; i < ns; ++i) {
+                const double dx22 = lx[i] * lx[i] * 0.5, dy22 = ly[i] * ly[i] * 0.5, dxy = lx[i] * ly[i];
+                double wx = (double)cx + lx[i] + p[0] + p[2] * lx[i] + p[3] * ly[i];
+                double wy = (double)cy + ly[i] + p[1] + p[4] * lx[i] + p[5] * ly[i];
+                if (P == 12) {
+                    wx += p[6] * dx22 + p[7] * dxy + p[8] * dy22;
+                    wy += p[9] * dx22 + p[10] * dxy + p[11] * dy22;
+                }
+                double g2[2];
+                orc_gradient(dpre, wx, wy, g2);
+                /* dW/dz of the current warp at (lx, ly) */
+                double a00 = 1.0 + p[2], a01 = p[3], a10 = p[4], a11 = 1.0 + p[5];
+                if (P == 12) {
+                    a00 += p[6] * lx[i] + p[7] * ly[i];
+                    a01 += p[7] * lx[i] + p[8] * ly[i];
+                    a10 += p[9] * lx[i] + p[10] * ly[i];
+                    a11 += p[10] * lx[i] + p[11] * ly[i];
+                }
+                const double ge = g2[0] * a00 + g2[1] * a10, he = g2[0] * a01 + g2[1] * a11;
+                double sd[12] = {ge, he, ge * lx[i], ge * ly[i], he * lx[i], he * ly[i],
+                                 ge * dx22, ge * dxy, ge * dy22, he * dx22, he * dxy, he * dy22};
+                const double e = znssd ? fn[i] - (gv[i] - gmean) / gnorm : fv[i] - gv[i];
+                for (int a = 0; a < P; ++a) {
+                    for (int b2 = 0; b2 < P; ++b2) Hm[a * P + b2] += sd[a] * sd[b2];
+                    G[a] += e * sd[a];
+                }
+                corr += e * e;
+            }
+            if (znssd) {
+                const double sh = 2.0 / (fnorm * fnorm), sg = 2.0 / fnorm;
+                for (int k = 0; k < P * P; ++k) Hm[k] *= sh;
+                for (int a = 0; a < P; ++a) G[a] *= sg;
+            }
+            if (lm) {
+                double mx = Hm[0];
+                for (int a = 1; a < P; ++a)
+                    if (Hm[a * P + a] > mx) mx = Hm[a * P + a];
+                const double lambda = 1e-4 * mx;
+                for (int a = 0; a < P; ++a) Hm[a * P + a] += lambda;
+            }
+            int tr[12], ispos = 0;
+            if (orc_ldlt_factor(Hm, P, tr, &ispos) != 0 || !ispos) {
+                res->status = ORC_NUMERICAL_FAILURE;
+                goto done;
+            }
+            double dp[12] = {0};
+            orc_ldlt_solve(Hm, P, tr, G, dp);
+            if (!all_finite(dp, P)) {
+                res->status = ORC_NUMERICAL_FAILURE;
+                goto done;
+            }
+            double nrm = 0.0;
+            for (int a = 0; a < P; ++a) nrm += dp[a] * dp[a];
+            nrm = sqrt(nrm);
+            double test = nrm;
+            if (nrm > cap) {
+                for (int a = 0; a < P; ++a) dp[a] *= cap / nrm;
+                if (!precap_norm) test = cap;
+            }
+            double pn[12] = {0};
+            orc_fgn_compose(p, dp, P, pn);
+            memcpy(p, pn, sizeof(double) * P);
+            if (!all_finite(p, P)) {
+                res->status = ORC_NUMERICAL_FAILURE;
+                goto done;
+            }
+            if (test < cfg->convergence_threshold) {
+                converged = 1;
+                break;
+            }
+        }
+        for (int a = 0; a < P; ++a) res->p[a] = p[a];
+        res->correlation = corr;
+        if (!converged && isfinite(corr)) converged = 1;
+        res->status = converged ? ORC_SUCCESS : ORC_NOT_CONVERGED;
+        res->valid = converged;
+    }
+done:
+    free(lx);
+    free(ly);
+    free(fv);
+    free(fn);
+    free(gv);
+}
+
+void orc_fgn_solve_batch(const orc_config *cfg, const float *ref, int W, int H, const orc_precomputed *dpre,
+                         const uint8_t *roi, const double *xy, const orc_initial *init, int n, orc_result *out,
+                         int threads)
+{
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 8) num_threads(threads)
+#else
+    (void)threads;
+#endif
+    for (int i = 0; i < n; ++i) orc_fgn_solve(cfg, ref, W, H, dpre, roi, xy[2 * i], xy[2 * i + 1], &init[i], &out[i]);
+}
+
 int orc_poi_grid(int W, int H, int spacing, double *xy)
 {
     if (spacing < 0) spacing = 0;

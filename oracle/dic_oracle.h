@@ -119,6 +119,18 @@ void orc_icgn_solve_batch(const orc_config *cfg, const float *reference,
                           const double *poi_xy, const orc_initial *init, int n,
                           orc_result *out, int threads);
 
+/* --- Forward-compositional Gauss-Newton (src/subset/solver/forward_gauss_newton.cpp) --- */
+/* compose_warp, :64-76 (first order) and :1025-1053 (second order) */
+void orc_fgn_compose(const double *p, const double *dp, int P, double *out);
+/* One POI.  Uses only the deformed image's interpolant (value and gradient);
+ * the reference image gives f.  roi may be NULL (unmasked variants). */
+void orc_fgn_solve(const orc_config *cfg, const float *reference, int width, int height,
+                   const orc_precomputed *deformed, const uint8_t *roi, double px, double py,
+                   const orc_initial *init, orc_result *out);
+void orc_fgn_solve_batch(const orc_config *cfg, const float *reference, int width, int height,
+                         const orc_precomputed *deformed, const uint8_t *roi, const double *poi_xy,
+                         const orc_initial *init, int n, orc_result *out, int threads);
+
 /* POI grid of ReliabilityPropagation::propagate (reliability_propagation.cpp:154-176). */
 int orc_poi_grid(int width, int height, int spacing, double *xy_out /* may be NULL */);
 

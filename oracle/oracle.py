@@ -73,6 +73,9 @@ def load():
     lib.orc_ic_second_order_update.argtypes = [vp, vp, vp]
     lib.orc_icgn_solve_batch.argtypes = [ctypes.POINTER(OrcConfig), vp, ci, ci, vp, vp,
                                          ctypes.POINTER(OrcPrecomputed), vp, vp, vp, ci, vp, ci]
+    lib.orc_fgn_compose.argtypes = [vp, vp, ci, vp]
+    lib.orc_fgn_solve_batch.argtypes = [ctypes.POINTER(OrcConfig), vp, ci, ci, ctypes.POINTER(OrcPrecomputed), vp,
+                                        vp, vp, ci, vp, ci]
     lib.orc_poi_grid.argtypes = [ci, ci, ci, vp]
     lib.orc_poi_grid.restype = ci
     _lib = lib
@@ -229,4 +232,32 @@ def icgn_solve(reference: np.ndarray, pre: Precomputed, grad_x: np.ndarray, grad
     out = np.zeros(xy.shape[0], RESULT_DTYPE)
     load().orc_icgn_solve_batch(ctypes.byref(cfg), _p(ref), W, H, _p(gx), _p(gy), ctypes.byref(pre.s), _p(r),
                                 _p(xy), _p(ini), xy.shape[0], _p(out), int(threads))
+    return out
+
+
+def fgn_compose(p: np.ndarray, dp: np.ndarray) -> np.ndarray:
+    """compose_warp of forward_gauss_newton.cpp (:64-76 first order, :1025-1053 second order)."""
+    P = len(p)
+    a = np.ascontiguousarray(p, np.float64)
+    b = np.ascontiguousarray(dp, np.float64)
+    out = np.zeros(P)
+    load().orc_fgn_compose(_p(a), _p(b), P, _p(out))
+    return out
+
+
+def fgn_solve(reference: np.ndarray, pre: Precomputed, poi_xy: np.ndarray, init: np.ndarray, radius: int = 15,
+              max_iterations: int = 30, threshold: float = 1e-3, shape_order: int = 1, objective: int = 0,
+              roi: np.ndarray | None = None, min_valid_samples: int = 12, min_valid_sample_ratio: float = 0.5,
+              threads: int = 0) -> np.ndarray:
+    """ForwardGaussNewtonSolver (forward_gauss_newton.cpp) over POIs; `pre` is the deformed image's precompute."""
+    ref = np.ascontiguousarray(reference, np.float32)
+    H, W = ref.shape
+    cfg = OrcConfig(int(radius), int(max_iterations), float(threshold), 1 if shape_order == 2 else 0, int(objective),
+                    int(min_valid_samples), 0, float(min_valid_sample_ratio))
+    xy = np.ascontiguousarray(poi_xy, np.float64).reshape(-1, 2)
+    ini = np.ascontiguousarray(init, INITIAL_DTYPE)
+    r = np.ascontiguousarray(roi, np.uint8) if roi is not None else None
+    out = np.zeros(xy.shape[0], RESULT_DTYPE)
+    load().orc_fgn_solve_batch(ctypes.byref(cfg), _p(ref), W, H, ctypes.byref(pre.s), _p(r), _p(xy), _p(ini),
+                               xy.shape[0], _p(out), int(threads))
     return out

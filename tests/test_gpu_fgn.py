@@ -1,0 +1,86 @@
+"""Forward-compositional Gauss-Newton (optimizer = FGN) on the GPU against the
+oracle's restatement of forward_gauss_newton.cpp: identical statuses and
+valid flags, |du|, |dv| <= 1e-5 px and |dZNCC| <= 1e-6 in FP64 over
+{order 1, 2} x {ZNSSD, SSD} x {unmasked, masked}; mixed precision within
+1e-3 px of FP64; the step caps show in one-step runs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _report(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def pair(synth):
+    f = synth.affine(0.6, -0.4, 2e-3, 1e-3, -1e-3, 1.5e-3, 95.5, 95.5)
+    ref = synth.quantize_unit(synth.render(192, 192, 560, 11))
+    dfm = synth.quantize_unit(synth.render(192, 192, 560, 11, f))
+    xy = synth.poi_grid(192, 192, 9)
+    xy = xy[synth.solvable_mask(xy, 192, 192, 15)]
+    return ref, dfm, f, xy
+
+
+def _gpu(pkg, ctx, ref, dfm, xy, ini, order, objective, precision, roi=None, max_it=30, thr=1e-3):
+    cfg = pkg.SubsetConfig(subset_radius=15, max_iterations=max_it, convergence_threshold=thr,
+                           shape_function="second_order" if order == 2 else "first_order", objective=objective,
+                           optimizer="forward_gauss_newton")
+    cfg.image_precompute.degree = 3
+    params = pkg.make_params(cfg, precision, masked=roi is not None)
+    dref, ddef = ctx.upload_image(ref), ctx.upload_image(dfm)
+    co = ctx.prefilter(ddef, 3, 3, True, precision)
+    grad = ctx.ref_gradient(dref, 0, 3, 3, precision)
+    return ctx.solve(params, dref, grad, co, xy, ini, roi=roi)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("objective", ["znssd", "ssd"])
+@pytest.mark.parametrize("masked", [False, True])
+def test_fgn_fp64_parity(pkg, ctx, orc, pair, order, objective, masked):
+    ref, dfm, f, xy = pair
+    u, v, _ = f.displacement(xy[:, 0], xy[:, 1])
+    ini = pkg.initial_guesses(np.round(u), np.round(v))
+    roi = None
+    if masked:
+        roi = np.ones(ref.shape, np.uint8)
+        roi[:, 150:] = 0
+    gpu = _gpu(pkg, ctx, ref, dfm, xy, ini, order, objective, pkg.PRECISION_FP64, roi)
+    cpu = orc.fgn_solve(ref, orc.Precomputed(dfm, degree=3), xy, ini, radius=15, shape_order=order,
+                        objective=1 if objective == "ssd" else 0, roi=roi)
+    assert np.array_equal(gpu["status"], cpu["status"]), np.nonzero(gpu["status"] != cpu["status"])
+    assert np.array_equal(gpu["valid"], cpu["valid"])
+    ok = cpu["status"] == 0
+    du = np.abs(gpu["p"][ok, :2] - cpu["p"][ok, :2]).max(initial=0)
+    scale = 2.0 if objective == "znssd" else max(1.0, float(np.abs(cpu["correlation"][ok]).max(initial=1.0)))
+    dz = (np.abs(gpu["correlation"][ok] - cpu["correlation"][ok]) / scale).max(initial=0)
+    same_it = np.mean(gpu["iterations"][ok] == cpu["iterations"][ok]) if ok.any() else 1.0
+    _report(f"fgn order={order} {objective} masked={masked}: {ok.sum()}/{len(ok)} ok, |du| {du:.2e}, "
+            f"dcorr {dz:.2e}, same iterations {same_it:.3f}")
+    assert ok.sum() > 0.5 * len(ok)
+    assert du <= 1e-5 and dz <= 1e-6
+    bad = ~ok
+    assert np.array_equal(gpu["p"][bad, :2], cpu["p"][bad, :2])
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_fgn_mixed_within_1e3_of_fp64(pkg, ctx, pair, order):
+    ref, dfm, f, xy = pair
+    u, v, _ = f.displacement(xy[:, 0], xy[:, 1])
+    ini = pkg.initial_guesses(np.round(u), np.round(v))
+    fp = _gpu(pkg, ctx, ref, dfm, xy, ini, order, "znssd", pkg.PRECISION_FP64)
+    mx = _gpu(pkg, ctx, ref, dfm, xy, ini, order, "znssd", pkg.PRECISION_MIXED)
+    ok = (fp["status"] == 0) & (mx["status"] == 0)
+    d = np.abs(mx["p"][ok, :2] - fp["p"][ok, :2]).max(initial=0)
+    _report(f"fgn mixed order={order}: {ok.sum()} ok, max |mixed-fp64| {d:.2e}")
+    assert ok.mean() > 0.95 and d <= 1e-3
+
+
+def test_fgn_step_cap_one_step(pkg, ctx, orc, pair):
+    ref, dfm, f, xy = pair
+    ini = pkg.initial_guesses(np.zeros(len(xy)) + 2.5, np.zeros(len(xy)))
+    gpu = _gpu(pkg, ctx, ref, dfm, xy, ini, 1, "znssd", pkg.PRECISION_FP64, max_it=1)
+    ok = gpu["status"] == 0
+    step = np.linalg.norm(gpu["p"][ok, :6] - np.array([2.5, 0, 0, 0, 0, 0]), axis=1)
+    assert ok.any() and np.all(step <= 0.5 + 1e-12) and np.any(np.abs(step - 0.5) < 1e-12)

@@ -11,13 +11,15 @@ from ._lib import (Context, DicError, INITIAL_DTYPE, RESULT_DTYPE, IcgnParams, l
                    BATCH_PERSISTENT, BATCH_ORDER_NATURAL, BATCH_ORDER_COST, BATCH_ORDER_INTERLEAVE,
                    BATCH_ORDER_XCD, Server, ServedRequest)
 from .subset import (ICGNSolver, SubsetConfig, BSplinePrecomputeConfig, config_from_dict, compute_grid,
-                     initial_guesses, make_params, results_to_dict, PreparedPair)
+                     initial_guesses, make_params, results_to_dict, PreparedPair, ForwardGaussNewtonSolver,
+                     make_solver)
 
 __all__ = [
     "Context", "DicError", "INITIAL_DTYPE", "RESULT_DTYPE", "IcgnParams", "load_library", "device_count",
     "PRECISION_FP64", "PRECISION_MIXED", "SUCCESS", "NOT_CONVERGED", "INVALID_INPUT", "NUMERICAL_FAILURE",
     "GRADIENT_CENTRAL", "GRADIENT_BSPLINE_HALF", "ICGNSolver", "SubsetConfig", "BSplinePrecomputeConfig",
     "config_from_dict", "compute_grid", "initial_guesses", "make_params", "results_to_dict", "PreparedPair",
+    "ForwardGaussNewtonSolver", "make_solver",
     "Batch", "BatchItem", "BatchTuning", "BATCH_GRID", "BATCH_PERSISTENT", "BATCH_ORDER_NATURAL",
     "BATCH_ORDER_COST", "BATCH_ORDER_INTERLEAVE", "BATCH_ORDER_XCD", "Server", "ServedRequest",
 ]

@@ -528,18 +528,20 @@ int dic_bspline_sample(dic_ctx *ctx, const dic_coeffs *c, const double *xy, int 
 static int validate_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
                           const dic_coeffs *def, int n)
 {
-    if (!params || !ref || !grad || !def || n < 0)
+    if (!params || !ref || !def || n < 0)
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "null argument or negative POI count");
-    if (params->optimizer == DIC_OPTIMIZER_FGN)
-        return set_error(ctx, DIC_ERR_UNSUPPORTED, "forward Gauss-Newton is not implemented on the GPU path");
-    if (params->optimizer != DIC_OPTIMIZER_ICGN) return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "unknown optimizer");
+    if (params->optimizer != DIC_OPTIMIZER_ICGN && params->optimizer != DIC_OPTIMIZER_FGN)
+        return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "unknown optimizer");
+    // forward Gauss-Newton differentiates the deformed image; IC-GN needs the reference gradients
+    const bool need_grad = params->optimizer == DIC_OPTIMIZER_ICGN;
+    if (need_grad && !grad) return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "IC-GN needs reference gradients");
     if (params->shape_order != 1 && params->shape_order != 2)
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "shape_order must be 1 or 2");
     if (params->objective != DIC_OBJECTIVE_ZNSSD && params->objective != DIC_OBJECTIVE_SSD)
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "unknown objective");
-    if (params->precision != def->precision || params->precision != grad->precision)
+    if (params->precision != def->precision || (grad && params->precision != grad->precision))
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "precision of params, coefficients and gradients differ");
-    if (grad->width != ref->width || grad->height != ref->height)
+    if (grad && (grad->width != ref->width || grad->height != ref->height))
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "gradient planes do not match the reference image");
     return DIC_OK;
 }
@@ -562,8 +564,8 @@ static int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_imag
     L.width = same ? ref->width : 0;
     L.height = same ? ref->height : 0;
     L.ref = ref->data;
-    L.gx = grad->gx;
-    L.gy = grad->gy;
+    L.gx = grad ? grad->gx : nullptr;
+    L.gy = grad ? grad->gy : nullptr;
     L.coeffs = def->data;
     L.coeff_bytes = def->bytes;
     L.pitch = def->pitch;

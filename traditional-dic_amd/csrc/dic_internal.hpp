@@ -82,6 +82,7 @@ struct IcgnArgs {
     int radius, max_it;
     double thr;
     int min_needed;
+    int fgn; // forward-compositional Gauss-Newton instead of IC-GN
     const double *xy;
     const dic_initial *init;
     dic_result *out;

@@ -1133,9 +1133,508 @@ __global__ void fill_u32_kernel(unsigned *p, int n, unsigned v)
     if (i < n) p[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Forward-compositional Gauss-Newton (ForwardGaussNewtonSolver,
+// src/subset/solver/forward_gauss_newton.cpp).  Per iteration the deformed
+// image is sampled with its B-spline value and gradient at every warped
+// point; the steepest-descent vector is that gradient through dW/dz of the
+// current warp (:44-60, :986-1019); H, G are rebuilt, LDLT-factored, and
+// dp = +H^-1 G is capped and composed as W(p) o W(dp) (:64-76, :1025-1053).
+// Per-sample g, dW/dz-projected gradients (ge, he) live in LDS between the
+// passes: A (warp + sample), B (deformed norm, ZNSSD), C (G and C), then the
+// Hessian from 3 x NM moments of {ge^2, ge he, he^2} times monomials.
+// Variant constants as the reference has them: step cap 0.5 / 0.35 / 0.3,
+// which norm the convergence test reads, Levenberg-Marquardt on three of the
+// four second-order variants.
+// ---------------------------------------------------------------------------
+template <int P, int OBJ, bool MASKED> struct FgnRules {
+    static constexpr bool znssd = OBJ == DIC_OBJECTIVE_ZNSSD;
+    // :367, :573, :763, :936, :1260, :1485, :1700, :1902
+    static constexpr double cap = P == 6 ? (znssd ? 0.5 : 0.35) : (znssd ? (MASKED ? 0.3 : 0.5) : 0.35);
+    // the convergence test reads the norm before the cap (:763-767, :1485-1489)
+    static constexpr bool precap = (P == 6 && !znssd && !MASKED) || (P == 12 && znssd && MASKED);
+    // H_ii += 1e-4 max_i H_ii (:1242-1245, :1682-1685, :1886-1889)
+    static constexpr bool lm = P == 12 && !(znssd && MASKED);
+};
+
+// Eigen LDLT::solve of one right-hand side in place (lane 0): P b, L^-1,
+// D^+ (|d| > DBL_MIN), L^-T, P^T.
+template <int P> __device__ void ldlt_solve_vec(const IcgnShared &sh, double *x)
+{
+#pragma unroll 1
+    for (int k = 0; k < P; ++k) {
+        const int t = sh.tr[k];
+        if (t != k) {
+            const double v = x[k];
+            x[k] = x[t];
+            x[t] = v;
+        }
+    }
+#pragma unroll 1
+    for (int i = 0; i < P; ++i) {
+        double acc = x[i];
+#pragma unroll 1
+        for (int j = 0; j < i; ++j) acc -= sh.H[i * P + j] * x[j];
+        x[i] = acc;
+    }
+#pragma unroll 1
+    for (int i = 0; i < P; ++i) {
+        const double d = sh.H[i * P + i];
+        x[i] = fabs(d) > DBL_MIN ? x[i] / d : 0.0;
+    }
+#pragma unroll 1
+    for (int i = P - 1; i >= 0; --i) {
+        double acc = x[i];
+#pragma unroll 1
+        for (int j = i + 1; j < P; ++j) acc -= sh.H[j * P + i] * x[j];
+        x[i] = acc;
+    }
+#pragma unroll 1
+    for (int k = P - 1; k >= 0; --k) {
+        const int t = sh.tr[k];
+        if (t != k) {
+            const double v = x[k];
+            x[k] = x[t];
+            x[t] = v;
+        }
+    }
+}
+
+// compose_warp: W(x; pn) = W(x; p) o W(x; dp) (:64-76 first order,
+// :1025-1053 second order, first order in dp)
+template <int P> __device__ void fgn_compose(const double *p, const double *dp, double *pn)
+{
+    pn[0] = dp[0] + p[0] + p[2] * dp[0] + p[3] * dp[1];
+    pn[1] = dp[1] + p[1] + p[4] * dp[0] + p[5] * dp[1];
+    if constexpr (P == 6) {
+        pn[2] = dp[2] + p[2] * (1.0 + dp[2]) + p[3] * dp[4];
+        pn[3] = dp[3] + p[2] * dp[3] + p[3] * (1.0 + dp[5]);
+        pn[4] = dp[4] + p[4] * (1.0 + dp[2]) + p[5] * dp[4];
+        pn[5] = dp[5] + p[4] * dp[3] + p[5] * (1.0 + dp[5]);
+    } else {
+        pn[2] = dp[2] + p[2] * (1.0 + dp[2]) + p[3] * dp[4] + p[6] * dp[0] + p[7] * dp[1];
+        pn[3] = dp[3] + p[2] * dp[3] + p[3] * (1.0 + dp[5]) + p[7] * dp[0] + p[8] * dp[1];
+        pn[4] = dp[4] + p[4] * (1.0 + dp[2]) + p[5] * dp[4] + p[9] * dp[0] + p[10] * dp[1];
+        pn[5] = dp[5] + p[4] * dp[3] + p[5] * (1.0 + dp[5]) + p[10] * dp[0] + p[11] * dp[1];
+        pn[6] = dp[6] + p[2] * dp[6] + p[3] * dp[9] + p[6] * (1.0 + 2.0 * dp[2]) + 2.0 * p[7] * dp[4];
+        pn[7] = dp[7] + p[2] * dp[7] + p[3] * dp[10] + p[6] * dp[3] + p[7] * (1.0 + dp[2] + dp[5]) + p[8] * dp[4];
+        pn[8] = dp[8] + p[2] * dp[8] + p[3] * dp[11] + p[7] * dp[3] + p[8] * (1.0 + 2.0 * dp[5]);
+        pn[9] = dp[9] + p[4] * dp[6] + p[5] * dp[9] + p[9] * (1.0 + 2.0 * dp[2]) + 2.0 * p[10] * dp[4];
+        pn[10] = dp[10] + p[4] * dp[7] + p[5] * dp[10] + p[9] * dp[3] + p[10] * (1.0 + dp[2] + dp[5]) +
+                 p[11] * dp[4];
+        pn[11] = dp[11] + p[4] * dp[8] + p[5] * dp[11] + p[10] * dp[3] + p[11] * (1.0 + 2.0 * dp[5]);
+    }
+}
+
+// B-spline value and gradient at (ix + tx, iy + ty) (BSplineInterpolator::value
+// and ::gradient, bspline.cpp:458-500, as a tensor product)
+template <int D, typename R>
+__device__ __forceinline__ void sample_vg(__amdgpu_buffer_rsrc_t rs, int pitch, int b, int ix, int iy, R tx, R ty,
+                                          R &v, R &gx, R &gy)
+{
+    using S = Spline<D>;
+    const int voff = ((iy + b - S::off) * pitch + (ix + b - S::off)) * (int)sizeof(R);
+    const int rowb = pitch * (int)sizeof(R);
+    R wx[S::n], wy[S::n], dwx[S::n], dwy[S::n];
+    S::weights(tx, wx);
+    S::weights(ty, wy);
+    S::dweights(tx, dwx);
+    S::dweights(ty, dwy);
+    v = gx = gy = R(0);
+#pragma unroll
+    for (int j = 0; j < S::n; ++j) {
+        R c[S::n];
+        RowLoad<R>::template row<S::n>(rs, voff, j * rowb, c);
+        R rw = R(0), rd = R(0);
+#pragma unroll
+        for (int k = 0; k < S::n; ++k) {
+            rw += wx[k] * c[k];
+            rd += dwx[k] * c[k];
+        }
+        v += wy[j] * rw;
+        gx += wy[j] * rd;
+        gy += dwy[j] * rw;
+    }
+}
+
+template <int P, int OBJ, int D, typename R, bool MASKED>
+__device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, IcgnShared &sh, R *lbuf)
+{
+    constexpr bool ZNSSD = OBJ == DIC_OBJECTIVE_ZNSSD;
+    constexpr bool FP64 = sizeof(R) == 8;
+    using Rules = FgnRules<P, OBJ, MASKED>;
+    constexpr int NM = P == 12 ? 15 : 6; // monomials dx^p dy^q with p + q <= 2 (P = 6) or 4 (P = 12)
+    const int lane = threadIdx.x;
+    const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<void *>(a.coeffs), (short)0, a.coeff_bytes, 0x00020000);
+    const int W = a.W, Hh = a.H;
+    const double px = a.xy[2 * poi], py = a.xy[2 * poi + 1];
+    const int cx = (int)round(px), cy = (int)round(py);
+    R *gv = lbuf, *gev = lbuf + a.nfull, *hev = lbuf + 2 * a.nfull;
+    auto usable = [&](int x, int y) -> bool {
+        if constexpr (MASKED) return x >= 0 && y >= 0 && x < W && y < Hh && a.roi[(size_t)y * W + x];
+        return true;
+    };
+
+    int status = DIC_INVALID_INPUT;
+    int iterations = 0;
+    bool early = W <= 0 || Hh <= 0 || a.radius < 1 || a.max_it < 0;
+    if (!early) {
+        if constexpr (!MASKED) early = cx - a.radius < 0 || cy - a.radius < 0 || cx + a.radius >= W ||
+                                       cy + a.radius >= Hh;
+        else early = !usable(cx, cy);
+    }
+    // reference statistics (forward_gauss_newton.cpp:233-274)
+    if (!early) {
+        double fsum_l = 0.0;
+        int cnt = 0;
+        for (int s = lane; s < a.nfull; s += 64) {
+            int dx, dy;
+            decode(a.table[s], dx, dy);
+            if (!usable(cx + dx, cy + dy)) continue;
+            fsum_l += (double)a.ref[(size_t)(cy + dy) * W + cx + dx];
+            ++cnt;
+        }
+        const int ns = wave_sum_i(cnt);
+        double fmean = 0.0, fnorm = 1.0;
+        if (ns < a.min_needed) {
+            early = true;
+        } else if constexpr (ZNSSD) {
+            fmean = wave_sum(fsum_l) / (double)ns;
+            double acc = 0.0;
+            for (int s = lane; s < a.nfull; s += 64) {
+                int dx, dy;
+                decode(a.table[s], dx, dy);
+                if (!usable(cx + dx, cy + dy)) continue;
+                const double df = (double)a.ref[(size_t)(cy + dy) * W + cx + dx] - fmean;
+                acc += df * df;
+            }
+            fnorm = sqrt(wave_sum(acc));
+            if (fnorm <= kEps) {
+                early = true;
+                status = DIC_NUMERICAL_FAILURE;
+            }
+        }
+        if (lane == 0) {
+            sh.fmean = fmean;
+            sh.fnorm = fnorm;
+            sh.ns = ns;
+        }
+    }
+
+    double corr = __builtin_inf();
+    bool solved = false;
+    if (!early) {
+        if (lane == 0) {
+            const dic_initial ini = a.init[poi];
+            sh.p[0] = ini.u;
+            sh.p[1] = ini.v;
+            sh.p[2] = ini.du_dx;
+            sh.p[3] = ini.du_dy;
+            sh.p[4] = ini.dv_dx;
+            sh.p[5] = ini.dv_dy;
+#pragma unroll
+            for (int j = 6; j < 12; ++j) sh.p[j] = 0.0;
+        }
+        __syncthreads();
+        const double Wm1 = (double)(W - 1), Hm1 = (double)(Hh - 1);
+        const int ns = sh.ns;
+        const double fmean = uniform_f64(sh.fmean), fnorm = uniform_f64(sh.fnorm);
+        bool converged = false, failed = false;
+        for (int it = 0; it < a.max_it; ++it) {
+            iterations = it + 1;
+            double p[12];
+#pragma unroll
+            for (int j = 0; j < 12; ++j) p[j] = (j < P) ? uniform_f64(sh.p[j]) : 0.0;
+            const bool wild = !(fabs(p[0]) < 1e7 && fabs(p[1]) < 1e7);
+            const double ui = wild ? 0.0 : floor(p[0]), vi = wild ? 0.0 : floor(p[1]);
+            const R uf = (R)(p[0] - ui), vf = (R)(p[1] - vi);
+            const int bx = cx + (int)ui, by = cy + (int)vi;
+            R q[12];
+#pragma unroll
+            for (int j = 2; j < 12; ++j) q[j] = (R)p[j];
+            // ---- pass A: warp, bounds (:283-300), value and gradient at the warped point ----
+            bool oob = wild;
+            Acc<R> sg;
+            for (int s = wild ? a.nfull : lane; s < a.nfull; s += 64) {
+                int dx, dy;
+                decode(a.table[s], dx, dy);
+                if (!usable(cx + dx, cy + dy)) continue;
+                int ix, iy;
+                R tx, ty;
+                const R lx = (R)dx, ly = (R)dy;
+                if constexpr (FP64) {
+                    double wx = (double)cx + lx + p[0] + p[2] * lx + p[3] * ly;
+                    double wy = (double)cy + ly + p[1] + p[4] * lx + p[5] * ly;
+                    if constexpr (P == 12) {
+                        const double dx22 = lx * lx * 0.5, dy22 = ly * ly * 0.5, dxy = lx * ly;
+                        wx += p[6] * dx22 + p[7] * dxy + p[8] * dy22;
+                        wy += p[9] * dx22 + p[10] * dxy + p[11] * dy22;
+                    }
+                    if (!(wx >= 0.0 && wy >= 0.0 && wx <= Wm1 && wy <= Hm1)) {
+                        oob = true;
+                        continue;
+                    }
+                    ix = min(max((int)floor(wx), 0), W - 1);
+                    iy = min(max((int)floor(wy), 0), Hh - 1);
+                    tx = (R)(wx - (double)ix);
+                    ty = (R)(wy - (double)iy);
+                } else {
+                    R xl = lx + uf + q[2] * lx + q[3] * ly;
+                    R yl = ly + vf + q[4] * lx + q[5] * ly;
+                    if constexpr (P == 12) {
+                        const R dx22 = lx * lx * R(0.5), dy22 = ly * ly * R(0.5), dxy = lx * ly;
+                        xl += q[6] * dx22 + q[7] * dxy + q[8] * dy22;
+                        yl += q[9] * dx22 + q[10] * dxy + q[11] * dy22;
+                    }
+                    const R fx = fmin(fmax(floor(xl), R(-1e7)), R(1e7));
+                    const R fy = fmin(fmax(floor(yl), R(-1e7)), R(1e7));
+                    ix = bx + (int)fx;
+                    iy = by + (int)fy;
+                    tx = xl - fx;
+                    ty = yl - fy;
+                    if (ix < 0 || iy < 0 || ix > W - 1 || iy > Hh - 1 || (ix == W - 1 && tx > R(0)) ||
+                        (iy == Hh - 1 && ty > R(0))) {
+                        oob = true;
+                        continue;
+                    }
+                }
+                R g, gx, gy;
+                sample_vg<D, R>(crs, a.pitch, a.b, ix, iy, tx, ty, g, gx, gy);
+                // dW/dz of the current warp at (lx, ly) (:994-998)
+                R a00 = R(1) + q[2], a01 = q[3], a10 = q[4], a11 = R(1) + q[5];
+                if constexpr (P == 12) {
+                    a00 += q[6] * lx + q[7] * ly;
+                    a01 += q[7] * lx + q[8] * ly;
+                    a10 += q[9] * lx + q[10] * ly;
+                    a11 += q[10] * lx + q[11] * ly;
+                }
+                gv[s] = g;
+                gev[s] = gx * a00 + gy * a10;
+                hev[s] = gx * a01 + gy * a11;
+                sg.add(g);
+            }
+            if (__any(oob)) {
+                status = DIC_INVALID_INPUT;
+                failed = true;
+                break;
+            }
+            // ---- pass B: deformed mean and norm (vector_norm, :31-39) ----
+            double gmean = 0.0, gnorm = 1.0;
+            if constexpr (ZNSSD) {
+                gmean = wave_sum(sg.value()) / (double)ns;
+                const R gm = (R)gmean;
+                Acc<R> sq;
+                for (int s = lane; s < a.nfull; s += 64) {
+                    if constexpr (MASKED) {
+                        int dx, dy;
+                        decode(a.table[s], dx, dy);
+                        if (!usable(cx + dx, cy + dy)) continue;
+                    }
+                    const R d = gv[s] - gm;
+                    sq.add(d * d);
+                }
+                gnorm = sqrt(wave_sum(sq.value()));
+                if (gnorm <= kEps) {
+                    status = DIC_NUMERICAL_FAILURE;
+                    failed = true;
+                    break;
+                }
+            }
+            // ---- pass C: G and C (:335-350) ----
+            {
+                const R fm = (R)fmean, fin = (R)(1.0 / fnorm), gm = (R)gmean, gin = (R)(1.0 / gnorm);
+                Acc<R> ag[P], ac;
+                for (int s = lane; s < a.nfull; s += 64) {
+                    int dx, dy;
+                    decode(a.table[s], dx, dy);
+                    const int x = cx + dx, y = cy + dy;
+                    if (!usable(x, y)) continue;
+                    const R f = (R)a.ref[(size_t)y * W + x];
+                    const R e = ZNSSD ? (f - fm) * fin - (gv[s] - gm) * gin : f - gv[s];
+                    R sd[P];
+                    steepest_descent<P, R>(gev[s], hev[s], (R)dx, (R)dy, sd);
+#pragma unroll
+                    for (int j = 0; j < P; ++j) ag[j].add(e * sd[j]);
+                    ac.add(e * e);
+                }
+                const double gsc = ZNSSD ? 2.0 / fnorm : 1.0;
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const double v = wave_sum(ag[j].value()) * gsc;
+                    if (lane == 0) sh.red[j] = v;
+                }
+                const double v = wave_sum(ac.value());
+                if (lane == 0) sh.red[P] = v;
+            }
+            // ---- Hessian from moments of {ge^2, ge he, he^2} x monomials ----
+#pragma unroll
+            for (int T = 0; T < 3; ++T) {
+                Acc<R> m[NM];
+                for (int s = lane; s < a.nfull; s += 64) {
+                    int dx, dy;
+                    decode(a.table[s], dx, dy);
+                    if (!usable(cx + dx, cy + dy)) continue;
+                    const R ge = gev[s], he = hev[s];
+                    const R w = (T == 0) ? ge * ge : ((T == 1) ? ge * he : he * he);
+                    const R X = (R)dx, Y = (R)dy, X2 = X * X, Y2 = Y * Y, XY = X * Y;
+                    const R mono[15] = {R(1), X, Y, X2, XY, Y2, X2 * X, X2 * Y, X * Y2, Y2 * Y,
+                                        X2 * X2, X2 * XY, X2 * Y2, XY * Y2, Y2 * Y2};
+#pragma unroll
+                    for (int k = 0; k < NM; ++k) m[k].add(w * mono[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < NM; ++k) {
+                    const double v = wave_sum(m[k].value());
+                    if (lane == 0) sh.mom[T][k] = v;
+                }
+            }
+            __syncthreads();
+            {
+                constexpr int ty[12] = {0, 1, 0, 0, 1, 1, 0, 0, 0, 1, 1, 1};
+                constexpr int pe[12] = {0, 0, 1, 0, 1, 0, 2, 1, 0, 2, 1, 0};
+                constexpr int qe[12] = {0, 0, 0, 1, 0, 1, 0, 1, 2, 0, 1, 2};
+                constexpr double ce[12] = {1, 1, 1, 1, 1, 1, 0.5, 1, 0.5, 0.5, 1, 0.5};
+                const double hsc = ZNSSD ? 2.0 / (fnorm * fnorm) : 1.0;
+                for (int e = lane; e < P * P; e += 64) {
+                    const int ia = e / P, ib = e % P;
+                    const int pp = pe[ia] + pe[ib], qq = qe[ia] + qe[ib], d = pp + qq;
+                    sh.H[e] = hsc * ce[ia] * ce[ib] * sh.mom[ty[ia] + ty[ib]][d * (d + 1) / 2 + qq];
+                }
+            }
+            __syncthreads();
+            if constexpr (Rules::lm) {
+                if (lane == 0) {
+                    double mx = sh.H[0];
+                    for (int i = 1; i < P; ++i) mx = fmax(mx, sh.H[i * P + i]);
+                    const double lambda = 1e-4 * mx;
+                    for (int i = 0; i < P; ++i) sh.H[i * P + i] += lambda;
+                }
+                __syncthreads();
+            }
+            int ok, pos;
+            if constexpr (P == 12) {
+                ldlt_factor_wave<P>(sh, lane, ok, pos);
+            } else {
+                if (lane == 0) {
+                    ldlt_factor<P>(sh.H, sh.tr, sh.x, &ok, &pos);
+                    sh.state = ok && pos;
+                }
+                __syncthreads();
+                ok = pos = sh.state;
+            }
+            if (!(ok && pos)) {
+                status = DIC_NUMERICAL_FAILURE;
+                failed = true;
+                break;
+            }
+            corr = sh.red[P];
+            // lane 0: dp = +H^-1 G, cap, compose (:351-390)
+            if (lane == 0) {
+                double dp[12] = {0.0};
+#pragma unroll
+                for (int j = 0; j < P; ++j) dp[j] = sh.red[j];
+                ldlt_solve_vec<P>(sh, dp);
+                int st = 0; // 0 go on, 1 converged, 2 failed
+                bool fin = true;
+                double nrm = 0.0;
+                for (int j = 0; j < P; ++j) {
+                    fin = fin && isfinite(dp[j]);
+                    nrm += dp[j] * dp[j];
+                }
+                if (!fin) {
+                    st = 2;
+                } else {
+                    nrm = sqrt(nrm);
+                    double test = nrm;
+                    if (nrm > Rules::cap) {
+                        const double sc = Rules::cap / nrm;
+                        for (int j = 0; j < P; ++j) dp[j] *= sc;
+                        if (!Rules::precap) test = Rules::cap;
+                    }
+                    double pn[12] = {0.0};
+                    fgn_compose<P>(sh.p, dp, pn);
+                    bool okp = true;
+                    for (int j = 0; j < P; ++j) {
+                        sh.p[j] = pn[j];
+                        okp = okp && isfinite(pn[j]);
+                    }
+                    if (!okp) st = 2;
+                    else if (test < a.thr) st = 1;
+                }
+                sh.state = st;
+            }
+            __syncthreads();
+            const int st = sh.state;
+            if (st == 2) {
+                status = DIC_NUMERICAL_FAILURE;
+                failed = true;
+                break;
+            }
+            if (st == 1) {
+                converged = true;
+                break;
+            }
+        }
+        if (!failed) {
+            if (!converged && isfinite(corr)) converged = true;
+            status = converged ? DIC_SUCCESS : DIC_NOT_CONVERGED;
+            solved = true;
+        }
+    }
+
+    if (lane == 0) {
+        const dic_initial ini = a.init[poi];
+        dic_result r;
+        r.x = px;
+        r.y = py;
+#pragma unroll
+        for (int j = 0; j < 12; ++j) r.p[j] = 0.0;
+        if (solved) {
+#pragma unroll
+            for (int j = 0; j < P; ++j) r.p[j] = sh.p[j];
+            r.correlation = corr;
+        } else {
+            r.p[0] = ini.u;
+            r.p[1] = ini.v;
+            r.correlation = ini.confidence;
+        }
+        r.status = status;
+        r.valid = (status == DIC_SUCCESS) ? 1 : 0;
+        r.iterations = iterations;
+        r.reserved = 0;
+        a.out[poi] = r;
+    }
+}
+
+template <int P, int OBJ, int D, typename R, bool MASKED>
+__global__ __launch_bounds__(64, (kMinWaves<P, R>)) void fgn_kernel(IcgnArgs a)
+{
+    __shared__ IcgnShared sh;
+    extern __shared__ double fbuf[];
+    const int per = (a.n + 7) >> 3;
+    const int poi = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    if (poi >= a.n) return;
+    fgn_solve_poi<P, OBJ, D, R, MASKED>(a, poi, sh, reinterpret_cast<R *>(fbuf));
+}
+
+template <int P, int OBJ, int D, typename R, bool MASKED>
+hipError_t launch_fgn_variant(const IcgnArgs &args, hipStream_t s)
+{
+    const size_t lds = 3 * sizeof(R) * (size_t)args.nfull;
+    auto kern = fgn_kernel<P, OBJ, D, R, MASKED>;
+    if (lds > 48 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(8 * ((args.n + 7) / 8)), dim3(64), lds, s, args);
+    return hipGetLastError();
+}
+
 template <int P, int OBJ, int D, typename R, bool MASKED>
 hipError_t launch_variant(const IcgnArgs &args, hipStream_t s)
 {
+    if (args.fgn) return launch_fgn_variant<P, OBJ, D, R, MASKED>(args, s);
     constexpr bool TWOPASS = OBJ == DIC_OBJECTIVE_ZNSSD && sizeof(R) == 8;
     const size_t lds = TWOPASS ? sizeof(double) * (size_t)args.nfull : 0;
     auto kern = icgn_kernel<P, OBJ, D, R, MASKED>;
@@ -1201,6 +1700,7 @@ hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s)
         if (ms > needed) needed = ms;
     }
     a.min_needed = needed;
+    a.fgn = pr.optimizer == DIC_OPTIMIZER_FGN ? 1 : 0;
     a.xy = L.poi_xy;
     a.init = L.init;
     a.out = L.out;

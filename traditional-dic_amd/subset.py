@@ -12,6 +12,8 @@ The numerics run in libdic_hip.so; nothing here computes a correlation.
 """
 from __future__ import annotations
 
+import dataclasses
+
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -209,6 +211,26 @@ class ICGNSolver:
     def solve_with_mask(self, reference, deformed, roi, point, initial) -> np.void:
         return self.solve_batch(reference, deformed, np.asarray(point, np.float64).reshape(1, 2),
                                 np.atleast_1d(np.asarray(initial, INITIAL_DTYPE)), roi=roi)[0]
+
+
+class ForwardGaussNewtonSolver(ICGNSolver):
+    """ForwardGaussNewtonSolver (include/dic/subset/solver/forward_gauss_newton.hpp:11-107)
+    on the GPU, same call pattern as ICGNSolver: the deformed image's B-spline
+    gradient at the warped points drives the steepest descent, H is rebuilt
+    every iteration, dp = +H^-1 G is capped and composed forward."""
+
+    def __init__(self, config: SubsetConfig | dict | None = None, precision: str = "fp64",
+                 gradient: str = "central", device: int = 0, context: _lib.Context | None = None):
+        super().__init__(config, precision, gradient, device, context)
+        self.config = dataclasses.replace(self.config, optimizer="forward_gauss_newton")
+
+
+def make_solver(config: SubsetConfig | dict | None = None, precision: str = "fp64", **kw) -> ICGNSolver:
+    """The solver the configuration names (reliability_propagation.cpp:191-196)."""
+    cfg = config if isinstance(config, SubsetConfig) else config_from_dict(config)
+    if cfg.optimizer == "forward_gauss_newton":
+        return ForwardGaussNewtonSolver(cfg, precision, **kw)
+    return ICGNSolver(cfg, precision, **kw)
 
 
 def results_to_dict(results: np.ndarray) -> dict:
