@@ -302,6 +302,39 @@ const char *dic_server_error(const dic_server *s);
 /* Finishes all submitted requests, then stops the worker and frees everything. */
 void dic_server_destroy(dic_server *s);
 
+/* ---- Reliability-guided propagation over the POI grid: replaces
+ * ReliabilityPropagation::propagate (src/subset/seed/reliability_propagation.cpp:130-394).
+ * Grid: step = spacing + 1, ceil(W / step) x ceil(H / step) points, row-major.
+ * Each seed (aligned to its nearest grid point and re-solved there when it
+ * moved, :206-262) starts a walk over 4-neighbours with the parent's affine
+ * extrapolation as guess (:314-327) and the reference's acceptance test
+ * (Success, valid, correlation < max_znssd, |du|, |dv| < step from the guess,
+ * finite; :359-368); rejected, unvisited and out-of-ROI points keep the
+ * default record (status InvalidInput, valid 0, x and y set).  The reference
+ * pops one point at a time from a heap ordered by correlation; here every
+ * frontier (all points accepted in the previous wave) is expanded at once,
+ * best correlation first, and its new neighbours are solved in one launch,
+ * so which parent seeds a neighbour, and hence which points a strict
+ * max_znssd rejects, can differ from the reference's order.  params->masked
+ * plays truncate_roi_subsets (solve_with_mask).  roi is host memory, W x H. */
+typedef struct dic_propagation_params {
+    int32_t spacing;      /* propagation_spacing (>= 0) */
+    int32_t reserved;
+    double max_znssd;     /* propagation_max_znssd */
+} dic_propagation_params;
+
+typedef struct dic_seed {
+    double x, y;          /* PropagationSeed::point */
+    dic_initial displacement; /* u, v, gradients; confidence = correlation */
+} dic_seed;
+
+/* grid_out must hold ceil(W/step) * ceil(H/step) records; *grid_w, *grid_h,
+ * *points_computed (may be NULL) receive the grid size and the count. */
+int dic_propagate(dic_ctx *ctx, const dic_icgn_params *params, const dic_propagation_params *prop,
+                  const dic_image *ref, const dic_grad *grad, const dic_coeffs *def, const uint8_t *roi_host,
+                  const dic_seed *seeds, int n_seeds, dic_result *grid_out, int *grid_w, int *grid_h,
+                  int *points_computed);
+
 #ifdef __cplusplus
 }
 #endif

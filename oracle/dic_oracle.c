@@ -1142,6 +1142,158 @@ void orc_fgn_solve_batch(const orc_config *cfg, const float *ref, int W, int H, 
     for (int i = 0; i < n; ++i) orc_fgn_solve(cfg, ref, W, H, dpre, roi, xy[2 * i], xy[2 * i + 1], &init[i], &out[i]);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Reliability-guided propagation, reliability_propagation.cpp:130-394: each */
+/* seed (aligned to the nearest grid point, :206-262) starts a best-first    */
+/* walk over the 4-neighbours (up, right, down, left) of the popped point     */
+/* (min-heap on correlation, :44-53); a neighbour is visited once (:298);     */
+/* outside the ROI or the subset margin it is only marked (:300-311); its     */
+/* guess is the parent's affine extrapolation (:314-327); it is accepted if   */
+/* Success, valid, correlation < max_znssd, |du|, |dv| < step from the guess  */
+/* and finite (:359-368), and only accepted points are stored and pushed.     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double corr;
+    int gx, gy;
+    double u, v, ux, uy, vx, vy;
+} prop_item;
+
+static void heap_push(prop_item *h, int *n, prop_item it)
+{
+    int i = (*n)++;
+    h[i] = it;
+    while (i > 0) {
+        const int pa = (i - 1) / 2;
+        if (!(h[i].corr < h[pa].corr)) break;
+        prop_item t = h[i];
+        h[i] = h[pa];
+        h[pa] = t;
+        i = pa;
+    }
+}
+
+static prop_item heap_pop(prop_item *h, int *n)
+{
+    prop_item top = h[0];
+    h[0] = h[--(*n)];
+    int i = 0;
+    for (;;) {
+        const int l = 2 * i + 1, r = l + 1;
+        int m = i;
+        if (l < *n && h[l].corr < h[m].corr) m = l;
+        if (r < *n && h[r].corr < h[m].corr) m = r;
+        if (m == i) break;
+        prop_item t = h[i];
+        h[i] = h[m];
+        h[m] = t;
+        i = m;
+    }
+    return top;
+}
+
+static void prop_solve(const orc_config *cfg, int optimizer, const float *ref, int W, int H, const double *gx,
+                       const double *gy, const orc_precomputed *dpre, const uint8_t *roi, double x, double y,
+                       const orc_initial *ini, orc_result *out)
+{
+    if (optimizer == 1) orc_fgn_solve(cfg, ref, W, H, dpre, roi, x, y, ini, out);
+    else orc_icgn_solve(cfg, ref, NULL, W, H, gx, gy, dpre, roi, x, y, ini, out);
+}
+
+int orc_propagate(const orc_config *cfg, int optimizer, int spacing, double max_znssd, int truncate_roi,
+                  const float *ref, int W, int H, const double *gxp, const double *gyp, const orc_precomputed *dpre,
+                  const uint8_t *roi, const double *seeds, int n_seeds, orc_result *grid)
+{
+    if (spacing < 0) spacing = 0;
+    const int step = spacing + 1, r = cfg->subset_radius;
+    const int gw = (W + step - 1) / step, gh = (H + step - 1) / step, total = gw * gh;
+    for (int i = 0; i < total; ++i) {
+        memset(&grid[i], 0, sizeof(orc_result));
+        grid[i].x = (double)((i % gw) * step);
+        grid[i].y = (double)((i / gw) * step);
+        grid[i].status = ORC_INVALID_INPUT;
+    }
+    if (W <= 0 || H <= 0 || roi == NULL || r < 1 || n_seeds <= 0) return 0;
+    const uint8_t *sroi = truncate_roi ? roi : NULL;
+    uint8_t *calc = (uint8_t *)calloc((size_t)total, 1);
+    prop_item *heap = (prop_item *)malloc(sizeof(prop_item) * ((size_t)total + 1));
+    int computed = 0;
+    for (int k = 0; k < n_seeds; ++k) {
+        const double *sd = seeds + 9 * k;
+        const int sxr = (int)round(sd[0]), syr = (int)round(sd[1]);
+        if (!(sxr >= 0 && syr >= 0 && sxr < W && syr < H) || !roi[(size_t)syr * W + sxr]) continue;
+        int gxs = (int)llround((double)sxr / step), gys = (int)llround((double)syr / step);
+        gxs = gxs < 0 ? 0 : (gxs > gw - 1 ? gw - 1 : gxs);
+        gys = gys < 0 ? 0 : (gys > gh - 1 ? gh - 1 : gys);
+        const int sx = gxs * step, sy = gys * step;
+        if (!(sx >= 0 && sy >= 0 && sx < W && sy < H) || !roi[(size_t)sy * W + sx]) continue;
+        const int sidx = gys * gw + gxs;
+        if (calc[sidx]) continue;
+        orc_result al;
+        memset(&al, 0, sizeof(al));
+        al.p[0] = sd[2], al.p[1] = sd[3], al.p[2] = sd[4], al.p[3] = sd[5], al.p[4] = sd[6], al.p[5] = sd[7];
+        al.correlation = sd[8];
+        if (sx != sxr || sy != syr) {
+            orc_initial ini = {sd[2], sd[3], sd[4], sd[5], sd[6], sd[7], sd[8]};
+            prop_solve(cfg, optimizer, ref, W, H, gxp, gyp, dpre, sroi, (double)sx, (double)sy, &ini, &al);
+            if (!al.valid || al.status != ORC_SUCCESS || !all_finite(al.p, 6) || !isfinite(al.correlation)) continue;
+        }
+        calc[sidx] = 1;
+        grid[sidx] = al;
+        grid[sidx].x = (double)sx;
+        grid[sidx].y = (double)sy;
+        grid[sidx].valid = 1;
+        grid[sidx].status = ORC_SUCCESS;
+        ++computed;
+        int hn = 0;
+        prop_item it0 = {al.correlation > 0.0 ? al.correlation : 0.0, gxs, gys, al.p[0], al.p[1], al.p[2],
+                         al.p[3], al.p[4], al.p[5]};
+        heap_push(heap, &hn, it0);
+        while (hn > 0) {
+            const prop_item it = heap_pop(heap, &hn);
+            const int px = it.gx * step, py = it.gy * step;
+            const int nb[4][2] = {{px, py - step}, {px + step, py}, {px, py + step}, {px - step, py}};
+            for (int q = 0; q < 4; ++q) {
+                const int nx = nb[q][0], ny = nb[q][1];
+                if (!(nx >= 0 && ny >= 0 && nx < W && ny < H)) continue;
+                const int gnx = nx / step, gny = ny / step;
+                if (gnx < 0 || gny < 0 || gnx >= gw || gny >= gh) continue;
+                const int nidx = gny * gw + gnx;
+                if (calc[nidx]) continue;
+                if (!roi[(size_t)ny * W + nx]) {
+                    calc[nidx] = 1;
+                    continue;
+                }
+                if (nx - r < 0 || ny - r < 0 || nx + r >= W || ny + r >= H) {
+                    calc[nidx] = 1;
+                    continue;
+                }
+                const double ddx = (double)(nx - px), ddy = (double)(ny - py);
+                const double iu = it.u + it.ux * ddx + it.uy * ddy;
+                const double iv = it.v + it.vx * ddx + it.vy * ddy;
+                orc_initial ini = {iu, iv, it.ux, it.uy, it.vx, it.vy, 0.0};
+                orc_result res;
+                prop_solve(cfg, optimizer, ref, W, H, gxp, gyp, dpre, sroi, (double)nx, (double)ny, &ini, &res);
+                calc[nidx] = 1;
+                const int ok = res.valid && res.status == ORC_SUCCESS && res.correlation < max_znssd &&
+                               fabs(iu - res.p[0]) < (double)step && fabs(iv - res.p[1]) < (double)step &&
+                               all_finite(res.p, 6) && isfinite(res.correlation);
+                if (ok) {
+                    grid[nidx] = res;
+                    grid[nidx].x = (double)nx;
+                    grid[nidx].y = (double)ny;
+                    ++computed;
+                    prop_item nxt = {res.correlation, gnx, gny, res.p[0], res.p[1], res.p[2], res.p[3], res.p[4],
+                                     res.p[5]};
+                    heap_push(heap, &hn, nxt);
+                }
+            }
+        }
+    }
+    free(calc);
+    free(heap);
+    return computed;
+}
+
 int orc_poi_grid(int W, int H, int spacing, double *xy)
 {
     if (spacing < 0) spacing = 0;

@@ -131,6 +131,16 @@ void orc_fgn_solve_batch(const orc_config *cfg, const float *reference, int widt
                          const orc_precomputed *deformed, const uint8_t *roi, const double *poi_xy,
                          const orc_initial *init, int n, orc_result *out, int threads);
 
+/* --- Reliability-guided propagation (src/subset/seed/reliability_propagation.cpp:130-394) --- */
+/* seeds: n x 9 doubles (x, y, u, v, du_dx, du_dy, dv_dx, dv_dy, correlation).
+ * grid: ceil(W / step) x ceil(H / step) results, step = spacing + 1.  The solver
+ * is IC-GN (optimizer 0) or forward Gauss-Newton (1); truncate_roi selects the
+ * masked solves.  Returns points_computed. */
+int orc_propagate(const orc_config *cfg, int optimizer, int spacing, double max_znssd, int truncate_roi,
+                  const float *reference, int width, int height, const double *grad_x, const double *grad_y,
+                  const orc_precomputed *deformed, const uint8_t *roi, const double *seeds, int n_seeds,
+                  orc_result *grid);
+
 /* POI grid of ReliabilityPropagation::propagate (reliability_propagation.cpp:154-176). */
 int orc_poi_grid(int width, int height, int spacing, double *xy_out /* may be NULL */);
 

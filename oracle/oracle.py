@@ -76,6 +76,9 @@ def load():
     lib.orc_fgn_compose.argtypes = [vp, vp, ci, vp]
     lib.orc_fgn_solve_batch.argtypes = [ctypes.POINTER(OrcConfig), vp, ci, ci, ctypes.POINTER(OrcPrecomputed), vp,
                                         vp, vp, ci, vp, ci]
+    lib.orc_propagate.argtypes = [ctypes.POINTER(OrcConfig), ci, ci, ctypes.c_double, ci, vp, ci, ci, vp, vp,
+                                  ctypes.POINTER(OrcPrecomputed), vp, vp, ci, vp]
+    lib.orc_propagate.restype = ci
     lib.orc_poi_grid.argtypes = [ci, ci, ci, vp]
     lib.orc_poi_grid.restype = ci
     _lib = lib
@@ -261,3 +264,26 @@ def fgn_solve(reference: np.ndarray, pre: Precomputed, poi_xy: np.ndarray, init:
     load().orc_fgn_solve_batch(ctypes.byref(cfg), _p(ref), W, H, ctypes.byref(pre.s), _p(r), _p(xy), _p(ini),
                                xy.shape[0], _p(out), int(threads))
     return out
+
+
+def propagate(reference: np.ndarray, pre: Precomputed, grad_x, grad_y, roi: np.ndarray, seeds: np.ndarray,
+              radius: int = 15, max_iterations: int = 30, threshold: float = 1e-3, shape_order: int = 1,
+              objective: int = 0, optimizer: int = 0, spacing: int = 5, max_znssd: float = 0.5,
+              truncate_roi: bool = False, min_valid_samples: int = 12, min_valid_sample_ratio: float = 0.5):
+    """ReliabilityPropagation::propagate (reliability_propagation.cpp:130-394).
+    seeds: (n, 9) x, y, u, v, du_dx, du_dy, dv_dx, dv_dy, correlation.
+    Returns (grid results (gh, gw), points_computed)."""
+    ref = np.ascontiguousarray(reference, np.float32)
+    H, W = ref.shape
+    step = max(spacing, 0) + 1
+    gw, gh = (W + step - 1) // step, (H + step - 1) // step
+    cfg = OrcConfig(int(radius), int(max_iterations), float(threshold), 1 if shape_order == 2 else 0, int(objective),
+                    int(min_valid_samples), 0, float(min_valid_sample_ratio))
+    gx = np.ascontiguousarray(grad_x, np.float64) if grad_x is not None else None
+    gy = np.ascontiguousarray(grad_y, np.float64) if grad_y is not None else None
+    r = np.ascontiguousarray(roi, np.uint8)
+    sd = np.ascontiguousarray(seeds, np.float64).reshape(-1, 9)
+    out = np.zeros(gw * gh, RESULT_DTYPE)
+    n = load().orc_propagate(ctypes.byref(cfg), int(optimizer), int(spacing), float(max_znssd), int(truncate_roi),
+                             _p(ref), W, H, _p(gx), _p(gy), ctypes.byref(pre.s), _p(r), _p(sd), sd.shape[0], _p(out))
+    return out.reshape(gh, gw), n

@@ -3,11 +3,13 @@
 Inputs (read as data, paths inside the Traditional-DIC tree):
   case/stereo_DIC/plate_center_load/cam1/00_L.bmp, cam1/04_L.bmp,
   cam2/00_R.bmp, cam2/04_R.bmp         8-bit speckle images, 1440 x 1080
+  ROI.bmp                               region of interest (read_mask: L > 0)
   tests/data/f0b/stereo_plate/fields/{reference_disparity,left_temporal,
   deformed_disparity}.npz               per-POI golden fields (97200 rows)
 
 Output: the four images as uint8 (the reference loads them with
-read_gray = float32 / max, workflows/subset_2d.py:21-27) and, per field,
+read_gray = float32 / max, workflows/subset_2d.py:21-27), the ROI as 0/1
+uint8 (read_mask, workflows/subset_2d.py:30-31) and, per field,
 the rows with valid == 1 (x, y, u, v, correlation) plus the row count.
 The fields were produced with config/subset_2d.yaml: radius 41, quintic
 B-spline, first-order IC-GN, ZNSSD, 50 iterations, threshold 1e-6, spacing 3.
@@ -37,6 +39,7 @@ def main(ref_root: str):
         f = np.asarray(im.convert("F"), dtype=np.float32)
         assert np.array_equal(f, u8.astype(np.float32)), "palette is not the identity grey ramp"
         out["img_" + key] = u8
+    out["roi"] = (np.asarray(Image.open(os.path.join(case, "ROI.bmp")).convert("L")) > 0).astype(np.uint8)
     fdir = os.path.join(ref_root, "tests", "data", "f0b", "stereo_plate", "fields")
     for name, (a, b) in FIELDS.items():
         z = np.load(os.path.join(fdir, name + ".npz"), allow_pickle=False)

@@ -1,0 +1,106 @@
+"""Reliability-guided propagation on the GPU (dic_propagate) against the
+reference's golden fields of the stereo-plate case (radius 41, quintic,
+first order, 50 iterations, ||dp|| < 1e-6, spacing 3, max ZNSSD 2.0, the
+reference's ROI), seeded at one golden POI: the valid set must equal the
+golden one exactly; u, v within 1e-5 px for >= 97% of the POIs and within
+1e-4 px for all (the golden run used other seeds, so guesses come from other
+parents; see tests/test_oracle_propagation.py).  On a window the GPU walk is
+also compared with the oracle's best-first walk from the same seed."""
+import numpy as np
+import pytest
+
+from conftest import gradient_init, read_gray
+from test_oracle_propagation import crop_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _report(msg):
+    print(msg, flush=True)
+
+
+def _cfg(pkg):
+    return pkg.config_from_dict({
+        "subset": {"radius": 41}, "shape_function": {"order": 1},
+        "optimization": {"method": "icgn", "max_iterations": 50, "convergence_threshold": 1e-6},
+        "correlation": {"criterion": "znssd"}, "interpolation": {"degree": 5},
+        "reliability_propagation": {"spacing": 3, "max_znssd": 2.0}})
+
+
+def _seed(pkg, x, y, u, v, g, corr):
+    s = np.zeros(1, pkg.SEED_DTYPE)
+    s["x"], s["y"] = x, y
+    d = s["displacement"]
+    d["u"], d["v"], d["du_dx"], d["du_dy"], d["dv_dx"], d["dv_dy"], d["confidence"] = u, v, *g, corr
+    return s
+
+
+def _compare(got_grid, golden):
+    flat = got_grid.ravel()
+    valid = flat[flat["valid"] == 1]
+    got = {(int(r["x"]), int(r["y"])): r for r in valid}
+    want = {(int(r[0]), int(r[1])): r for r in golden}
+    assert set(got) == set(want), (len(got), len(want), len(set(got) ^ set(want)))
+    keys = list(want)
+    du = np.array([max(abs(got[k]["p"][0] - want[k][2]), abs(got[k]["p"][1] - want[k][3])) for k in keys])
+    return du, keys
+
+
+# The golden value at this POI comes from the golden run's seed, solved on the
+# seed path's non-prefiltered interpolant (seed_selector.cpp:166-173), not by
+# propagation; every solver here lands ~3e-3 px away from it.
+GOLDEN_SEED_ARTIFACTS = {"left_temporal": {(852, 624)}}
+
+
+@pytest.mark.parametrize("name", ["left_temporal", "deformed_disparity", "reference_disparity"])
+def test_gpu_propagation_reproduces_golden_field(pkg, ctx, stereo, name):
+    rows = stereo["field_" + name]
+    a, b = stereo["pair_" + name]
+    ref, dfm = read_gray(stereo["img_" + a]), read_gray(stereo["img_" + b])
+    g = gradient_init(rows)
+    c = rows[:, :2].mean(0)
+    k = int(np.argmin(np.hypot(rows[:, 0] - c[0], rows[:, 1] - c[1])))
+    seed = _seed(pkg, rows[k, 0], rows[k, 1], rows[k, 2], rows[k, 3], g[k], rows[k, 4])
+    solver = pkg.ICGNSolver(_cfg(pkg), precision="fp64", context=ctx)
+    fields, grid, n = pkg.propagate(ref, dfm, stereo["roi"], seed, solver=solver)
+    assert grid.shape == (270, 360) and n == rows.shape[0]
+    du, keys = _compare(grid, rows)
+    far = {keys[i] for i in np.nonzero(du > 1e-4)[0]}
+    _report(f"gpu propagation {name}: {n} valid = golden, outside 1e-5: {(du > 1e-5).sum()}, "
+            f"beyond 1e-4: {sorted(far)} (max {du.max():.2e})")
+    assert (du > 1e-5).sum() <= 0.03 * du.size
+    assert far <= GOLDEN_SEED_ARTIFACTS.get(name, set())
+    assert fields["valid"].sum() == n and np.all(fields["u"][~fields["valid"]] == 0)
+
+
+def test_gpu_propagation_matches_oracle_window(pkg, ctx, orc, stereo):
+    ref, dfm, roi, seed_row, golden = crop_case(stereo, "left_temporal")
+    pre = orc.Precomputed(dfm, degree=5)
+    gx, gy = orc.reference_gradient_cache(ref)
+    cpu, ncpu = orc.propagate(ref, pre, gx, gy, roi, seed_row, radius=41, max_iterations=50, threshold=1e-6,
+                              spacing=3, max_znssd=2.0)
+    s = seed_row[0]
+    seed = _seed(pkg, s[0], s[1], s[2], s[3], s[4:8], s[8])
+    solver = pkg.ICGNSolver(_cfg(pkg), precision="fp64", context=ctx)
+    _, gpu, ngpu = pkg.propagate(ref, dfm, roi, seed, solver=solver)
+    assert ngpu == ncpu
+    assert np.array_equal(gpu["valid"], cpu["valid"])
+    v = cpu["valid"] == 1
+    d = np.abs(gpu["p"][v][:, :2] - cpu["p"][v][:, :2]).max(axis=1)  # noqa: E501
+    _report(f"gpu vs oracle propagation: {ngpu} valid each, max |du| {d.max():.2e}, outside 1e-5 {(d > 1e-5).sum()}")
+    assert (d > 1e-5).sum() <= 0.03 * d.size and d.max() <= 1e-4
+
+
+def test_gpu_propagation_strict_threshold_and_bad_seed(pkg, ctx, stereo):
+    rows = stereo["field_left_temporal"]
+    ref, dfm = read_gray(stereo["img_00_L"]), read_gray(stereo["img_04_L"])
+    cfg = _cfg(pkg)
+    cfg.propagation_max_znssd = 0.0  # nothing passes correlation < 0
+    solver = pkg.ICGNSolver(cfg, precision="fp64", context=ctx)
+    k = 100
+    seed = _seed(pkg, rows[k, 0], rows[k, 1], rows[k, 2], rows[k, 3], (0, 0, 0, 0), rows[k, 4])
+    _, grid, n = pkg.propagate(ref, dfm, stereo["roi"], seed, solver=solver)
+    assert n == 1 and grid["valid"].sum() == 1
+    outside = _seed(pkg, 5.0, 5.0, 0.0, 0.0, (0, 0, 0, 0), 0.0)  # outside the ROI
+    _, grid, n = pkg.propagate(ref, dfm, stereo["roi"], outside, solver=solver)
+    assert n == 0 and grid["valid"].sum() == 0 and np.all(grid["status"] == 2)

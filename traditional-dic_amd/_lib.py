@@ -43,6 +43,7 @@ EXPORTED_SYMBOLS = (
     "dic_batch_autotune", "dic_batch_destroy",
     "dic_host_alloc", "dic_host_free", "dic_server_create", "dic_server_submit", "dic_server_wait",
     "dic_server_frames_done", "dic_server_stats_get", "dic_server_now", "dic_server_error", "dic_server_destroy",
+    "dic_propagate",
 )
 
 # Displacement2D + iterations (dic_result); InitialDisplacement (dic_initial)
@@ -151,6 +152,13 @@ class ServerStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class PropagationParams(ctypes.Structure):
+    _fields_ = [("spacing", ctypes.c_int32), ("reserved", ctypes.c_int32), ("max_znssd", ctypes.c_double)]
+
+
+SEED_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("displacement", INITIAL_DTYPE)])
+
+
 class DicError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"[dic_hip error {code}] {message}")
@@ -214,6 +222,9 @@ _SIGNATURES = {
     "dic_server_now": ([_vp], ctypes.c_double),
     "dic_server_error": ([_vp], ctypes.c_char_p),
     "dic_server_destroy": ([_vp], None),
+    "dic_propagate": ([_vp, ctypes.POINTER(IcgnParams), ctypes.POINTER(PropagationParams), _vp, _vp, _vp, _vp, _vp,
+                       _i32, _vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                       ctypes.POINTER(ctypes.c_int)], _i32),
 }
 
 _lib = None
@@ -381,6 +392,23 @@ class Context:
         self.check(self.lib.dic_icgn_solve_batch(self.handle, ctypes.byref(params), ref.handle, grad.handle,
                                                  coeffs.handle, roi_p, _ptr(xy), _ptr(ini), n, _ptr(out)))
         return out
+
+    def propagate(self, params: IcgnParams, spacing: int, max_znssd: float, ref: "DeviceImage", grad,
+                  coeffs: "Coefficients", roi: np.ndarray, seeds: np.ndarray):
+        """dic_propagate: returns (grid results (gh, gw), points_computed)."""
+        r = np.ascontiguousarray(roi, np.uint8)
+        sd = np.ascontiguousarray(seeds, SEED_DTYPE)
+        step = max(int(spacing), 0) + 1
+        W, H = ref.width, ref.height
+        gw, gh = (W + step - 1) // step, (H + step - 1) // step
+        out = np.zeros(gw * gh, RESULT_DTYPE)
+        pp = PropagationParams(int(spacing), 0, float(max_znssd))
+        a, b, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.check(self.lib.dic_propagate(self.handle, ctypes.byref(params), ctypes.byref(pp), ref.handle,
+                                          grad.handle if grad is not None else None, coeffs.handle, _ptr(r),
+                                          _ptr(sd), sd.shape[0], _ptr(out), ctypes.byref(a), ctypes.byref(b),
+                                          ctypes.byref(n)))
+        return out.reshape(gh, gw), n.value
 
     def solve_device(self, params: IcgnParams, ref: "DeviceImage", grad: "Gradient", coeffs: "Coefficients",
                      xy_ptr: int, init_ptr: int, n: int, out_ptr: int, roi_ptr: int | None = None,

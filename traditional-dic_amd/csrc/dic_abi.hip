@@ -525,7 +525,11 @@ int dic_bspline_sample(dic_ctx *ctx, const dic_coeffs *c, const double *xy, int 
     return rc;
 }
 
-static int validate_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
+} // extern "C"
+
+namespace dic {
+
+int validate_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
                           const dic_coeffs *def, int n)
 {
     if (!params || !ref || !def || n < 0)
@@ -546,7 +550,7 @@ static int validate_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic
     return DIC_OK;
 }
 
-static int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
+int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
                      const dic_coeffs *def, const uint8_t *roi_dev, const double *xy, const dic_initial *init, int n,
                      dic_result *out, hipStream_t s)
 {
@@ -581,6 +585,10 @@ static int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_imag
     L.n = n;
     return check_hip(ctx, launch_icgn(L, s), "icgn launch");
 }
+
+} // namespace dic
+
+extern "C" {
 
 int dic_icgn_solve_batch_device(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref,
                                 const dic_grad *grad, const dic_coeffs *def, const uint8_t *roi_device,

@@ -59,6 +59,13 @@ void ctx_free(dic_ctx *ctx, void *ptr, size_t bytes);
 int set_error(dic_ctx *ctx, int code, const std::string &msg);
 int check_hip(dic_ctx *ctx, hipError_t e, const char *what);
 
+// dic_abi.hip: argument checks and one solve launch over device arrays
+int validate_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
+                   const dic_coeffs *def, int n);
+int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
+              const dic_coeffs *def, const uint8_t *roi_dev, const double *xy, const dic_initial *init, int n,
+              dic_result *out, hipStream_t s);
+
 // one-line sample table of a circular subset (dy outer, dx inner, dx^2+dy^2 <= r^2)
 int get_subset_table(dic_ctx *ctx, int radius, const int **table, int *count);
 

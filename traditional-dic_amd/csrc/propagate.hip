@@ -1,0 +1,218 @@
+// propagate.hip -- reliability-guided propagation over the POI grid
+// (dic_propagate): the reference's seed alignment, neighbour rules, guess
+// extrapolation and acceptance test, with each frontier solved in one launch.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "dic_internal.hpp"
+
+using namespace dic;
+
+namespace {
+
+struct Item { // reliability_propagation.cpp:44-53
+    double corr;
+    int gx, gy;
+    double u, v, ux, uy, vx, vy;
+};
+
+struct Cand {
+    int idx, nx, ny;
+    double iu, iv; // extrapolated guess
+    const Item *parent;
+};
+
+bool finite6(const dic_result &r)
+{
+    for (int j = 0; j < 6; ++j)
+        if (!std::isfinite(r.p[j])) return false;
+    return std::isfinite(r.correlation);
+}
+
+struct DeviceScratch {
+    dic_ctx *ctx = nullptr;
+    double *xy = nullptr;
+    dic_initial *init = nullptr;
+    dic_result *out = nullptr;
+    uint8_t *roi = nullptr;
+    size_t cap = 0, roi_bytes = 0;
+    ~DeviceScratch()
+    {
+        ctx_free(ctx, xy, 16 * cap);
+        ctx_free(ctx, init, sizeof(dic_initial) * cap);
+        ctx_free(ctx, out, sizeof(dic_result) * cap);
+        ctx_free(ctx, roi, roi_bytes);
+    }
+};
+
+// one launch over n POIs (host arrays in, host results out)
+int solve_host(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
+               const dic_coeffs *def, DeviceScratch &d, const std::vector<double> &xy,
+               const std::vector<dic_initial> &ini, std::vector<dic_result> &res)
+{
+    const int n = (int)ini.size();
+    res.resize(n);
+    if (n == 0) return DIC_OK;
+    hipStream_t s = ctx->stream;
+    int rc = check_hip(ctx, hipMemcpyAsync(d.xy, xy.data(), 16 * (size_t)n, hipMemcpyHostToDevice, s), "POI upload");
+    if (rc == DIC_OK)
+        rc = check_hip(ctx, hipMemcpyAsync(d.init, ini.data(), sizeof(dic_initial) * n, hipMemcpyHostToDevice, s),
+                       "guess upload");
+    if (rc == DIC_OK) rc = run_solve(ctx, params, ref, grad, def, params->masked ? d.roi : nullptr, d.xy, d.init, n,
+                                     d.out, s);
+    if (rc == DIC_OK)
+        rc = check_hip(ctx, hipMemcpyAsync(res.data(), d.out, sizeof(dic_result) * n, hipMemcpyDeviceToHost, s),
+                       "result download");
+    if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(s), "propagation sync");
+    return rc;
+}
+
+} // namespace
+
+extern "C" int dic_propagate(dic_ctx *ctx, const dic_icgn_params *params, const dic_propagation_params *prop,
+                             const dic_image *ref, const dic_grad *grad, const dic_coeffs *def,
+                             const uint8_t *roi, const dic_seed *seeds, int n_seeds, dic_result *grid_out,
+                             int *grid_w, int *grid_h, int *points_computed)
+{
+    if (!ctx || !prop || !grid_out || n_seeds < 0 || (n_seeds > 0 && !seeds)) return DIC_ERR_INVALID_ARGUMENT;
+    int rc = validate_solve(ctx, params, ref, grad, def, 0);
+    if (rc != DIC_OK) return rc;
+    const int W = ref->width, H = ref->height;
+    const int step = std::max(prop->spacing, 0) + 1;
+    const int gw = (W + step - 1) / step, gh = (H + step - 1) / step, total = gw * gh;
+    if (grid_w) *grid_w = gw;
+    if (grid_h) *grid_h = gh;
+    if (points_computed) *points_computed = 0;
+    for (int i = 0; i < total; ++i) { // defaults (:167-171)
+        dic_result &r = grid_out[i];
+        r = dic_result{};
+        r.x = (double)((i % gw) * step);
+        r.y = (double)((i / gw) * step);
+        r.status = DIC_INVALID_INPUT;
+    }
+    // input validation (:137-149): empty ROI or no seeds gives the empty result
+    if (!roi || W != def->width || H != def->height || params->subset_radius < 1 || n_seeds == 0) return DIC_OK;
+    const int radius = params->subset_radius;
+    const double max_znssd = prop->max_znssd;
+
+    DeviceScratch d;
+    d.ctx = ctx;
+    d.cap = (size_t)std::max(total, 1);
+    rc = check_hip(ctx, ctx_malloc(ctx, (void **)&d.xy, 16 * d.cap), "propagation buffers");
+    if (rc == DIC_OK) rc = check_hip(ctx, ctx_malloc(ctx, (void **)&d.init, sizeof(dic_initial) * d.cap), "buffers");
+    if (rc == DIC_OK) rc = check_hip(ctx, ctx_malloc(ctx, (void **)&d.out, sizeof(dic_result) * d.cap), "buffers");
+    if (rc == DIC_OK && params->masked) {
+        d.roi_bytes = (size_t)W * H;
+        rc = check_hip(ctx, ctx_malloc(ctx, (void **)&d.roi, d.roi_bytes), "roi buffer");
+        if (rc == DIC_OK)
+            rc = check_hip(ctx, hipMemcpy(d.roi, roi, d.roi_bytes, hipMemcpyHostToDevice), "roi upload");
+    }
+    if (rc != DIC_OK) return rc;
+
+    auto valid_px = [&](int x, int y) { return x >= 0 && y >= 0 && x < W && y < H && roi[(size_t)y * W + x]; };
+    std::vector<uint8_t> calculated((size_t)total, 0);
+    std::vector<int> claimed((size_t)total, -1);
+    std::vector<double> xy;
+    std::vector<dic_initial> ini;
+    std::vector<dic_result> res;
+    int computed = 0;
+
+    for (int k = 0; k < n_seeds; ++k) {
+        const dic_seed &sd = seeds[k];
+        const int sxr = (int)std::lround(sd.x), syr = (int)std::lround(sd.y);
+        if (!valid_px(sxr, syr)) continue;
+        const int gxs = std::clamp((int)std::llround((double)sxr / step), 0, gw - 1);
+        const int gys = std::clamp((int)std::llround((double)syr / step), 0, gh - 1);
+        const int sx = gxs * step, sy = gys * step;
+        if (!valid_px(sx, sy)) continue;
+        const int sidx = gys * gw + gxs;
+        if (calculated[sidx]) continue;
+        dic_result al{};
+        al.p[0] = sd.displacement.u;
+        al.p[1] = sd.displacement.v;
+        al.p[2] = sd.displacement.du_dx;
+        al.p[3] = sd.displacement.du_dy;
+        al.p[4] = sd.displacement.dv_dx;
+        al.p[5] = sd.displacement.dv_dy;
+        al.correlation = sd.displacement.confidence;
+        if (sx != sxr || sy != syr) { // re-solve the seed on the grid point (:233-261)
+            xy.assign({(double)sx, (double)sy});
+            ini.assign(1, sd.displacement);
+            rc = solve_host(ctx, params, ref, grad, def, d, xy, ini, res);
+            if (rc != DIC_OK) return rc;
+            al = res[0];
+            if (!al.valid || al.status != DIC_SUCCESS || !finite6(al)) continue;
+        }
+        calculated[sidx] = 1;
+        al.x = (double)sx;
+        al.y = (double)sy;
+        al.valid = 1;
+        al.status = DIC_SUCCESS;
+        grid_out[sidx] = al;
+        ++computed;
+        std::vector<Item> frontier{{std::max(0.0, al.correlation), gxs, gys, al.p[0], al.p[1], al.p[2], al.p[3],
+                                    al.p[4], al.p[5]}};
+        while (!frontier.empty()) {
+            // best (lowest correlation) parents claim their neighbours first
+            std::stable_sort(frontier.begin(), frontier.end(),
+                             [](const Item &a, const Item &b) { return a.corr < b.corr; });
+            std::vector<Cand> cands;
+            for (const Item &it : frontier) {
+                const int px = it.gx * step, py = it.gy * step;
+                const int nb[4][2] = {{px, py - step}, {px + step, py}, {px, py + step}, {px - step, py}};
+                for (int q = 0; q < 4; ++q) {
+                    const int nx = nb[q][0], ny = nb[q][1];
+                    if (!(nx >= 0 && ny >= 0 && nx < W && ny < H)) continue;
+                    const int gnx = nx / step, gny = ny / step;
+                    if (gnx < 0 || gny < 0 || gnx >= gw || gny >= gh) continue;
+                    const int nidx = gny * gw + gnx;
+                    if (calculated[nidx] || claimed[nidx] >= 0) continue;
+                    if (!roi[(size_t)ny * W + nx]) { // :300-304
+                        calculated[nidx] = 1;
+                        continue;
+                    }
+                    if (nx - radius < 0 || ny - radius < 0 || nx + radius >= W || ny + radius >= H) { // :306-311
+                        calculated[nidx] = 1;
+                        continue;
+                    }
+                    const double ddx = (double)(nx - px), ddy = (double)(ny - py);
+                    claimed[nidx] = (int)cands.size();
+                    cands.push_back({nidx, nx, ny, it.u + it.ux * ddx + it.uy * ddy, it.v + it.vx * ddx + it.vy * ddy,
+                                     &it});
+                }
+            }
+            if (cands.empty()) break;
+            xy.resize(2 * cands.size());
+            ini.resize(cands.size());
+            for (size_t c = 0; c < cands.size(); ++c) {
+                const Cand &cd = cands[c];
+                xy[2 * c] = (double)cd.nx;
+                xy[2 * c + 1] = (double)cd.ny;
+                ini[c] = {cd.iu, cd.iv, cd.parent->ux, cd.parent->uy, cd.parent->vx, cd.parent->vy, 0.0};
+            }
+            rc = solve_host(ctx, params, ref, grad, def, d, xy, ini, res);
+            if (rc != DIC_OK) return rc;
+            std::vector<Item> next;
+            for (size_t c = 0; c < cands.size(); ++c) {
+                const Cand &cd = cands[c];
+                const dic_result &r = res[c];
+                calculated[cd.idx] = 1;
+                claimed[cd.idx] = -1;
+                const bool accepted = r.valid && r.status == DIC_SUCCESS && r.correlation < max_znssd &&
+                                      std::fabs(cd.iu - r.p[0]) < (double)step &&
+                                      std::fabs(cd.iv - r.p[1]) < (double)step && finite6(r);
+                if (!accepted) continue;
+                grid_out[cd.idx] = r;
+                grid_out[cd.idx].x = (double)cd.nx;
+                grid_out[cd.idx].y = (double)cd.ny;
+                ++computed;
+                next.push_back({r.correlation, cd.nx / step, cd.ny / step, r.p[0], r.p[1], r.p[2], r.p[3], r.p[4],
+                                r.p[5]});
+            }
+            frontier.swap(next);
+        }
+    }
+    if (points_computed) *points_computed = computed;
+    return DIC_OK;
+}

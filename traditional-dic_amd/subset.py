@@ -233,6 +233,21 @@ def make_solver(config: SubsetConfig | dict | None = None, precision: str = "fp6
     return ICGNSolver(cfg, precision, **kw)
 
 
+def propagate(reference: np.ndarray, deformed: np.ndarray, roi: np.ndarray, seeds: np.ndarray,
+              config: SubsetConfig | dict | None = None, precision: str = "fp64",
+              solver: ICGNSolver | None = None):
+    """ReliabilityPropagation::propagate (reliability_propagation.cpp:130-394) on
+    the GPU from caller-supplied seeds (SEED_DTYPE: x, y, displacement).
+    Returns (results_to_dict of the grid, grid results (gh, gw), points_computed)."""
+    solver = solver or make_solver(config, precision)
+    cfg = solver.config
+    pair = solver.prepare(reference, deformed)
+    params = make_params(cfg, solver.precision, masked=bool(cfg.truncate_roi_subsets))
+    grid, n = solver.ctx.propagate(params, cfg.propagation_spacing, cfg.propagation_max_znssd, pair.ref, pair.grad,
+                                   pair.coeffs, np.asarray(roi, np.uint8), seeds)
+    return results_to_dict(grid.ravel()), grid, n
+
+
 def results_to_dict(results: np.ndarray) -> dict:
     """Same keys as bind_subset.cpp:102-153 (first-order parameters only)."""
     return {
