@@ -250,6 +250,57 @@ def stereo_leg(pkg, ctx, synth, steps=5):
     return out
 
 
+def propagation_leg(pkg, ctx, reps=3):
+    """Reliability-guided propagation over the reference's stereo-plate case
+    (1440x1080, ROI, radius 41, quintic, ||dp|| < 1e-6, spacing 3, one seed),
+    left_temporal pair, plain FP64: the GPU frontier walk (all 7654 ROI grid
+    points) timed end to end from resident images; the oracle's sequential
+    best-first walk timed on a 340x320 window of the same case (2688 POIs)
+    on one host thread, as the reference runs it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import gradient_init, read_gray  # fixture helpers (data only)
+    st = np.load(os.path.join(ROOT, "tests", "golden", "stereo_plate.npz"), allow_pickle=False)
+    rows = st["field_left_temporal"]
+    ref, dfm = read_gray(st["img_00_L"]), read_gray(st["img_04_L"])
+    g = gradient_init(rows)
+    c = rows[:, :2].mean(0)
+    k = int(np.argmin(np.hypot(rows[:, 0] - c[0], rows[:, 1] - c[1])))
+    seed = np.zeros(1, pkg.SEED_DTYPE)
+    seed["x"], seed["y"] = rows[k, 0], rows[k, 1]
+    d = seed["displacement"]
+    d["u"], d["v"], d["du_dx"], d["du_dy"], d["dv_dx"], d["dv_dy"], d["confidence"] = \
+        rows[k, 2], rows[k, 3], *g[k], rows[k, 4]
+    cfg = pkg.config_from_dict({
+        "subset": {"radius": 41}, "optimization": {"method": "icgn", "max_iterations": 50,
+                                                  "convergence_threshold": 1e-6},
+        "interpolation": {"degree": 5}, "reliability_propagation": {"spacing": 3, "max_znssd": 2.0}})
+    solver = pkg.ICGNSolver(cfg, precision="fp64", context=ctx)
+    pkg.propagate(ref, dfm, st["roi"], seed, solver=solver)  # warm (prepares the pair)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, grid, n = pkg.propagate(ref, dfm, st["roi"], seed, solver=solver)
+        ts.append(time.perf_counter() - t0)
+    t_gpu = float(np.median(ts))
+    from oracle import oracle as orc
+    from test_oracle_propagation import crop_case
+    cref, cdfm, croi, cseed, _ = crop_case(st, "left_temporal")
+    t0 = time.perf_counter()
+    pre = orc.Precomputed(cdfm, degree=5)
+    gx, gy = orc.reference_gradient_cache(cref)
+    _, ncpu = orc.propagate(cref, pre, gx, gy, croi, cseed, radius=41, max_iterations=50, threshold=1e-6, spacing=3,
+                            max_znssd=2.0)
+    t_cpu = time.perf_counter() - t0
+    out = {"workload": "stereo-plate left_temporal (reference case), 1440x1080, ROI, radius 41, quintic, "
+                       "thr 1e-6, spacing 3, one seed",
+           "gpu_points": int(n), "gpu_s": round(t_gpu, 4), "gpu_points_per_s": round(n / t_gpu, 1),
+           "cpu_oracle_points": int(ncpu), "cpu_oracle_s": round(t_cpu, 3),
+           "cpu_oracle_points_per_s": round(ncpu / t_cpu, 1), "cpu_threads": 1,
+           "valid_equals_golden": bool(int(grid["valid"].sum()) == rows.shape[0])}
+    log(f"propagation: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +312,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream serving leg")
     ap.add_argument("--serving-seconds", type=float, default=6.0)
     ap.add_argument("--no-stereo", action="store_true", help="skip the stereo matching leg")
+    ap.add_argument("--no-propagation", action="store_true", help="skip the propagation leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a cold 1 GiB device buffer (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -391,6 +443,7 @@ def main():
     batched = None if args.no_batched else batched_leg(pkg, ctx, synth)
     serving = None if args.no_serving else serving_leg(pkg, ctx, synth, duration_s=args.serving_seconds)
     stereo = None if args.no_stereo else stereo_leg(pkg, ctx, synth)
+    propagation = None if args.no_propagation else propagation_leg(pkg, ctx)
 
     line = {
         "metric": "converged POIs/s (FP32 vs FP64 ICGN, |du|<1e-3 px); p50/p99 request latency",
@@ -422,6 +475,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "stereo": stereo,
+        "propagation": propagation,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
