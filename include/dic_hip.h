@@ -115,6 +115,8 @@ void dic_event_destroy(void *event);
 int dic_image_upload(dic_ctx *ctx, const float *host, int width, int height, dic_image **out);
 /* Borrow device memory the caller keeps alive (no copy). */
 int dic_image_wrap_device(dic_ctx *ctx, const float *device_ptr, int width, int height, dic_image **out);
+/* Copies the W x H pixels to host (Image::data(), image.hpp:58-81). */
+int dic_image_download(dic_ctx *ctx, const dic_image *img, float *host);
 void dic_image_free(dic_image *img);
 
 /* ---- reference gradients ----
@@ -334,6 +336,108 @@ int dic_propagate(dic_ctx *ctx, const dic_icgn_params *params, const dic_propaga
                   const dic_image *ref, const dic_grad *grad, const dic_coeffs *def, const uint8_t *roi_host,
                   const dic_seed *seeds, int n_seeds, dic_result *grid_out, int *grid_w, int *grid_h,
                   int *points_computed);
+
+/* ---- Seeding: the step before propagation ----
+ * Integer-pixel search: replaces IntegerSearchInitializer
+ * (src/initialization/integer_search.cpp:303-645).  Per point, the ZNCC of a
+ * circular subset (radius subset_radius, FP64, the reference's two-pass
+ * mean/norm in its summation order, no fused multiply-add) is evaluated at
+ * integer shifts and the first strict maximum in row-major scan order wins:
+ *   DIC_SEARCH_GLOBAL  (estimate / estimate_with_mask): with pyramid_enabled,
+ *     every shift that keeps the subset inside the image on a grid of step
+ *     pyramid_scale with the subsampled subset, then the full subset within
+ *     max(pyramid_refinement_radius, ceil(1.5 r), step) of the coarse winner;
+ *     without it, every shift at step 1.  With search_radius > 0 the window
+ *     [-search_radius, search_radius]^2 is searched too and preferred when its
+ *     ZNCC + 0.02 >= the global one (:267-280).
+ *   DIC_SEARCH_AROUND_PRIOR (estimate[_with_mask]_around_displacement): the
+ *     window of radius search_radius around llround(prior).
+ * masked = 1 restricts the subset samples to the ROI (the _with_mask variants).
+ * Where the reference's unmasked around-prior form reads outside the image
+ * (it throws), the point is reported invalid. */
+enum { DIC_SEARCH_GLOBAL = 0, DIC_SEARCH_AROUND_PRIOR = 1 };
+
+typedef struct dic_integer_search_params { /* SeedIntegerSearchConfig, seed_config.hpp:33-40 */
+    int32_t subset_radius;             /* >= 1 */
+    int32_t search_radius;             /* >= 0 */
+    int32_t pyramid_enabled;
+    int32_t pyramid_scale;             /* >= 1 */
+    int32_t pyramid_refinement_radius; /* >= 1 */
+    int32_t mode;                      /* DIC_SEARCH_* */
+    int32_t masked;                    /* 1: roi required */
+    int32_t reserved;
+} dic_integer_search_params;
+
+/* InitialDisplacement with every field (initializer.hpp:33-44). */
+typedef struct dic_estimate {
+    double u, v, du_dx, du_dy, dv_dx, dv_dy, confidence;
+    int32_t valid, reserved;
+    double zncc, znssd;
+} dic_estimate;
+
+/* points_xy: 2n host doubles; prior_uv: 2n (AROUND_PRIOR only, else NULL);
+ * roi_host: W x H bytes (masked only); out: n host records.  ref and def must
+ * have the same size (else every point is invalid, as in the reference). */
+int dic_integer_search(dic_ctx *ctx, const dic_integer_search_params *params, const dic_image *ref,
+                       const dic_image *def, const uint8_t *roi_host, const double *points_xy,
+                       const double *prior_uv, int n, dic_estimate *out);
+
+/* Seed selection: replaces SeedSelector::generate_candidates and
+ * select_best_seed (src/subset/seed/seed_selector.cpp:146-316) on the
+ * integer-search path, with SubsetInitializer's sub-pixel refinement
+ * (src/initialization/subset_initializer.cpp:128-264).  Candidates are the
+ * rounded k-means centres of (a deterministic sample of) the ROI points that
+ * keep the subset margin, or, where a centre fails the ROI / margin / texture
+ * test, the most textured of the 50 sampled points nearest to it (equally
+ * distant points are taken in sampled order; the reference's std::sort leaves
+ * that order unspecified).  Each candidate is searched (GLOBAL, masked when
+ * truncate_roi_subsets) and refined by IC-GN on the non-prefiltered spline of
+ * the deformed image with central-difference reference gradients (or by
+ * forward Gauss-Newton on the prefiltered spline), all in FP64.  The SIFT
+ * prior (OpenCV) is not part of this library.  seed_count <= 2048. */
+enum { DIC_QUALITY_ZNCC = 0, DIC_QUALITY_ZNSSD = 1, DIC_QUALITY_SSD = 2 };
+
+typedef struct dic_seed_params {
+    dic_integer_search_params search;  /* mode and masked are set by the selector */
+    int32_t subpixel_enabled;          /* SeedSubpixelRefinementConfig, seed_config.hpp:42-50 */
+    int32_t subpixel_shape_order;      /* 1 or 2 */
+    int32_t subpixel_optimizer;        /* DIC_OPTIMIZER_* */
+    int32_t subpixel_objective;        /* DIC_OBJECTIVE_* */
+    int32_t subpixel_radius;
+    int32_t subpixel_max_iterations;
+    double subpixel_convergence_threshold;
+    int32_t seed_count;                /* SeedSelectionConfig, seed_config.hpp:58-69 */
+    int32_t quality_metric;            /* DIC_QUALITY_* */
+    double min_zncc, max_znssd, max_ssd, min_displacement_norm, min_texture_std;
+    int32_t kmeans_iterations;
+    int32_t kmeans_sample_limit;
+    int32_t truncate_roi_subsets;      /* SubsetConfig, subset_config.hpp:33-50 */
+    int32_t min_valid_samples;
+    double min_valid_sample_ratio;
+    int32_t degree;                    /* image_precompute: spline degree and border */
+    int32_t border;
+} dic_seed_params;
+
+/* SeedEvaluation (seed_selector.hpp:20-27). */
+typedef struct dic_seed_evaluation {
+    double x, y;
+    dic_result displacement;           /* Displacement2D; iterations unused */
+    double quality, displacement_norm;
+    int32_t quality_passed, valid;
+} dic_seed_evaluation;
+
+/* Candidate points (host xy pairs, at most capacity) into xy_out, count in *n_out. */
+int dic_seed_candidates(dic_ctx *ctx, const dic_seed_params *params, const dic_image *ref,
+                        const uint8_t *roi_host, double *xy_out, int capacity, int *n_out);
+/* Evaluations of every candidate (at most capacity) and the index of the best
+ * seed in *best_index (-1: SeedSelectionResult::found == false). */
+int dic_select_seed(dic_ctx *ctx, const dic_seed_params *params, const dic_image *ref, const dic_image *def,
+                    const uint8_t *roi_host, dic_seed_evaluation *evaluations, int capacity, int *n_out,
+                    int *best_index);
+
+/* mirror_pad_image (src/subset/padding.cpp:11-59): a new (W+2p) x (H+2p) image
+ * whose border repeats the image mirrored about its edges (edge pixels doubled). */
+int dic_image_mirror_pad(dic_ctx *ctx, const dic_image *img, int pad, dic_image **out);
 
 #ifdef __cplusplus
 }

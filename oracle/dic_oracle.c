@@ -1306,3 +1306,633 @@ int orc_poi_grid(int W, int H, int spacing, double *xy)
         }
     return gw * gh;
 }
+
+/* ======================================================================
+ * Seeding: integer-pixel ZNCC search and seed selection
+ * ====================================================================== */
+
+#define ORC_SEARCH_EPS 1e-12    /* kEpsilon, integer_search.cpp:10 */
+#define ORC_LOCAL_MARGIN 0.02   /* kLocalPreferenceZnccMargin, :11 */
+
+typedef struct {
+    int dx, dy;
+    double val;
+} orc_sample;
+
+typedef struct {
+    int u, v;
+    double z;
+} orc_cand;
+
+static int imax(int a, int b) { return a > b ? a : b; }
+static int imin(int a, int b) { return a < b ? a : b; }
+
+/* full_circle_sample_count, :35-47 */
+static int circle_count(int r, int step)
+{
+    step = imax(1, step);
+    int c = 0;
+    for (int dy = -r; dy <= r; dy += step)
+        for (int dx = -r; dx <= r; dx += step)
+            if (dx * dx + dy * dy <= r * r) ++c;
+    return c;
+}
+
+/* min_required_samples, :49-52 */
+static int min_required(int r, int step)
+{
+    return imax(12, (int)ceil(0.5 * (double)circle_count(r, step)));
+}
+
+/* collect_circular_reference_samples (:62-79) when roi is NULL, else
+ * collect_masked_reference_samples (:81-105).  The unmasked form returns -1
+ * where a sample leaves the image (Image::at throws there, image.cpp:154-160). */
+static int collect_samples(const float *ref, int W, int H, const uint8_t *roi, int cx, int cy, int r, int step,
+                           orc_sample *s)
+{
+    step = imax(1, step);
+    int n = 0;
+    for (int dy = -r; dy <= r; dy += step)
+        for (int dx = -r; dx <= r; dx += step) {
+            if (dx * dx + dy * dy > r * r) continue;
+            const int x = cx + dx, y = cy + dy;
+            const int inside = x >= 0 && y >= 0 && x < W && y < H;
+            if (roi) {
+                if (!inside || !roi[(size_t)y * W + x]) continue;
+            } else if (!inside) {
+                return -1;
+            }
+            s[n].dx = dx;
+            s[n].dy = dy;
+            s[n].val = (double)ref[(size_t)y * W + x];
+            ++n;
+        }
+    return n;
+}
+
+/* sample_mean / sample_norm, :107-125 */
+static void sample_stats(const orc_sample *s, int n, double *mean, double *norm)
+{
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) m += s[i].val;
+    m = n > 0 ? m / (double)n : 0.0;
+    double q = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double d = s[i].val - m;
+        q += d * d;
+    }
+    *mean = m;
+    *norm = q;
+}
+
+/* circular_subset_zncc, :127-165 */
+static double shift_zncc(const float *def, int W, int H, const orc_sample *s, int n, double rm, double rn, int cx,
+                         int cy, int u, int v)
+{
+    if (n == 0 || rn <= ORC_SEARCH_EPS) return -INFINITY;
+    double dm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const int x = cx + u + s[i].dx, y = cy + v + s[i].dy;
+        if (!(x >= 0 && y >= 0 && x < W && y < H)) return -INFINITY;
+        dm += (double)def[(size_t)y * W + x];
+    }
+    dm /= (double)n;
+    double num = 0.0, dn = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double rd = s[i].val - rm;
+        const double dd = (double)def[(size_t)(cy + v + s[i].dy) * W + (cx + u + s[i].dx)] - dm;
+        num += rd * dd;
+        dn += dd * dd;
+    }
+    if (dn <= ORC_SEARCH_EPS) return -INFINITY;
+    return num / sqrt(rn * dn);
+}
+
+/* search_displacements, :167-206: row-major scan, the first strict maximum wins */
+static orc_cand search_window(const float *def, int W, int H, const orc_sample *s, int n, double rm, double rn,
+                              int cx, int cy, int r, int u0, int u1, int v0, int v1, int step)
+{
+    orc_cand best = {0, 0, -INFINITY};
+    step = imax(1, step);
+    for (int v = v0; v <= v1; v += step)
+        for (int u = u0; u <= u1; u += step) {
+            const int x = cx + u, y = cy + v;
+            if (!(x - r >= 0 && y - r >= 0 && x + r < W && y + r < H)) continue; /* subset_in_bounds :54-60 */
+            const double z = shift_zncc(def, W, H, s, n, rm, rn, cx, cy, u, v);
+            if (z > best.z) {
+                best.u = u;
+                best.v = v;
+                best.z = z;
+            }
+        }
+    return best;
+}
+
+/* search_ncorr_multigrid, :208-265 */
+static orc_cand search_multigrid(const float *def, int W, int H, const orc_sample *s, int n, double rm, double rn,
+                                 const orc_sample *cs_, int cn, double crm, double crn, int cx, int cy, int r,
+                                 int coarse_step, int refine_radius)
+{
+    orc_cand none = {0, 0, -INFINITY};
+    const int min_u = r - cx, max_u = W - 1 - r - cx;
+    const int min_v = r - cy, max_v = H - 1 - r - cy;
+    if (min_u > max_u || min_v > max_v) return none;
+    coarse_step = imax(1, coarse_step);
+    const orc_cand coarse =
+        search_window(def, W, H, cs_, cn, crm, crn, cx, cy, r, min_u, max_u, min_v, max_v, coarse_step);
+    if (!isfinite(coarse.z)) return coarse;
+    const int rr = imax(refine_radius, coarse_step);
+    return search_window(def, W, H, s, n, rm, rn, cx, cy, r, imax(min_u, coarse.u - rr), imin(max_u, coarse.u + rr),
+                         imax(min_v, coarse.v - rr), imin(max_v, coarse.v + rr), 1);
+}
+
+/* prefer_local_when_ambiguous, :267-280 */
+static orc_cand prefer_local(orc_cand global, orc_cand local)
+{
+    if (!isfinite(local.z)) return global;
+    if (!isfinite(global.z)) return local;
+    if (local.z + ORC_LOCAL_MARGIN >= global.z) return local;
+    return global;
+}
+
+/* znssd_from_zncc, :22-28 */
+static double znssd_from_zncc(double z)
+{
+    if (!isfinite(z)) return INFINITY;
+    const double c = z < -1.0 ? -1.0 : (z > 1.0 ? 1.0 : z);
+    const double q = 2.0 * (1.0 - c);
+    return q > 0.0 ? q : 0.0;
+}
+
+static void estimate_from(orc_cand best, orc_estimate *out)
+{
+    memset(out, 0, sizeof(*out));
+    if (!isfinite(best.z)) return;
+    out->u = (double)best.u;
+    out->v = (double)best.v;
+    out->confidence = znssd_from_zncc(best.z);
+    out->valid = 1;
+    out->zncc = best.z;
+    out->znssd = out->confidence;
+}
+
+/* One point of IntegerSearchInitializer (constructor clamps :293-301). */
+static void integer_search_one(const orc_search_params *p, const float *ref, const float *def, int W, int H,
+                               const uint8_t *roi, double px, double py, const double *prior, orc_sample *s,
+                               orc_sample *cs_, orc_estimate *out)
+{
+    memset(out, 0, sizeof(*out));
+    const int r = imax(1, p->subset_radius);
+    const int sr = imax(0, p->search_radius);
+    const int scale = imax(1, p->pyramid_scale);
+    const int refine_param = imax(1, p->pyramid_refinement_radius);
+    const uint8_t *mroi = p->masked ? roi : NULL;
+    if (p->around_prior) {
+        /* estimate_around_displacement :519-579 / _with_mask_ :581-645 */
+        if (!isfinite(prior[0]) || !isfinite(prior[1])) return;
+        const int cx = (int)llround(px), cy = (int)llround(py);
+        const int n = collect_samples(ref, W, H, mroi, cx, cy, r, 1, s);
+        if (n < min_required(r, 1)) return;
+        double rm, rn;
+        sample_stats(s, n, &rm, &rn);
+        if (rn <= ORC_SEARCH_EPS) return;
+        const int pu = (int)llround(prior[0]), pv = (int)llround(prior[1]);
+        const int min_u = r - cx, max_u = W - 1 - r - cx, min_v = r - cy, max_v = H - 1 - r - cy;
+        const orc_cand best = search_window(def, W, H, s, n, rm, rn, cx, cy, r, imax(min_u, pu - sr),
+                                            imin(max_u, pu + sr), imax(min_v, pv - sr), imin(max_v, pv + sr), 1);
+        estimate_from(best, out);
+        return;
+    }
+    const int cx = (int)round(px), cy = (int)round(py);
+    int n, cn;
+    double rm, rn, crm, crn;
+    const int coarse_step = p->pyramid_enabled ? scale : 1;
+    const int refine_radius = imax(refine_param, (int)ceil(1.5 * (double)r));
+    orc_cand best = {0, 0, -INFINITY};
+    if (!mroi) {
+        /* estimate_with_interpolators :318-403 */
+        if (!(cx - r >= 0 && cy - r >= 0 && cx + r < W && cy + r < H)) return;
+        n = collect_samples(ref, W, H, NULL, cx, cy, r, 1, s);
+        sample_stats(s, n, &rm, &rn);
+        if (n <= 0 || rn <= ORC_SEARCH_EPS) return;
+        cn = collect_samples(ref, W, H, NULL, cx, cy, r, coarse_step, cs_);
+        sample_stats(cs_, cn, &crm, &crn);
+        if (cn > 0 && crn > ORC_SEARCH_EPS)
+            best = search_multigrid(def, W, H, s, n, rm, rn, cs_, cn, crm, crn, cx, cy, r, coarse_step, refine_radius);
+    } else {
+        /* estimate_with_mask_interpolators :425-517 */
+        if (!(cx >= 0 && cy >= 0 && cx < W && cy < H) || !mroi[(size_t)cy * W + cx]) return;
+        n = collect_samples(ref, W, H, mroi, cx, cy, r, 1, s);
+        if (n < min_required(r, 1)) return;
+        sample_stats(s, n, &rm, &rn);
+        if (rn <= ORC_SEARCH_EPS) return;
+        cn = collect_samples(ref, W, H, mroi, cx, cy, r, coarse_step, cs_);
+        sample_stats(cs_, cn, &crm, &crn);
+        if (cn >= min_required(r, coarse_step) && crn > ORC_SEARCH_EPS)
+            best = search_multigrid(def, W, H, s, n, rm, rn, cs_, cn, crm, crn, cx, cy, r, coarse_step, refine_radius);
+    }
+    if (sr > 0) {
+        const orc_cand local = search_window(def, W, H, s, n, rm, rn, cx, cy, r, -sr, sr, -sr, sr, 1);
+        best = prefer_local(best, local);
+    }
+    estimate_from(best, out);
+}
+
+void orc_integer_search(const orc_search_params *p, const float *ref, const float *def, int W, int H,
+                        const uint8_t *roi, const double *xy, const double *prior_uv, int n, orc_estimate *out,
+                        int threads)
+{
+    const int r = imax(1, p->subset_radius);
+    const size_t cap = (size_t)(2 * r + 1) * (size_t)(2 * r + 1);
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel num_threads(threads)
+    {
+        orc_sample *s = (orc_sample *)malloc(cap * sizeof(orc_sample));
+        orc_sample *c = (orc_sample *)malloc(cap * sizeof(orc_sample));
+#pragma omp for schedule(dynamic, 1)
+        for (int i = 0; i < n; ++i)
+            integer_search_one(p, ref, def, W, H, roi, xy[2 * i], xy[2 * i + 1],
+                               prior_uv ? prior_uv + 2 * i : NULL, s, c, out + i);
+        free(s);
+        free(c);
+    }
+}
+
+double orc_subset_zncc(const float *ref, const float *def, int W, int H, int r, int cx, int cy, int u, int v)
+{
+    orc_sample *s = (orc_sample *)malloc((size_t)(2 * r + 1) * (2 * r + 1) * sizeof(orc_sample));
+    const int n = collect_samples(ref, W, H, NULL, cx, cy, r, 1, s);
+    double z = -INFINITY;
+    if (n > 0) {
+        double rm, rn;
+        sample_stats(s, n, &rm, &rn);
+        z = shift_zncc(def, W, H, s, n, rm, rn, cx, cy, u, v);
+    }
+    free(s);
+    return z;
+}
+
+/* ---- seed selection (seed_selector.cpp) ---- */
+
+double orc_texture_std(const float *ref, int W, int H, int x, int y, int r)
+{
+    if (x - r < 0 || y - r < 0 || x + r >= W || y + r >= H) return 0.0; /* :425-430 */
+    double sum = 0.0, sq = 0.0;
+    int count = 0;
+    for (int yy = y - r; yy <= y + r; ++yy)
+        for (int xx = x - r; xx <= x + r; ++xx) {
+            const double v = (double)ref[(size_t)yy * W + xx];
+            sum += v;
+            sq += v * v;
+            ++count;
+        }
+    if (count == 0) return 0.0;
+    const double mean = sum / (double)count;
+    const double var = sq / (double)count - mean * mean;
+    return sqrt(var > 0.0 ? var : 0.0);
+}
+
+/* deterministic_sample :34-48.  With limit == 1 the reference's step is
+ * (n-1)/0 = inf, 0 * inf = NaN and llround(NaN) converts to an index past the
+ * end, which min() clamps to the last point (x86-64). */
+int orc_deterministic_sample(int n, int limit, int *idx)
+{
+    if (limit <= 0 || n <= limit) {
+        for (int i = 0; i < n; ++i) idx[i] = i;
+        return n;
+    }
+    if (limit == 1) {
+        idx[0] = n - 1;
+        return 1;
+    }
+    const double step = (double)(n - 1) / (double)(limit - 1);
+    for (int i = 0; i < limit; ++i) {
+        long long k = llround((double)i * step);
+        idx[i] = k > n - 1 ? n - 1 : (int)k;
+    }
+    return limit;
+}
+
+static double dist2(double x0, double y0, double x1, double y1)
+{
+    const double dx = x0 - x1, dy = y0 - y1;
+    return dx * dx + dy * dy;
+}
+
+/* initialize_centers (:50-68) + run_kmeans (:70-110) */
+int orc_kmeans(const int *pts, int n, int requested, int iterations, double *c)
+{
+    if (n <= 0 || requested <= 0) return 0;
+    const int k = imin(requested, n);
+    const double step = k == 1 ? 0.0 : (double)(n - 1) / (double)(k - 1);
+    for (int i = 0; i < k; ++i) {
+        long long j = llround((double)i * step);
+        if (j > n - 1) j = n - 1;
+        c[2 * i] = (double)pts[2 * j];
+        c[2 * i + 1] = (double)pts[2 * j + 1];
+    }
+    if (iterations <= 0) return k;
+    double *sums = (double *)malloc(sizeof(double) * 2 * k);
+    int *counts = (int *)malloc(sizeof(int) * k);
+    for (int it = 0; it < iterations; ++it) {
+        memset(sums, 0, sizeof(double) * 2 * k);
+        memset(counts, 0, sizeof(int) * k);
+        for (int p = 0; p < n; ++p) {
+            const double px = (double)pts[2 * p], py = (double)pts[2 * p + 1];
+            int best = 0;
+            double bd = INFINITY;
+            for (int i = 0; i < k; ++i) {
+                const double d = dist2(px, py, c[2 * i], c[2 * i + 1]);
+                if (d < bd) {
+                    bd = d;
+                    best = i;
+                }
+            }
+            sums[2 * best] += px;
+            sums[2 * best + 1] += py;
+            ++counts[best];
+        }
+        for (int i = 0; i < k; ++i)
+            if (counts[i] > 0) {
+                c[2 * i] = sums[2 * i] / (double)counts[i];
+                c[2 * i + 1] = sums[2 * i + 1] / (double)counts[i];
+            }
+    }
+    free(sums);
+    free(counts);
+    return k;
+}
+
+static int seed_radius(const orc_seed_params *p)
+{
+    return imax(p->search.subset_radius, p->subpixel_radius);
+}
+
+/* is_candidate_margin_valid, :385-419 */
+static int margin_valid(const orc_seed_params *p, const uint8_t *roi, int W, int H, int x, int y)
+{
+    const int radius = seed_radius(p);
+    const int margin = radius + (p->truncate_roi_subsets ? p->search.search_radius : 0);
+    if (x - margin < 0 || y - margin < 0 || x + margin >= W || y + margin >= H) return 0;
+    if (!p->truncate_roi_subsets) return 1;
+    int valid = 0, full = 0;
+    for (int yy = y - radius; yy <= y + radius; ++yy)
+        for (int xx = x - radius; xx <= x + radius; ++xx) {
+            const int dx = xx - x, dy = yy - y;
+            if (dx * dx + dy * dy <= radius * radius) {
+                ++full;
+                if (xx >= 0 && yy >= 0 && xx < W && yy < H && roi[(size_t)yy * W + xx]) ++valid;
+            }
+        }
+    const int need = imax(p->min_valid_samples, (int)ceil(p->min_valid_sample_ratio * (double)full));
+    return valid >= imax(6, need);
+}
+
+static int roi_valid(const uint8_t *roi, int W, int H, int x, int y)
+{
+    return x >= 0 && y >= 0 && x < W && y < H && roi[(size_t)y * W + x];
+}
+
+typedef struct {
+    double d;
+    int i;
+} orc_dist_index;
+
+static int cmp_dist_index(const void *a, const void *b)
+{
+    const orc_dist_index *x = (const orc_dist_index *)a, *y = (const orc_dist_index *)b;
+    if (x->d < y->d) return -1;
+    if (x->d > y->d) return 1;
+    return (x->i > y->i) - (x->i < y->i);
+}
+
+/* generate_candidates, :208-316.  The fallback's nearest-point order uses
+ * std::sort (:293-298), which leaves the order of equally distant points
+ * unspecified; here equal distances keep the sampled order. */
+int orc_seed_candidates(const orc_seed_params *p, const float *ref, int W, int H, const uint8_t *roi,
+                        double *xy_out, int cap)
+{
+    const int radius = seed_radius(p);
+    const int margin = radius + (p->truncate_roi_subsets ? p->search.search_radius : 0);
+    int *pts = (int *)malloc(sizeof(int) * 2 * (size_t)W * H);
+    int n = 0;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+            if (roi[(size_t)y * W + x] && x - margin >= 0 && y - margin >= 0 && x + margin < W && y + margin < H) {
+                pts[2 * n] = x;
+                pts[2 * n + 1] = y;
+                ++n;
+            }
+    if (n == 0)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x)
+                if (roi[(size_t)y * W + x]) {
+                    pts[2 * n] = x;
+                    pts[2 * n + 1] = y;
+                    ++n;
+                }
+    int count = 0;
+    if (n == 0) {
+        free(pts);
+        return 0;
+    }
+    const int requested = imax(1, p->seed_count);
+    const int limit = imax(1, p->kmeans_sample_limit);
+    int *idx = (int *)malloc(sizeof(int) * (size_t)(n < limit ? n : limit));
+    const int ns = orc_deterministic_sample(n, limit, idx);
+    int *sp = (int *)malloc(sizeof(int) * 2 * (size_t)ns);
+    for (int i = 0; i < ns; ++i) {
+        sp[2 * i] = pts[2 * idx[i]];
+        sp[2 * i + 1] = pts[2 * idx[i] + 1];
+    }
+    free(idx);
+    free(pts);
+    const int tr = p->search.subset_radius;
+    if (requested == 1) {
+        double best = -1.0;
+        int found = 0, bx = 0, by = 0;
+        for (int i = 0; i < ns; ++i) {
+            if (!margin_valid(p, roi, W, H, sp[2 * i], sp[2 * i + 1])) continue;
+            const double t = orc_texture_std(ref, W, H, sp[2 * i], sp[2 * i + 1], tr);
+            if (!found || t > best) {
+                best = t;
+                bx = sp[2 * i];
+                by = sp[2 * i + 1];
+                found = 1;
+            }
+        }
+        if (found && cap > 0) {
+            xy_out[0] = (double)bx;
+            xy_out[1] = (double)by;
+            count = 1;
+        }
+        free(sp);
+        return count;
+    }
+    double *centers = (double *)malloc(sizeof(double) * 2 * (size_t)imin(requested, ns));
+    const int k = orc_kmeans(sp, ns, requested, imax(0, p->kmeans_iterations), centers);
+    orc_dist_index *order = (orc_dist_index *)malloc(sizeof(orc_dist_index) * (size_t)ns);
+    for (int c = 0; c < k && count < cap; ++c) {
+        const double cx = centers[2 * c], cy = centers[2 * c + 1];
+        const int rx = (int)llround(cx), ry = (int)llround(cy);
+        if (roi_valid(roi, W, H, rx, ry) && margin_valid(p, roi, W, H, rx, ry) &&
+            orc_texture_std(ref, W, H, rx, ry, tr) >= p->min_texture_std) {
+            xy_out[2 * count] = (double)rx;
+            xy_out[2 * count + 1] = (double)ry;
+            ++count;
+            continue;
+        }
+        for (int i = 0; i < ns; ++i) {
+            order[i].d = dist2((double)sp[2 * i], (double)sp[2 * i + 1], cx, cy);
+            order[i].i = i;
+        }
+        qsort(order, (size_t)ns, sizeof(orc_dist_index), cmp_dist_index);
+        const int lim = imin(50, ns);
+        double best = -1.0;
+        int bi = order[0].i;
+        for (int i = 0; i < lim; ++i) {
+            const int j = order[i].i;
+            const double t = orc_texture_std(ref, W, H, sp[2 * j], sp[2 * j + 1], tr);
+            if (t > best) {
+                best = t;
+                bi = j;
+            }
+        }
+        xy_out[2 * count] = (double)sp[2 * bi];
+        xy_out[2 * count + 1] = (double)sp[2 * bi + 1];
+        ++count;
+    }
+    free(order);
+    free(centers);
+    free(sp);
+    return count;
+}
+
+/* quality_passes, :448-460 */
+static int quality_passes(const orc_seed_params *p, double q)
+{
+    if (p->quality_metric == 0) return q >= p->min_zncc;
+    if (p->quality_metric == 2) return q <= p->max_ssd;
+    return q <= p->max_znssd;
+}
+
+/* seed_is_better, :112-136 */
+static int seed_is_better(const orc_seed_eval *a, const orc_seed_eval *cur, int metric)
+{
+    const double eps = 1.0e-9;
+    if (!cur->valid) return 1;
+    if (metric == 0) {
+        if (a->quality > cur->quality + eps) return 1;
+        if (a->quality < cur->quality - eps) return 0;
+    } else {
+        if (a->quality < cur->quality - eps) return 1;
+        if (a->quality > cur->quality + eps) return 0;
+    }
+    return a->displacement_norm > cur->displacement_norm;
+}
+
+/* zncc_from_znssd, subset_initializer.cpp:13-19 */
+static double zncc_from_znssd(double q)
+{
+    if (!isfinite(q)) return -1.0;
+    const double z = 1.0 - 0.5 * q;
+    return z < -1.0 ? -1.0 : (z > 1.0 ? 1.0 : z);
+}
+
+/* select_best_seed (:146-206) with evaluate_candidate (:466-539) on the
+ * integer-search path (no SIFT matches) and SubsetInitializer's refinement
+ * (subset_initializer.cpp:128-264): IC-GN on the non-prefiltered lazy spline
+ * of the deformed image with central-difference reference gradients, or
+ * forward Gauss-Newton on the configured spline. */
+int orc_select_seed(const orc_seed_params *p, const float *ref, const float *def, int W, int H, const uint8_t *roi,
+                    orc_seed_eval *out, int cap, int *n_out)
+{
+    *n_out = 0;
+    const int kmax = imax(1, p->seed_count);
+    double *xy = (double *)malloc(sizeof(double) * 2 * (size_t)kmax);
+    const int k = orc_seed_candidates(p, ref, W, H, roi, xy, imin(cap, kmax));
+    orc_search_params sp = p->search;
+    sp.around_prior = 0;
+    sp.masked = p->truncate_roi_subsets ? 1 : 0;
+    orc_estimate *est = (orc_estimate *)calloc((size_t)(k > 0 ? k : 1), sizeof(orc_estimate));
+    orc_integer_search(&sp, ref, def, W, H, roi, xy, NULL, k, est, 0);
+    if (p->subpixel_enabled && k > 0) {
+        orc_config cfg;
+        memset(&cfg, 0, sizeof(cfg));
+        cfg.subset_radius = imax(1, p->subpixel_radius);
+        cfg.max_iterations = imax(0, p->subpixel_max_iterations);
+        cfg.convergence_threshold = p->subpixel_convergence_threshold;
+        cfg.second_order = p->subpixel_shape_order == 2;
+        cfg.objective = p->subpixel_objective;
+        cfg.min_valid_samples = p->min_valid_samples;
+        cfg.min_valid_sample_ratio = p->min_valid_sample_ratio;
+        orc_precomputed pre;
+        const int fgn = p->subpixel_optimizer == 1;
+        /* IC-GN: compute_lazy with use_exact_prefilter = false (seed_selector.cpp:163-173);
+         * forward Gauss-Newton: its own interpolator on the configured spline (forward_gauss_newton.cpp:98) */
+        orc_precompute(def, W, H, p->degree, p->border, fgn ? 1 : 0, 0, &pre);
+        double *gx = NULL, *gy = NULL;
+        if (!fgn) {
+            gx = (double *)malloc(sizeof(double) * (size_t)W * H);
+            gy = (double *)malloc(sizeof(double) * (size_t)W * H);
+            orc_reference_gradient_cache(ref, W, H, gx, gy); /* central_difference_gradient, icgn.cpp:42-53 */
+        }
+        for (int i = 0; i < k; ++i) {
+            if (!est[i].valid) continue;
+            const orc_initial ini = {est[i].u, est[i].v, est[i].du_dx, est[i].du_dy, est[i].dv_dx, est[i].dv_dy,
+                                     est[i].confidence};
+            orc_result res;
+            if (fgn)
+                orc_fgn_solve(&cfg, ref, W, H, &pre, NULL, xy[2 * i], xy[2 * i + 1], &ini, &res);
+            else
+                orc_icgn_solve(&cfg, ref, NULL, W, H, gx, gy, &pre, p->truncate_roi_subsets ? roi : NULL,
+                               xy[2 * i], xy[2 * i + 1], &ini, &res);
+            if (res.valid && res.status == ORC_SUCCESS) {
+                est[i].u = res.p[0];
+                est[i].v = res.p[1];
+                est[i].du_dx = res.p[2];
+                est[i].du_dy = res.p[3];
+                est[i].dv_dx = res.p[4];
+                est[i].dv_dy = res.p[5];
+                est[i].confidence = res.correlation;
+                est[i].zncc = zncc_from_znssd(res.correlation);
+                est[i].znssd = res.correlation;
+            }
+        }
+        free(gx);
+        free(gy);
+        orc_precomputed_free(&pre);
+    }
+    int best = -1;
+    for (int i = 0; i < k; ++i) {
+        orc_seed_eval *e = out + i;
+        memset(e, 0, sizeof(*e));
+        e->x = xy[2 * i];
+        e->y = xy[2 * i + 1];
+        e->displacement.status = ORC_INVALID_INPUT; /* Displacement2D defaults, result.hpp:32-51 */
+        const orc_estimate *q = est + i;
+        if (q->valid && isfinite(q->u) && isfinite(q->v) && isfinite(q->confidence)) {
+            e->displacement.x = e->x;
+            e->displacement.y = e->y;
+            e->displacement.p[0] = q->u;
+            e->displacement.p[1] = q->v;
+            e->displacement.p[2] = q->du_dx;
+            e->displacement.p[3] = q->du_dy;
+            e->displacement.p[4] = q->dv_dx;
+            e->displacement.p[5] = q->dv_dy;
+            e->displacement.correlation = q->confidence;
+            e->displacement.status = ORC_SUCCESS;
+            e->displacement.valid = 1;
+            e->quality = p->quality_metric == 0 ? q->zncc : q->znssd;
+            e->displacement_norm = hypot(q->u, q->v);
+            e->quality_passed = quality_passes(p, e->quality);
+            e->valid = 1;
+        }
+        if (e->valid && e->quality_passed && e->displacement_norm >= p->min_displacement_norm)
+            if (best < 0 || seed_is_better(e, out + best, p->quality_metric)) best = i;
+    }
+    *n_out = k;
+    free(est);
+    free(xy);
+    return best;
+}

@@ -144,6 +144,82 @@ int orc_propagate(const orc_config *cfg, int optimizer, int spacing, double max_
 /* POI grid of ReliabilityPropagation::propagate (reliability_propagation.cpp:154-176). */
 int orc_poi_grid(int width, int height, int spacing, double *xy_out /* may be NULL */);
 
+/* --- Seeding: integer-pixel ZNCC search (src/initialization/integer_search.cpp)
+ * and seed selection (src/subset/seed/seed_selector.cpp,
+ * src/initialization/subset_initializer.cpp).  Layouts are shared with the HIP
+ * library's dic_integer_search_params / dic_estimate / dic_seed_params /
+ * dic_seed_evaluation. --- */
+typedef struct orc_search_params {
+    int32_t subset_radius;             /* SeedIntegerSearchConfig (seed_config.hpp:33-40) */
+    int32_t search_radius;
+    int32_t pyramid_enabled;
+    int32_t pyramid_scale;
+    int32_t pyramid_refinement_radius;
+    int32_t around_prior;              /* 1: estimate[_with_mask]_around_displacement */
+    int32_t masked;                    /* 1: the _with_mask variants (roi required) */
+    int32_t reserved;
+} orc_search_params;
+
+/* InitialDisplacement with every field (initializer.hpp:33-44). */
+typedef struct orc_estimate {
+    double u, v, du_dx, du_dy, dv_dx, dv_dy, confidence;
+    int32_t valid, reserved;
+    double zncc, znssd;
+} orc_estimate;
+
+typedef struct orc_seed_params {
+    orc_search_params search;          /* around_prior / masked are set by the selector */
+    int32_t subpixel_enabled;          /* SeedSubpixelRefinementConfig (seed_config.hpp:42-50) */
+    int32_t subpixel_shape_order;      /* 1 or 2 */
+    int32_t subpixel_optimizer;        /* 0 IC-GN, 1 forward Gauss-Newton */
+    int32_t subpixel_objective;        /* 0 ZNSSD, 1 SSD */
+    int32_t subpixel_radius;
+    int32_t subpixel_max_iterations;
+    double subpixel_convergence_threshold;
+    int32_t seed_count;                /* SeedSelectionConfig (seed_config.hpp:58-69) */
+    int32_t quality_metric;            /* 0 ZNCC, 1 ZNSSD, 2 SSD */
+    double min_zncc, max_znssd, max_ssd, min_displacement_norm, min_texture_std;
+    int32_t kmeans_iterations;
+    int32_t kmeans_sample_limit;
+    int32_t truncate_roi_subsets;      /* SubsetConfig (subset_config.hpp:33-50) */
+    int32_t min_valid_samples;
+    double min_valid_sample_ratio;
+    int32_t degree;                    /* image_precompute of the seed interpolant */
+    int32_t border;
+} orc_seed_params;
+
+/* SeedEvaluation (seed_selector.hpp:20-27). */
+typedef struct orc_seed_eval {
+    double x, y;
+    orc_result displacement;
+    double quality, displacement_norm;
+    int32_t quality_passed, valid;
+} orc_seed_eval;
+
+/* IntegerSearchInitializer::estimate / estimate_with_mask /
+ * estimate[_with_mask]_around_displacement (integer_search.cpp:303-645) for n
+ * points.  prior_uv: 2n (around_prior only).  roi: W x H (masked only). */
+void orc_integer_search(const orc_search_params *p, const float *reference, const float *deformed, int width,
+                        int height, const uint8_t *roi, const double *xy, const double *prior_uv, int n,
+                        orc_estimate *out, int threads);
+/* ZNCC of one circular subset at integer shift (u, v) (circular_subset_zncc,
+ * :127-165, full circle at sample step 1); -inf where the reference does. */
+double orc_subset_zncc(const float *reference, const float *deformed, int width, int height, int radius, int cx,
+                       int cy, int u, int v);
+/* SeedSelector::local_texture_std (seed_selector.cpp:421-446). */
+double orc_texture_std(const float *reference, int width, int height, int x, int y, int radius);
+/* deterministic_sample index map (seed_selector.cpp:34-48): idx_out[limit'] for n points. */
+int orc_deterministic_sample(int n, int limit, int *idx_out);
+/* initialize_centers + run_kmeans (seed_selector.cpp:50-110) over integer points (xy pairs). */
+int orc_kmeans(const int *points, int n, int requested, int iterations, double *centers_out);
+/* SeedSelector::generate_candidates (seed_selector.cpp:208-316); returns the count (<= cap). */
+int orc_seed_candidates(const orc_seed_params *p, const float *reference, int width, int height,
+                        const uint8_t *roi, double *xy_out, int cap);
+/* SeedSelector::select_best_seed (seed_selector.cpp:146-206): evaluations of every
+ * candidate into out (<= cap), *n_out; returns the index of the best seed or -1. */
+int orc_select_seed(const orc_seed_params *p, const float *reference, const float *deformed, int width, int height,
+                    const uint8_t *roi, orc_seed_eval *out, int cap, int *n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,60 @@ class OrcPrecomputed(ctypes.Structure):
                 ("grad_x", ctypes.c_void_p), ("grad_y", ctypes.c_void_p)]
 
 
+ESTIMATE_DTYPE = np.dtype([
+    ("u", "<f8"), ("v", "<f8"), ("du_dx", "<f8"), ("du_dy", "<f8"), ("dv_dx", "<f8"), ("dv_dy", "<f8"),
+    ("confidence", "<f8"), ("valid", "<i4"), ("reserved", "<i4"), ("zncc", "<f8"), ("znssd", "<f8"),
+])
+SEED_EVAL_DTYPE = np.dtype([
+    ("x", "<f8"), ("y", "<f8"), ("displacement", RESULT_DTYPE), ("quality", "<f8"),
+    ("displacement_norm", "<f8"), ("quality_passed", "<i4"), ("valid", "<i4"),
+])
+
+
+class OrcSearchParams(ctypes.Structure):
+    _fields_ = [("subset_radius", ctypes.c_int32), ("search_radius", ctypes.c_int32),
+                ("pyramid_enabled", ctypes.c_int32), ("pyramid_scale", ctypes.c_int32),
+                ("pyramid_refinement_radius", ctypes.c_int32), ("around_prior", ctypes.c_int32),
+                ("masked", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class OrcSeedParams(ctypes.Structure):
+    _fields_ = [("search", OrcSearchParams), ("subpixel_enabled", ctypes.c_int32),
+                ("subpixel_shape_order", ctypes.c_int32), ("subpixel_optimizer", ctypes.c_int32),
+                ("subpixel_objective", ctypes.c_int32), ("subpixel_radius", ctypes.c_int32),
+                ("subpixel_max_iterations", ctypes.c_int32), ("subpixel_convergence_threshold", ctypes.c_double),
+                ("seed_count", ctypes.c_int32), ("quality_metric", ctypes.c_int32), ("min_zncc", ctypes.c_double),
+                ("max_znssd", ctypes.c_double), ("max_ssd", ctypes.c_double),
+                ("min_displacement_norm", ctypes.c_double), ("min_texture_std", ctypes.c_double),
+                ("kmeans_iterations", ctypes.c_int32), ("kmeans_sample_limit", ctypes.c_int32),
+                ("truncate_roi_subsets", ctypes.c_int32), ("min_valid_samples", ctypes.c_int32),
+                ("min_valid_sample_ratio", ctypes.c_double), ("degree", ctypes.c_int32), ("border", ctypes.c_int32)]
+
+
+# Defaults of SeedIntegerSearchConfig / SeedSubpixelRefinementConfig / SeedSelectionConfig
+# (seed_config.hpp:33-69) and the SubsetConfig fields the selector reads (subset_config.hpp:33-50).
+SEARCH_DEFAULTS = dict(subset_radius=10, search_radius=30, pyramid_enabled=1, pyramid_scale=4,
+                       pyramid_refinement_radius=4, around_prior=0, masked=0)
+SEED_DEFAULTS = dict(subpixel_enabled=1, subpixel_shape_order=1, subpixel_optimizer=0, subpixel_objective=0,
+                     subpixel_radius=15, subpixel_max_iterations=30, subpixel_convergence_threshold=1e-3,
+                     seed_count=256, quality_metric=1, min_zncc=0.7, max_znssd=0.2, max_ssd=0.05,
+                     min_displacement_norm=0.0, min_texture_std=0.0, kmeans_iterations=20,
+                     kmeans_sample_limit=20000, truncate_roi_subsets=0, min_valid_samples=12,
+                     min_valid_sample_ratio=0.5, degree=5, border=3)
+
+
+def search_params(**kw) -> OrcSearchParams:
+    d = dict(SEARCH_DEFAULTS)
+    d.update(kw)
+    return OrcSearchParams(**d)
+
+
+def seed_params(search: dict | None = None, **kw) -> OrcSeedParams:
+    d = dict(SEED_DEFAULTS)
+    d.update(kw)
+    return OrcSeedParams(search=search_params(**(search or {})), **d)
+
+
 _lib = None
 
 
@@ -81,6 +135,20 @@ def load():
     lib.orc_propagate.restype = ci
     lib.orc_poi_grid.argtypes = [ci, ci, ci, vp]
     lib.orc_poi_grid.restype = ci
+    lib.orc_integer_search.argtypes = [ctypes.POINTER(OrcSearchParams), vp, vp, ci, ci, vp, vp, vp, ci, vp, ci]
+    lib.orc_subset_zncc.argtypes = [vp, vp, ci, ci, ci, ci, ci, ci, ci]
+    lib.orc_subset_zncc.restype = cd
+    lib.orc_texture_std.argtypes = [vp, ci, ci, ci, ci, ci]
+    lib.orc_texture_std.restype = cd
+    lib.orc_deterministic_sample.argtypes = [ci, ci, vp]
+    lib.orc_deterministic_sample.restype = ci
+    lib.orc_kmeans.argtypes = [vp, ci, ci, ci, vp]
+    lib.orc_kmeans.restype = ci
+    lib.orc_seed_candidates.argtypes = [ctypes.POINTER(OrcSeedParams), vp, ci, ci, vp, vp, ci]
+    lib.orc_seed_candidates.restype = ci
+    lib.orc_select_seed.argtypes = [ctypes.POINTER(OrcSeedParams), vp, vp, ci, ci, vp, vp, ci,
+                                    ctypes.POINTER(ci)]
+    lib.orc_select_seed.restype = ci
     _lib = lib
     return lib
 
@@ -287,3 +355,72 @@ def propagate(reference: np.ndarray, pre: Precomputed, grad_x, grad_y, roi: np.n
     n = load().orc_propagate(ctypes.byref(cfg), int(optimizer), int(spacing), float(max_znssd), int(truncate_roi),
                              _p(ref), W, H, _p(gx), _p(gy), ctypes.byref(pre.s), _p(r), _p(sd), sd.shape[0], _p(out))
     return out.reshape(gh, gw), n
+
+
+def integer_search(reference: np.ndarray, deformed: np.ndarray, xy: np.ndarray, params: OrcSearchParams | None = None,
+                   roi: np.ndarray | None = None, prior_uv: np.ndarray | None = None, threads: int = 0) -> np.ndarray:
+    """IntegerSearchInitializer over points (integer_search.cpp:303-645); ESTIMATE_DTYPE records."""
+    params = params or search_params()
+    ref = np.ascontiguousarray(reference, np.float32)
+    dfm = np.ascontiguousarray(deformed, np.float32)
+    H, W = ref.shape
+    pts = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    pr = np.ascontiguousarray(prior_uv, np.float64).reshape(-1, 2) if prior_uv is not None else None
+    r = np.ascontiguousarray(roi, np.uint8) if roi is not None else None
+    out = np.zeros(pts.shape[0], ESTIMATE_DTYPE)
+    load().orc_integer_search(ctypes.byref(params), _p(ref), _p(dfm), W, H, _p(r), _p(pts), _p(pr), pts.shape[0],
+                              _p(out), int(threads))
+    return out
+
+
+def subset_zncc(reference: np.ndarray, deformed: np.ndarray, radius: int, cx: int, cy: int, u: int, v: int) -> float:
+    ref = np.ascontiguousarray(reference, np.float32)
+    dfm = np.ascontiguousarray(deformed, np.float32)
+    H, W = ref.shape
+    return load().orc_subset_zncc(_p(ref), _p(dfm), W, H, radius, cx, cy, u, v)
+
+
+def texture_std(reference: np.ndarray, x: int, y: int, radius: int) -> float:
+    ref = np.ascontiguousarray(reference, np.float32)
+    H, W = ref.shape
+    return load().orc_texture_std(_p(ref), W, H, x, y, radius)
+
+
+def deterministic_sample(n: int, limit: int) -> np.ndarray:
+    idx = np.zeros(max(1, min(n, max(limit, 1)) if limit > 0 else n), np.int32)
+    k = load().orc_deterministic_sample(n, limit, _p(idx))
+    return idx[:k]
+
+
+def kmeans(points: np.ndarray, requested: int, iterations: int) -> np.ndarray:
+    pts = np.ascontiguousarray(points, np.int32).reshape(-1, 2)
+    c = np.zeros((max(1, min(requested, pts.shape[0])), 2))
+    k = load().orc_kmeans(_p(pts), pts.shape[0], requested, iterations, _p(c))
+    return c[:k]
+
+
+def seed_candidates(reference: np.ndarray, roi: np.ndarray, params: OrcSeedParams | None = None) -> np.ndarray:
+    params = params or seed_params()
+    ref = np.ascontiguousarray(reference, np.float32)
+    H, W = ref.shape
+    r = np.ascontiguousarray(roi, np.uint8)
+    cap = max(1, params.seed_count)
+    out = np.zeros((cap, 2))
+    k = load().orc_seed_candidates(ctypes.byref(params), _p(ref), W, H, _p(r), _p(out), cap)
+    return out[:k]
+
+
+def select_seed(reference: np.ndarray, deformed: np.ndarray, roi: np.ndarray, params: OrcSeedParams | None = None):
+    """SeedSelector::select_best_seed (seed_selector.cpp:146-206).  Returns
+    (evaluations SEED_EVAL_DTYPE, index of the best seed or -1)."""
+    params = params or seed_params()
+    ref = np.ascontiguousarray(reference, np.float32)
+    dfm = np.ascontiguousarray(deformed, np.float32)
+    H, W = ref.shape
+    r = np.ascontiguousarray(roi, np.uint8)
+    cap = max(1, params.seed_count)
+    out = np.zeros(cap, SEED_EVAL_DTYPE)
+    n = ctypes.c_int(0)
+    best = load().orc_select_seed(ctypes.byref(params), _p(ref), _p(dfm), W, H, _p(r), _p(out), cap,
+                                  ctypes.byref(n))
+    return out[:n.value], best

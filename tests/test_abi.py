@@ -27,6 +27,10 @@ def test_struct_layouts(pkg):
     assert pkg.INITIAL_DTYPE.itemsize == 56
     assert ctypes.sizeof(pkg.IcgnParams) == 48
     assert pkg.RESULT_DTYPE.fields["status"][1] == 15 * 8
+    assert ctypes.sizeof(pkg._lib.IntegerSearchParams) == 32
+    assert ctypes.sizeof(pkg._lib.SeedParams) == 144
+    assert pkg._lib.ESTIMATE_DTYPE.itemsize == 80
+    assert pkg._lib.SEED_EVAL_DTYPE.itemsize == 176
 
 
 def test_version_string(pkg):
@@ -55,3 +59,39 @@ def test_cpp_host_layer_builds():
     """include/dic_hip.hpp compiles with plain g++ into the C++ example."""
     exe = os.path.join(ROOT, "traditional-dic_amd", "solve_pair")
     assert os.path.exists(exe) and os.access(exe, os.X_OK)
+
+
+def test_seed_config_from_dict_mirrors_binding(pkg):
+    """bind_subset.cpp:201-228: initialization and seed_selection keys; the
+    seed objective follows correlation.criterion; keys the binding does not
+    read keep their defaults."""
+    import pytest
+    cfg = pkg.config_from_dict({
+        "subset": {"radius": 41, "truncate_roi_subsets": True},
+        "correlation": {"criterion": "ssd"},
+        "initialization": {
+            "method": "integer_search",
+            "integer_search": {"subset_radius": 10, "search_radius": 30, "sift_enabled": True,
+                               "pyramid_enabled": False, "pyramid_scale": 4, "pyramid_refinement_radius": 4},
+            "subpixel_refinement": {"enabled": True, "shape_function": "second_order", "optimizer": "fgn",
+                                    "subset_radius": 41, "max_iterations": 50, "convergence_threshold": 1e-6}},
+        "seed_selection": {"seed_count": 16, "max_znssd": 2.0, "min_zncc": 0.9, "quality_metric": "zncc",
+                           "min_texture_std": 0.02, "kmeans_iterations": 20, "kmeans_sample_limit": 20000},
+    })
+    si, ss = cfg.seed_initialization, cfg.seed_selection
+    assert (si.integer_search.subset_radius, si.integer_search.search_radius) == (10, 30)
+    assert si.integer_search.sift_enabled and not si.integer_search.pyramid_enabled
+    assert (si.subpixel.subset_radius, si.subpixel.max_iterations, si.subpixel.convergence_threshold) == (41, 50, 1e-6)
+    assert (si.subpixel.shape_function, si.subpixel.optimizer) == ("first_order", "icgn")  # not read
+    assert si.subpixel.objective == "ssd"
+    assert (ss.seed_count, ss.max_znssd, ss.min_texture_std) == (16, 2.0, 0.02)
+    assert (ss.min_zncc, ss.quality_metric) == (0.7, "znssd")  # not read
+    assert cfg.truncate_roi_subsets is False  # not read
+    with pytest.raises(RuntimeError, match="integer_search"):
+        pkg.config_from_dict({"initialization": {"method": "sift"}})
+    p = pkg.seed_params(cfg)
+    assert (p.search.subset_radius, p.search.search_radius, p.search.pyramid_enabled) == (10, 30, 0)
+    assert (p.subpixel_radius, p.subpixel_objective, p.seed_count) == (41, 1, 16)
+    # padding.cpp:28-36: search + max radius + border
+    cfg.truncate_roi_subsets = True
+    assert pkg.recommended_subset_padding(cfg) == 30 + 41 + 3

@@ -1,0 +1,209 @@
+"""Seeding on the GPU (dic_integer_search, dic_seed_candidates,
+dic_select_seed, dic_image_mirror_pad) against the oracle.
+
+The integer search is index work in FP64 without fused multiply-add on both
+sides: shifts, validity and every ZNCC value must be bit-identical.  The
+candidates are integer points: identical.  The seed evaluations add the FP64
+IC-GN refinement: identical candidates, validity and statuses, u, v within
+1e-5 px.  End to end, subset() (seed selection + propagation) on the
+stereo-plate case must reproduce the reference's golden valid set."""
+import numpy as np
+import pytest
+
+from conftest import read_gray
+from test_gpu_propagation import GOLDEN_SEED_ARTIFACTS
+
+pytestmark = pytest.mark.gpu
+
+
+def _report(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def pair(synth):
+    W, H = 256, 208
+    field = synth.affine(5.3, -3.6, 0.01, -0.004, 0.003, 0.008, W / 2, H / 2)
+    ref = synth.quantize_unit(synth.render(W, H, W * H // 40, 201))
+    dfm = synth.quantize_unit(synth.render(W, H, W * H // 40, 201, field))
+    yy, xx = np.mgrid[0:H, 0:W]
+    disk = (((xx - 120) ** 2 + (yy - 100) ** 2) < 85 ** 2).astype(np.uint8)
+    return ref, dfm, disk, field
+
+
+def _points():
+    xs = np.concatenate([np.linspace(4, 250, 9), [30.5, 31.5, 128.49, 128.5]])
+    ys = np.concatenate([np.linspace(3, 204, 5), [40.5, 41.5]])
+    X, Y = np.meshgrid(xs, ys)
+    return np.stack([X.ravel(), Y.ravel()], 1)
+
+
+SEARCH_CASES = {
+    "multigrid": dict(subset_radius=10, search_radius=0, pyramid_enabled=1),
+    "multigrid+local": dict(subset_radius=10, search_radius=12, pyramid_enabled=1),
+    "dense": dict(subset_radius=6, search_radius=8, pyramid_enabled=0),
+    "scale3": dict(subset_radius=9, search_radius=6, pyramid_enabled=1, pyramid_scale=3, pyramid_refinement_radius=20),
+    "masked": dict(subset_radius=8, search_radius=5, pyramid_enabled=1, masked=1),
+    "masked-dense": dict(subset_radius=5, search_radius=3, pyramid_enabled=0, masked=1),
+    "prior": dict(subset_radius=8, search_radius=4, mode=1),
+    "prior-masked": dict(subset_radius=8, search_radius=4, mode=1, masked=1),
+}
+
+
+@pytest.mark.parametrize("case", list(SEARCH_CASES))
+def test_integer_search_bit_identical_to_oracle(pkg, ctx, orc, pair, case):
+    ref, dfm, disk, field = pair
+    kw = dict(pyramid_scale=4, pyramid_refinement_radius=4, mode=0, masked=0)
+    kw.update(SEARCH_CASES[case])
+    gp = pkg._lib.IntegerSearchParams(**kw)
+    op = orc.search_params(**{("around_prior" if k == "mode" else k): v for k, v in kw.items()})
+    pts = _points()
+    prior = None
+    if kw["mode"] == 1:
+        u, v, _ = field.displacement(pts[:, 0], pts[:, 1])
+        prior = np.stack([u + 0.3, v - 1.4], 1)
+        prior[3] = [np.nan, 0.0]
+    roi = disk if kw["masked"] else None
+    _report(f"integer search {case}: {len(pts)} points, {kw}")
+    g_ref, g_dfm = ctx.upload_image(ref), ctx.upload_image(dfm)
+    got = ctx.integer_search(gp, g_ref, g_dfm, pts, prior, roi)
+    want = orc.integer_search(ref, dfm, pts, op, roi=roi, prior_uv=prior)
+    g_ref.free()
+    g_dfm.free()
+    assert np.array_equal(got["valid"], want["valid"])
+    ok = want["valid"] == 1
+    _report(f"  {ok.sum()} valid; zncc max |diff| "
+            f"{np.abs(got['zncc'][ok] - want['zncc'][ok]).max(initial=0):.1e}")
+    assert ok.sum() >= 10
+    assert got.tobytes() == want.tobytes()
+    # the true integer shift is found, except by the coarse grid alone, which
+    # aliases on fine speckle (the reference's multigrid does the same)
+    u, v, _ = field.displacement(pts[ok, 0], pts[ok, 1])
+    hit = np.mean((np.abs(got["u"][ok] - u) <= 1) & (np.abs(got["v"][ok] - v) <= 1))
+    _report(f"  true shift found at {hit:.0%} of the valid points")
+    if kw["search_radius"] > 0 or not kw["pyramid_enabled"]:
+        assert hit > 0.9
+
+
+CANDIDATE_CASES = {
+    "kmeans16": dict(seed_count=16),
+    "truncate-disk": dict(seed_count=24, truncate_roi_subsets=1, min_texture_std=0.02),
+    "fallback": dict(seed_count=12, min_texture_std=0.5),  # no centre passes: nearest-50 fallback everywhere
+    "single": dict(seed_count=1),
+    "sampled": dict(seed_count=20, kmeans_sample_limit=500, kmeans_iterations=7),
+    "limit1": dict(seed_count=3, kmeans_sample_limit=1),
+}
+
+
+def _seed_params(pkg, orc, search=None, **kw):
+    o = orc.seed_params(search=search or dict(subset_radius=8, search_radius=10), **kw)
+    g = pkg._lib.SeedParams()
+    for name, _ in orc.OrcSeedParams._fields_:
+        if name == "search":
+            for n2, _ in orc.OrcSearchParams._fields_:
+                setattr(g.search, "mode" if n2 == "around_prior" else n2, getattr(o.search, n2))
+        else:
+            setattr(g, name, getattr(o, name))
+    return g, o
+
+
+@pytest.mark.parametrize("case", list(CANDIDATE_CASES))
+def test_seed_candidates_identical_to_oracle(pkg, ctx, orc, pair, case):
+    ref, dfm, disk, _ = pair
+    kw = dict(subpixel_radius=12)
+    kw.update(CANDIDATE_CASES[case])
+    roi = disk if kw.get("truncate_roi_subsets") else np.ones(ref.shape, np.uint8)
+    if case == "fallback":
+        roi = disk
+    g, o = _seed_params(pkg, orc, **kw)
+    _report(f"candidates {case}: {kw}")
+    g_ref = ctx.upload_image(ref)
+    got = ctx.seed_candidates(g, g_ref, roi)
+    g_ref.free()
+    want = orc.seed_candidates(ref, roi, o)
+    _report(f"  {len(want)} candidates")
+    assert len(want) >= 1
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("truncate", [0, 1])
+def test_select_seed_matches_oracle(pkg, ctx, orc, pair, truncate):
+    ref, dfm, disk, field = pair
+    g, o = _seed_params(pkg, orc, search=dict(subset_radius=10, search_radius=12), subpixel_radius=15,
+                        seed_count=10, max_znssd=0.3, min_texture_std=0.02, subpixel_max_iterations=50,
+                        subpixel_convergence_threshold=1e-6, truncate_roi_subsets=truncate)
+    roi = disk
+    _report(f"select seed truncate={truncate}")
+    g_ref, g_dfm = ctx.upload_image(ref), ctx.upload_image(dfm)
+    got, gbest = ctx.select_seed(g, g_ref, g_dfm, roi)
+    g_ref.free()
+    g_dfm.free()
+    want, obest = orc.select_seed(ref, dfm, roi, o)
+    assert len(got) == len(want) >= 5
+    np.testing.assert_array_equal(got[["x", "y"]], want[["x", "y"]])
+    assert np.array_equal(got["valid"], want["valid"])
+    assert np.array_equal(got["displacement"]["status"], want["displacement"]["status"])
+    ok = want["valid"] == 1
+    du = np.abs(got["displacement"]["p"][ok, :6] - want["displacement"]["p"][ok, :6]).max(initial=0)
+    u, v, _ = field.displacement(want["x"][ok], want["y"][ok])
+    err = np.hypot(got["displacement"]["p"][ok, 0] - u, got["displacement"]["p"][ok, 1] - v).max(initial=0)
+    _report(f"  {ok.sum()} valid, best {gbest} (oracle {obest}), |gpu - oracle| {du:.1e}, |gpu - truth| {err:.1e}")
+    assert du <= 1e-5
+    assert np.abs(got["quality"][ok] - want["quality"][ok]).max(initial=0) <= 1e-6
+    assert gbest == obest >= 0
+    assert err < 0.05
+
+
+def test_mirror_pad_matches_numpy_symmetric(pkg, ctx):
+    rng = np.random.default_rng(4)
+    img = rng.random((37, 53)).astype(np.float32)
+    g = ctx.upload_image(img)
+    for pad in (0, 5, 60):  # 60 > both sides: repeated reflection
+        p = ctx.mirror_pad(g, pad)
+        np.testing.assert_array_equal(ctx.download_image(p), np.pad(img, pad, mode="symmetric"))
+        p.free()
+    g.free()
+
+
+def test_subset_pipeline_reproduces_golden_field(pkg, stereo):
+    """subset() with the golden configuration (config/subset_2d.yaml) except the
+    SIFT prior: seed selection by integer search, then propagation."""
+    rows = stereo["field_left_temporal"]
+    ref, dfm = read_gray(stereo["img_00_L"]), read_gray(stereo["img_04_L"])
+    cfg = {
+        "subset": {"radius": 41}, "shape_function": {"order": 1},
+        "optimization": {"method": "icgn", "max_iterations": 50, "convergence_threshold": 1e-6},
+        "correlation": {"criterion": "znssd"}, "interpolation": {"degree": 5},
+        "initialization": {"method": "integer_search",
+                           "integer_search": {"subset_radius": 10, "search_radius": 30, "sift_enabled": False,
+                                              "pyramid_enabled": False, "pyramid_scale": 4,
+                                              "pyramid_refinement_radius": 4},
+                           "subpixel_refinement": {"enabled": True, "subset_radius": 41, "max_iterations": 50,
+                                                   "convergence_threshold": 1e-6}},
+        "seed_selection": {"seed_count": 16, "max_znssd": 2.0, "min_displacement_norm": 0.0,
+                           "min_texture_std": 0.02, "kmeans_iterations": 20, "kmeans_sample_limit": 20000},
+        "reliability_propagation": {"spacing": 3, "max_znssd": 2.0},
+    }
+    dic = pkg.SubsetDIC(pkg.config_from_dict(cfg))
+    pts = dic.compute(ref, dfm, stereo["roi"].astype(bool))
+    seed = dic.last_seed
+    assert seed.found
+    b = seed.best_seed
+    _report(f"pipeline: best seed at ({b['x']:.0f}, {b['y']:.0f}) u {b['displacement']['p'][0]:.4f} "
+            f"v {b['displacement']['p'][1]:.4f} znssd {b['quality']:.4f}; {dic.points_computed} points")
+    assert pts.shape[0] == 270 * 360 and dic.points_computed == rows.shape[0]
+    valid = pts[pts["valid"] == 1]
+    got = {(int(r["x"]), int(r["y"])): r for r in valid}
+    want = {(int(r[0]), int(r[1])): r for r in rows}
+    assert set(got) == set(want)
+    du = np.array([max(abs(got[k]["p"][0] - want[k][2]), abs(got[k]["p"][1] - want[k][3])) for k in want])
+    keys = list(want)
+    far = {keys[i] for i in np.nonzero(du > 1e-4)[0]}
+    # the seed's own point carries the seed path's solution (non-prefiltered spline)
+    step = 4
+    seed_pt = (int(np.floor(b["x"] / step + 0.5)) * step, int(np.floor(b["y"] / step + 0.5)) * step)
+    _report(f"  outside 1e-5: {(du > 1e-5).sum()}, beyond 1e-4: {sorted(far)}")
+    assert (du > 1e-5).sum() <= 0.03 * du.size
+    assert far <= GOLDEN_SEED_ARTIFACTS["left_temporal"] | {seed_pt}
+    fields = pkg.results_to_dict(pts)
+    assert fields["valid"].sum() == rows.shape[0]

@@ -43,7 +43,8 @@ EXPORTED_SYMBOLS = (
     "dic_batch_autotune", "dic_batch_destroy",
     "dic_host_alloc", "dic_host_free", "dic_server_create", "dic_server_submit", "dic_server_wait",
     "dic_server_frames_done", "dic_server_stats_get", "dic_server_now", "dic_server_error", "dic_server_destroy",
-    "dic_propagate",
+    "dic_propagate", "dic_integer_search", "dic_seed_candidates", "dic_select_seed", "dic_image_mirror_pad",
+    "dic_image_download",
 )
 
 # Displacement2D + iterations (dic_result); InitialDisplacement (dic_initial)
@@ -158,6 +159,40 @@ class PropagationParams(ctypes.Structure):
 
 SEED_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("displacement", INITIAL_DTYPE)])
 
+SEARCH_GLOBAL, SEARCH_AROUND_PRIOR = 0, 1
+QUALITY_ZNCC, QUALITY_ZNSSD, QUALITY_SSD = 0, 1, 2
+
+# InitialDisplacement with every field (dic_estimate)
+ESTIMATE_DTYPE = np.dtype([
+    ("u", "<f8"), ("v", "<f8"), ("du_dx", "<f8"), ("du_dy", "<f8"), ("dv_dx", "<f8"), ("dv_dy", "<f8"),
+    ("confidence", "<f8"), ("valid", "<i4"), ("reserved", "<i4"), ("zncc", "<f8"), ("znssd", "<f8"),
+])
+# SeedEvaluation (dic_seed_evaluation)
+SEED_EVAL_DTYPE = np.dtype([
+    ("x", "<f8"), ("y", "<f8"), ("displacement", RESULT_DTYPE), ("quality", "<f8"),
+    ("displacement_norm", "<f8"), ("quality_passed", "<i4"), ("valid", "<i4"),
+])
+
+
+class IntegerSearchParams(ctypes.Structure):
+    _fields_ = [("subset_radius", ctypes.c_int32), ("search_radius", ctypes.c_int32),
+                ("pyramid_enabled", ctypes.c_int32), ("pyramid_scale", ctypes.c_int32),
+                ("pyramid_refinement_radius", ctypes.c_int32), ("mode", ctypes.c_int32),
+                ("masked", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class SeedParams(ctypes.Structure):
+    _fields_ = [("search", IntegerSearchParams), ("subpixel_enabled", ctypes.c_int32),
+                ("subpixel_shape_order", ctypes.c_int32), ("subpixel_optimizer", ctypes.c_int32),
+                ("subpixel_objective", ctypes.c_int32), ("subpixel_radius", ctypes.c_int32),
+                ("subpixel_max_iterations", ctypes.c_int32), ("subpixel_convergence_threshold", ctypes.c_double),
+                ("seed_count", ctypes.c_int32), ("quality_metric", ctypes.c_int32), ("min_zncc", ctypes.c_double),
+                ("max_znssd", ctypes.c_double), ("max_ssd", ctypes.c_double),
+                ("min_displacement_norm", ctypes.c_double), ("min_texture_std", ctypes.c_double),
+                ("kmeans_iterations", ctypes.c_int32), ("kmeans_sample_limit", ctypes.c_int32),
+                ("truncate_roi_subsets", ctypes.c_int32), ("min_valid_samples", ctypes.c_int32),
+                ("min_valid_sample_ratio", ctypes.c_double), ("degree", ctypes.c_int32), ("border", ctypes.c_int32)]
+
 
 class DicError(RuntimeError):
     def __init__(self, code: int, message: str):
@@ -182,6 +217,7 @@ _SIGNATURES = {
     "dic_image_upload": ([_vp, _vp, _i32, _i32, ctypes.POINTER(_vp)], _i32),
     "dic_image_wrap_device": ([_vp, _vp, _i32, _i32, ctypes.POINTER(_vp)], _i32),
     "dic_image_free": ([_vp], None),
+    "dic_image_download": ([_vp, _vp, _vp], _i32),
     "dic_ref_gradient": ([_vp, _vp, _i32, _i32, _i32, _i32, ctypes.POINTER(_vp)], _i32),
     "dic_grad_download": ([_vp, _vp, _vp, _vp], _i32),
     "dic_grad_free": ([_vp], None),
@@ -225,6 +261,12 @@ _SIGNATURES = {
     "dic_propagate": ([_vp, ctypes.POINTER(IcgnParams), ctypes.POINTER(PropagationParams), _vp, _vp, _vp, _vp, _vp,
                        _i32, _vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                        ctypes.POINTER(ctypes.c_int)], _i32),
+    "dic_integer_search": ([_vp, ctypes.POINTER(IntegerSearchParams), _vp, _vp, _vp, _vp, _vp, _i32, _vp], _i32),
+    "dic_seed_candidates": ([_vp, ctypes.POINTER(SeedParams), _vp, _vp, _vp, _i32, ctypes.POINTER(ctypes.c_int)],
+                            _i32),
+    "dic_select_seed": ([_vp, ctypes.POINTER(SeedParams), _vp, _vp, _vp, _vp, _i32, ctypes.POINTER(ctypes.c_int),
+                         ctypes.POINTER(ctypes.c_int)], _i32),
+    "dic_image_mirror_pad": ([_vp, _vp, _i32, ctypes.POINTER(_vp)], _i32),
 }
 
 _lib = None
@@ -409,6 +451,46 @@ class Context:
                                           _ptr(sd), sd.shape[0], _ptr(out), ctypes.byref(a), ctypes.byref(b),
                                           ctypes.byref(n)))
         return out.reshape(gh, gw), n.value
+
+    def integer_search(self, params: IntegerSearchParams, ref: "DeviceImage", deformed: "DeviceImage",
+                       points_xy: np.ndarray, prior_uv: np.ndarray | None = None,
+                       roi: np.ndarray | None = None) -> np.ndarray:
+        """dic_integer_search over host points; ESTIMATE_DTYPE records."""
+        xy = np.ascontiguousarray(points_xy, np.float64).reshape(-1, 2)
+        pr = np.ascontiguousarray(prior_uv, np.float64).reshape(-1, 2) if prior_uv is not None else None
+        r = np.ascontiguousarray(roi, np.uint8) if roi is not None else None
+        out = np.zeros(xy.shape[0], ESTIMATE_DTYPE)
+        self.check(self.lib.dic_integer_search(self.handle, ctypes.byref(params), ref.handle, deformed.handle,
+                                               _ptr(r) if r is not None else None, _ptr(xy),
+                                               _ptr(pr) if pr is not None else None, xy.shape[0], _ptr(out)))
+        return out
+
+    def seed_candidates(self, params: SeedParams, ref: "DeviceImage", roi: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(roi, np.uint8)
+        out = np.zeros((max(1, params.seed_count), 2))
+        n = ctypes.c_int(0)
+        self.check(self.lib.dic_seed_candidates(self.handle, ctypes.byref(params), ref.handle, _ptr(r), _ptr(out),
+                                                out.shape[0], ctypes.byref(n)))
+        return out[:n.value]
+
+    def select_seed(self, params: SeedParams, ref: "DeviceImage", deformed: "DeviceImage", roi: np.ndarray):
+        """dic_select_seed: (SEED_EVAL_DTYPE evaluations, index of the best seed or -1)."""
+        r = np.ascontiguousarray(roi, np.uint8)
+        out = np.zeros(max(1, params.seed_count), SEED_EVAL_DTYPE)
+        n, best = ctypes.c_int(0), ctypes.c_int(-1)
+        self.check(self.lib.dic_select_seed(self.handle, ctypes.byref(params), ref.handle, deformed.handle, _ptr(r),
+                                            _ptr(out), out.shape[0], ctypes.byref(n), ctypes.byref(best)))
+        return out[:n.value], best.value
+
+    def download_image(self, img: "DeviceImage") -> np.ndarray:
+        out = np.empty((img.height, img.width), np.float32)
+        self.check(self.lib.dic_image_download(self.handle, img.handle, _ptr(out)))
+        return out
+
+    def mirror_pad(self, img: "DeviceImage", pad: int) -> "DeviceImage":
+        h = _vp()
+        self.check(self.lib.dic_image_mirror_pad(self.handle, img.handle, int(pad), ctypes.byref(h)))
+        return DeviceImage(self, h, img.width + 2 * max(0, pad), img.height + 2 * max(0, pad))
 
     def solve_device(self, params: IcgnParams, ref: "DeviceImage", grad: "Gradient", coeffs: "Coefficients",
                      xy_ptr: int, init_ptr: int, n: int, out_ptr: int, roi_ptr: int | None = None,

@@ -301,6 +301,17 @@ int dic_image_wrap_device(dic_ctx *ctx, const float *device_ptr, int width, int 
     return DIC_OK;
 }
 
+int dic_image_download(dic_ctx *ctx, const dic_image *img, float *host)
+{
+    if (!ctx) return DIC_ERR_INVALID_ARGUMENT;
+    if (!img || !img->data || !host) return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "image and host buffer required");
+    const size_t bytes = sizeof(float) * (size_t)img->width * img->height;
+    int rc = check_hip(ctx, hipMemcpyAsync(host, img->data, bytes, hipMemcpyDeviceToHost, ctx->stream),
+                       "image download");
+    if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "image download sync");
+    return rc;
+}
+
 void dic_image_free(dic_image *img)
 {
     if (!img) return;

@@ -31,18 +31,67 @@ class BSplinePrecomputeConfig:
 
 
 @dataclass
+class SeedIntegerSearchConfig:
+    """include/dic/initialization/seed_config.hpp:33-40."""
+    subset_radius: int = 10
+    search_radius: int = 30
+    sift_enabled: bool = False
+    pyramid_enabled: bool = True
+    pyramid_scale: int = 4
+    pyramid_refinement_radius: int = 4
+
+
+@dataclass
+class SeedSubpixelRefinementConfig:
+    """seed_config.hpp:42-50."""
+    enabled: bool = True
+    shape_function: str = "first_order"   # first_order | second_order
+    optimizer: str = "icgn"               # icgn | forward_gauss_newton
+    objective: str = "znssd"              # znssd | ssd
+    subset_radius: int = 15
+    max_iterations: int = 30
+    convergence_threshold: float = 1e-3
+
+
+@dataclass
+class SeedInitializationConfig:
+    """seed_config.hpp:52-56 (the integer-search method; SIFT is not part of this library)."""
+    method: str = "integer_search"
+    integer_search: SeedIntegerSearchConfig = field(default_factory=SeedIntegerSearchConfig)
+    subpixel: SeedSubpixelRefinementConfig = field(default_factory=SeedSubpixelRefinementConfig)
+
+
+@dataclass
+class SeedSelectionConfig:
+    """seed_config.hpp:58-69."""
+    seed_count: int = 256
+    threads: int = 1
+    quality_metric: str = "znssd"         # zncc | znssd | ssd
+    min_zncc: float = 0.7
+    max_znssd: float = 0.2
+    max_ssd: float = 0.05
+    min_displacement_norm: float = 0.0
+    min_texture_std: float = 0.0
+    kmeans_iterations: int = 20
+    kmeans_sample_limit: int = 20000
+
+
+@dataclass
 class SubsetConfig:
-    """Fields of dic::SubsetConfig that reach the solver (defaults as in subset_config.hpp:33-50)."""
+    """Fields of dic::SubsetConfig (defaults as in subset_config.hpp:33-50)."""
     subset_radius: int = 15
     max_iterations: int = 30
     convergence_threshold: float = 1e-3
     use_second_order: bool = False
+    search_radius: int = 20
     propagation_spacing: int = 5
     propagation_max_znssd: float = 0.5
     truncate_roi_subsets: bool = False
     min_valid_sample_ratio: float = 0.5
     min_valid_samples: int = 12
     image_precompute: BSplinePrecomputeConfig = field(default_factory=BSplinePrecomputeConfig)
+    seed_initialization: SeedInitializationConfig = field(default_factory=SeedInitializationConfig)
+    seed_selection: SeedSelectionConfig = field(default_factory=SeedSelectionConfig)
     shape_function: str = "first_order"   # first_order | second_order
     optimizer: str = "icgn"               # icgn | forward_gauss_newton
     objective: str = "znssd"              # znssd | ssd
@@ -53,9 +102,13 @@ class SubsetConfig:
 
 
 def config_from_dict(d: dict | None) -> SubsetConfig:
-    """Restates config_from_dict (bind_subset.cpp:158-248) for the fields on this path,
-    with the same validation messages.  Interpolation degree maps only 1 and 3
-    (any other value keeps the quintic default, :195-200)."""
+    """Restates config_from_dict (bind_subset.cpp:158-248) with the same
+    validation messages and the same reach: interpolation degree maps only 1
+    and 3 (any other value keeps the quintic default, :195-200); of the seed
+    sub-pixel refinement only enabled / subset_radius / max_iterations /
+    convergence_threshold are read (its shape function and optimizer keep
+    their defaults, its objective follows correlation.criterion, :190, :203-221);
+    subset.truncate_roi_subsets is not read."""
     cfg = SubsetConfig()
     if d is None:
         return cfg
@@ -82,11 +135,46 @@ def config_from_dict(d: dict | None) -> SubsetConfig:
         if crit not in ("ssd", "znssd"):
             raise RuntimeError("subset correlation.criterion must be 'ssd' or 'znssd'.")
         cfg.objective = crit
+        cfg.seed_initialization.subpixel.objective = crit
     it = d.get("interpolation")
     if isinstance(it, dict) and "degree" in it:
         deg = int(it["degree"])
         if deg in (1, 3):
             cfg.image_precompute.degree = deg
+    ini = d.get("initialization")
+    if isinstance(ini, dict):
+        if "method" in ini:
+            if str(ini["method"]) != "integer_search":
+                raise RuntimeError("subset initialization.method must be integer_search")
+            cfg.seed_initialization.method = "integer_search"
+        isr = ini.get("integer_search")
+        if isinstance(isr, dict):
+            c = cfg.seed_initialization.integer_search
+            c.subset_radius = int(isr.get("subset_radius", c.subset_radius))
+            c.search_radius = int(isr.get("search_radius", c.search_radius))
+            if "sift_enabled" in isr:
+                c.sift_enabled = bool(isr["sift_enabled"])
+            if "pyramid_enabled" in isr:
+                c.pyramid_enabled = bool(isr["pyramid_enabled"])
+            c.pyramid_scale = int(isr.get("pyramid_scale", c.pyramid_scale))
+            c.pyramid_refinement_radius = int(isr.get("pyramid_refinement_radius", c.pyramid_refinement_radius))
+        spr = ini.get("subpixel_refinement")
+        if isinstance(spr, dict):
+            c = cfg.seed_initialization.subpixel
+            if "enabled" in spr:
+                c.enabled = bool(spr["enabled"])
+            c.subset_radius = int(spr.get("subset_radius", c.subset_radius))
+            c.max_iterations = int(spr.get("max_iterations", c.max_iterations))
+            c.convergence_threshold = float(spr.get("convergence_threshold", c.convergence_threshold))
+    ss = d.get("seed_selection")
+    if isinstance(ss, dict):
+        c = cfg.seed_selection
+        c.seed_count = int(ss.get("seed_count", c.seed_count))
+        c.max_znssd = float(ss.get("max_znssd", c.max_znssd))
+        c.min_displacement_norm = float(ss.get("min_displacement_norm", c.min_displacement_norm))
+        c.min_texture_std = float(ss.get("min_texture_std", c.min_texture_std))
+        c.kmeans_iterations = int(ss.get("kmeans_iterations", c.kmeans_iterations))
+        c.kmeans_sample_limit = int(ss.get("kmeans_sample_limit", c.kmeans_sample_limit))
     rp = d.get("reliability_propagation")
     if isinstance(rp, dict):
         cfg.propagation_spacing = int(rp.get("spacing", cfg.propagation_spacing))
@@ -137,21 +225,26 @@ class PreparedPair:
     images, the reference gradient planes and the deformed image's B-spline
     coefficients (the reference's gradient cache + BSplinePrecomputedImage)."""
 
-    def __init__(self, ctx: _lib.Context, reference: np.ndarray, deformed: np.ndarray, cfg: SubsetConfig,
+    def __init__(self, ctx: _lib.Context, reference, deformed, cfg: SubsetConfig,
                  precision: int, gradient_mode: int = _lib.GRADIENT_CENTRAL):
-        ref = np.ascontiguousarray(reference, np.float32)
-        dfm = np.ascontiguousarray(deformed, np.float32)
+        """reference / deformed: host arrays (uploaded here) or DeviceImage handles (borrowed)."""
         self.ctx = ctx
-        self.width, self.height = ref.shape[1], ref.shape[0]
+        self._owned = not isinstance(reference, _lib.DeviceImage)
+        if self._owned:
+            ref = np.ascontiguousarray(reference, np.float32)
+            dfm = np.ascontiguousarray(deformed, np.float32)
+            self.ref = ctx.upload_image(ref)
+            self.deformed = ctx.upload_image(dfm)
+        else:
+            self.ref, self.deformed = reference, deformed
+        self.width, self.height = self.ref.width, self.ref.height
         self.precision = precision
-        self.ref = ctx.upload_image(ref)
-        self.deformed = ctx.upload_image(dfm)
         ip = cfg.image_precompute
         self.grad = ctx.ref_gradient(self.ref, gradient_mode, ip.degree, ip.border, precision)
         self.coeffs = ctx.prefilter(self.deformed, ip.degree, ip.border, ip.use_exact_prefilter, precision)
 
     def free(self):
-        for h in (self.coeffs, self.grad, self.deformed, self.ref):
+        for h in (self.coeffs, self.grad) + ((self.deformed, self.ref) if self._owned else ()):
             h.free()
 
 
