@@ -301,6 +301,110 @@ def propagation_leg(pkg, ctx, reps=3):
     return out
 
 
+GOLDEN_SEED_CONFIG = {  # the reference's config/subset_2d.yaml, without the SIFT prior
+    "subset": {"radius": 41}, "optimization": {"method": "icgn", "max_iterations": 50, "convergence_threshold": 1e-6},
+    "interpolation": {"degree": 5},
+    "initialization": {"method": "integer_search",
+                       "integer_search": {"subset_radius": 10, "search_radius": 30, "sift_enabled": False,
+                                          "pyramid_enabled": False, "pyramid_scale": 4,
+                                          "pyramid_refinement_radius": 4},
+                       "subpixel_refinement": {"enabled": True, "subset_radius": 41, "max_iterations": 50,
+                                               "convergence_threshold": 1e-6}},
+    "seed_selection": {"seed_count": 16, "max_znssd": 2.0, "min_texture_std": 0.02, "kmeans_iterations": 20,
+                       "kmeans_sample_limit": 20000},
+    "reliability_propagation": {"spacing": 3, "max_znssd": 2.0},
+}
+
+
+def _orc_seed_params(orc, p):
+    o = orc.OrcSeedParams()
+    for name, _ in orc.OrcSeedParams._fields_:
+        if name != "search":
+            setattr(o, name, getattr(p, name))
+    for name, _ in orc.OrcSearchParams._fields_:
+        setattr(o.search, name, getattr(p.search, "mode" if name == "around_prior" else name))
+    return o
+
+
+def _search_shifts(W, H, sp):
+    """Shifts the integer search evaluates per point (coarse grid or every
+    shift, refinement window, local window) and samples per shift."""
+    r, sr = sp.subset_radius, sp.search_radius
+    cs = sp.pyramid_scale if sp.pyramid_enabled else 1
+    coarse = -(-(W - 2 * r) // cs) * -(-(H - 2 * r) // cs)
+    rr = max(sp.pyramid_refinement_radius, int(np.ceil(1.5 * r)), cs)
+    refine = (2 * rr + 1) ** 2
+    local = (2 * sr + 1) ** 2 if sr > 0 else 0
+    d = np.arange(-r, r + 1)
+    full = int(((d[:, None] ** 2 + d[None, :] ** 2) <= r * r).sum())
+    dc = np.arange(-r, r + 1, cs)
+    sub = int(((dc[:, None] ** 2 + dc[None, :] ** 2) <= r * r).sum())
+    return coarse, sub, refine + local, full
+
+
+def seeding_leg(pkg, ctx, wl, reps=3):
+    """Seed selection (SeedSelector::select_best_seed, integer-search path):
+    (1) the reference's stereo-plate case with its configuration (16 k-means
+    candidates, subset radius 10, every shift of the whole image at step 1,
+    local window 30, sub-pixel IC-GN radius 41, thr 1e-6); (2) the Python
+    facade's defaults (64 candidates, coarse grid every 4 px, window 20,
+    sub-pixel radius 15) on the 2048^2 headline pair.  GPU: select_seed end to
+    end from resident images, and the integer search alone; CPU: the oracle's
+    select_seed on every host thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import read_gray  # fixture helper (data only)
+    from oracle import oracle as orc
+    st = np.load(os.path.join(ROOT, "tests", "golden", "stereo_plate.npz"), allow_pickle=False)
+    cases = [("stereo_plate_reference_config", read_gray(st["img_00_L"]), read_gray(st["img_04_L"]), st["roi"],
+              pkg.config_from_dict(GOLDEN_SEED_CONFIG)),
+             ("python_defaults_2048", wl.reference, wl.deformed, np.ones(wl.reference.shape, np.uint8),
+              pkg.config_from_dict({"subset": {"radius": 15},
+                                    "initialization": {"integer_search": {"subset_radius": 10, "search_radius": 20},
+                                                       "subpixel_refinement": {"subset_radius": 15}},
+                                    "seed_selection": {"seed_count": 64, "max_znssd": 0.3,
+                                                       "min_texture_std": 0.02}}))]
+    out = {}
+    for name, ref, dfm, roi, cfg in cases:
+        p = pkg.seed_params(cfg)
+        g_ref, g_dfm = ctx.upload_image(ref), ctx.upload_image(dfm)
+        ev, best = ctx.select_seed(p, g_ref, g_dfm, roi)  # warm
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ev, best = ctx.select_seed(p, g_ref, g_dfm, roi)
+            ts.append(time.perf_counter() - t0)
+        xy = np.stack([ev["x"], ev["y"]], 1)
+        sp = p.search
+        ctx.integer_search(sp, g_ref, g_dfm, xy)
+        tsr = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ctx.integer_search(sp, g_ref, g_dfm, xy)
+            tsr.append(time.perf_counter() - t0)
+        g_ref.free()
+        g_dfm.free()
+        t_sel, t_search = float(np.median(ts)), float(np.median(tsr))
+        H, W = ref.shape
+        n_coarse, s_coarse, n_fine, s_fine = _search_shifts(W, H, sp)
+        samples = len(ev) * (n_coarse * s_coarse + n_fine * s_fine)
+        flops = 7 * samples  # per sample: 1 add (mean pass), 2 sub + 2 mul + 2 add (products pass)
+        t0 = time.perf_counter()
+        oev, obest = orc.select_seed(ref, dfm, roi, _orc_seed_params(orc, p))
+        t_cpu = time.perf_counter() - t0
+        same = bool(len(oev) == len(ev) and best == obest and np.array_equal(oev[["x", "y"]], ev[["x", "y"]]))
+        b = ev[best] if best >= 0 else None
+        out[name] = {
+            "image": f"{W}x{H}", "candidates": int(len(ev)), "best": None if b is None else
+            [float(b["x"]), float(b["y"]), float(b["displacement"]["p"][0]), float(b["displacement"]["p"][1])],
+            "gpu_select_s": round(t_sel, 4), "gpu_search_s": round(t_search, 4),
+            "search_samples": int(samples), "search_gsamples_per_s": round(samples / t_search / 1e9, 2),
+            "search_fp64_tflops": round(flops / t_search / 1e12, 2),
+            "cpu_oracle_select_s": round(t_cpu, 3), "cpu_threads": os.cpu_count(),
+            "gpu_vs_cpu": round(t_cpu / t_sel, 1), "same_seed_as_oracle": same}
+        log(f"seeding {name}: {out[name]}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,6 +417,7 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=6.0)
     ap.add_argument("--no-stereo", action="store_true", help="skip the stereo matching leg")
     ap.add_argument("--no-propagation", action="store_true", help="skip the propagation leg")
+    ap.add_argument("--no-seeding", action="store_true", help="skip the seed-selection leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a cold 1 GiB device buffer (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -444,6 +549,7 @@ def main():
     serving = None if args.no_serving else serving_leg(pkg, ctx, synth, duration_s=args.serving_seconds)
     stereo = None if args.no_stereo else stereo_leg(pkg, ctx, synth)
     propagation = None if args.no_propagation else propagation_leg(pkg, ctx)
+    seeding = None if args.no_seeding else seeding_leg(pkg, ctx, wl)
 
     line = {
         "metric": "converged POIs/s (FP32 vs FP64 ICGN, |du|<1e-3 px); p50/p99 request latency",
@@ -476,6 +582,7 @@ def main():
         "serving": serving,
         "stereo": stereo,
         "propagation": propagation,
+        "seeding": seeding,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

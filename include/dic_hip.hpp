@@ -5,7 +5,9 @@
 //   ForwardGaussNewtonSolver            include/dic/subset/solver/forward_gauss_newton.hpp:11-107
 //   BSplineImagePreprocessor::compute   include/dic/interpolation/bspline.hpp:84-100
 //   BSplineInterpolator::value/gradient include/dic/interpolation/bspline.hpp:102-130
-//   Displacement2D, InitialDisplacement, SolverStatus, SubsetConfig
+//   IntegerSearchInitializer            include/dic/initialization/integer_search.hpp:32-71
+//   SeedSelector::select_best_seed      include/dic/subset/seed/seed_selector.hpp:20-64
+//   Displacement2D, InitialDisplacement, SolverStatus, SubsetConfig, the seed configs
 // Types are Eigen-free (points are plain doubles); each method keeps the
 // reference's argument meaning and reports per-POI outcomes in the result's
 // status exactly as the reference does.  Errors of the C-ABI (bad handles,
@@ -47,6 +49,39 @@ struct BSplinePrecomputeConfig {
     bool use_exact_prefilter{true};
 };
 
+// include/dic/initialization/seed_config.hpp:33-69
+struct SeedIntegerSearchConfig {
+    int subset_radius{10};
+    int search_radius{30};
+    bool pyramid_enabled{true};
+    int pyramid_scale{4};
+    int pyramid_refinement_radius{4};
+};
+
+struct SeedSubpixelRefinementConfig {
+    bool enabled{true};
+    bool second_order{false};         // shape_function == SecondOrder
+    bool forward_gauss_newton{false}; // optimizer == ForwardGaussNewton
+    bool ssd{false};                  // objective == SSD
+    int subset_radius{15};
+    int max_iterations{30};
+    double convergence_threshold{1e-3};
+};
+
+enum class SeedQualityMetric { ZNCC = DIC_QUALITY_ZNCC, ZNSSD = DIC_QUALITY_ZNSSD, SSD = DIC_QUALITY_SSD };
+
+struct SeedSelectionConfig {
+    int seed_count{256};
+    SeedQualityMetric quality_metric{SeedQualityMetric::ZNSSD};
+    double min_zncc{0.7};
+    double max_znssd{0.2};
+    double max_ssd{0.05};
+    double min_displacement_norm{0.0};
+    double min_texture_std{0.0};
+    int kmeans_iterations{20};
+    int kmeans_sample_limit{20000};
+};
+
 struct SubsetConfig {
     int subset_radius{15};
     int max_iterations{30};
@@ -55,9 +90,21 @@ struct SubsetConfig {
     bool second_order_shape{false}; // shape_function == SecondOrder
     bool ssd{false};                // objective == SSD (else ZNSSD)
     bool forward_gauss_newton{false}; // optimization == FGN (else IC-GN)
+    bool truncate_roi_subsets{false};
     double min_valid_sample_ratio{0.5};
     int min_valid_samples{12};
     BSplinePrecomputeConfig image_precompute{};
+    SeedIntegerSearchConfig seed_integer_search{};
+    SeedSubpixelRefinementConfig seed_subpixel{};
+    SeedSelectionConfig seed_selection{};
+};
+
+// InitialDisplacement with valid / zncc / znssd (the integer search's result).
+struct InitialEstimate {
+    double u{0.0}, v{0.0}, du_dx{0.0}, du_dy{0.0}, dv_dx{0.0}, dv_dy{0.0};
+    double confidence{0.0};
+    bool valid{false};
+    double zncc{0.0}, znssd{0.0};
 };
 
 inline void check(dic_ctx *ctx, int rc)
@@ -270,6 +317,163 @@ private:
         c.forward_gauss_newton = true;
         return c;
     }
+};
+
+inline dic_integer_search_params search_params(const SeedIntegerSearchConfig &c, int mode, bool masked)
+{
+    dic_integer_search_params p{};
+    p.subset_radius = c.subset_radius < 1 ? 1 : c.subset_radius; // integer_search.cpp:293-301
+    p.search_radius = c.search_radius < 0 ? 0 : c.search_radius;
+    p.pyramid_enabled = c.pyramid_enabled ? 1 : 0;
+    p.pyramid_scale = c.pyramid_scale < 1 ? 1 : c.pyramid_scale;
+    p.pyramid_refinement_radius = c.pyramid_refinement_radius < 1 ? 1 : c.pyramid_refinement_radius;
+    p.mode = mode;
+    p.masked = masked ? 1 : 0;
+    return p;
+}
+
+// IntegerSearchInitializer: estimate / estimate_with_mask /
+// estimate[_with_mask]_around_displacement over many points at once.
+class IntegerSearchInitializer {
+public:
+    explicit IntegerSearchInitializer(Context &ctx, SeedIntegerSearchConfig cfg = {}) : ctx_(ctx.get()), cfg_(cfg) {}
+
+    std::vector<InitialEstimate> estimate(const Image &reference, const Image &deformed,
+                                          const std::vector<double> &points_xy, const uint8_t *roi = nullptr) const
+    {
+        return run(reference, deformed, points_xy, nullptr, roi, DIC_SEARCH_GLOBAL);
+    }
+    std::vector<InitialEstimate> estimate_around_displacement(const Image &reference, const Image &deformed,
+                                                              const std::vector<double> &points_xy,
+                                                              const std::vector<double> &prior_uv,
+                                                              const uint8_t *roi = nullptr) const
+    {
+        if (prior_uv.size() != points_xy.size()) throw std::invalid_argument("dic_hip: one prior (u, v) per point");
+        return run(reference, deformed, points_xy, prior_uv.data(), roi, DIC_SEARCH_AROUND_PRIOR);
+    }
+
+private:
+    std::vector<InitialEstimate> run(const Image &reference, const Image &deformed, const std::vector<double> &xy,
+                                     const double *prior, const uint8_t *roi, int mode) const
+    {
+        const int n = (int)(xy.size() / 2);
+        const dic_integer_search_params p = search_params(cfg_, mode, roi != nullptr);
+        std::vector<dic_estimate> raw(n);
+        check(ctx_, dic_integer_search(ctx_, &p, reference.get(), deformed.get(), roi, xy.data(), prior, n,
+                                       raw.data()));
+        std::vector<InitialEstimate> out(n);
+        for (int i = 0; i < n; ++i)
+            out[i] = {raw[i].u,          raw[i].v,     raw[i].du_dx,    raw[i].du_dy,
+                      raw[i].dv_dx,      raw[i].dv_dy, raw[i].confidence, raw[i].valid != 0,
+                      raw[i].zncc,       raw[i].znssd};
+        return out;
+    }
+    dic_ctx *ctx_;
+    SeedIntegerSearchConfig cfg_;
+};
+
+struct SeedEvaluation {
+    double x{0.0}, y{0.0};
+    Displacement2D displacement{};
+    double quality{0.0};
+    double displacement_norm{0.0};
+    bool quality_passed{false};
+    bool valid{false};
+};
+
+struct SeedSelectionResult {
+    std::vector<SeedEvaluation> candidates;
+    SeedEvaluation best_seed{};
+    bool found{false};
+};
+
+// SeedSelector (integer-search path): generate_candidates / select_best_seed.
+class SeedSelector {
+public:
+    explicit SeedSelector(Context &ctx, SubsetConfig cfg = {}) : ctx_(ctx.get()), cfg_(cfg) {}
+
+    dic_seed_params params() const
+    {
+        dic_seed_params p{};
+        p.search = search_params(cfg_.seed_integer_search, DIC_SEARCH_GLOBAL, false);
+        const auto &s = cfg_.seed_subpixel;
+        p.subpixel_enabled = s.enabled ? 1 : 0;
+        p.subpixel_shape_order = s.second_order ? 2 : 1;
+        p.subpixel_optimizer = s.forward_gauss_newton ? DIC_OPTIMIZER_FGN : DIC_OPTIMIZER_ICGN;
+        p.subpixel_objective = s.ssd ? DIC_OBJECTIVE_SSD : DIC_OBJECTIVE_ZNSSD;
+        p.subpixel_radius = s.subset_radius < 1 ? 1 : s.subset_radius;
+        p.subpixel_max_iterations = s.max_iterations < 0 ? 0 : s.max_iterations;
+        p.subpixel_convergence_threshold = s.convergence_threshold;
+        const auto &q = cfg_.seed_selection;
+        p.seed_count = q.seed_count < 1 ? 1 : q.seed_count; // seed_selector.cpp:138-144
+        p.quality_metric = (int)q.quality_metric;
+        p.min_zncc = q.min_zncc;
+        p.max_znssd = q.max_znssd;
+        p.max_ssd = q.max_ssd;
+        p.min_displacement_norm = q.min_displacement_norm;
+        p.min_texture_std = q.min_texture_std;
+        p.kmeans_iterations = q.kmeans_iterations < 0 ? 0 : q.kmeans_iterations;
+        p.kmeans_sample_limit = q.kmeans_sample_limit < 1 ? 1 : q.kmeans_sample_limit;
+        p.truncate_roi_subsets = cfg_.truncate_roi_subsets ? 1 : 0;
+        p.min_valid_samples = cfg_.min_valid_samples;
+        p.min_valid_sample_ratio = cfg_.min_valid_sample_ratio;
+        p.degree = cfg_.image_precompute.degree;
+        p.border = cfg_.image_precompute.border;
+        return p;
+    }
+
+    std::vector<double> generate_candidates(const Image &reference, const uint8_t *roi) const
+    {
+        const dic_seed_params p = params();
+        std::vector<double> xy(2 * (size_t)p.seed_count);
+        int n = 0;
+        check(ctx_, dic_seed_candidates(ctx_, &p, reference.get(), roi, xy.data(), p.seed_count, &n));
+        xy.resize(2 * (size_t)n);
+        return xy;
+    }
+
+    SeedSelectionResult select_best_seed(const Image &reference, const Image &deformed, const uint8_t *roi) const
+    {
+        const dic_seed_params p = params();
+        std::vector<dic_seed_evaluation> raw((size_t)p.seed_count);
+        int n = 0, best = -1;
+        check(ctx_, dic_select_seed(ctx_, &p, reference.get(), deformed.get(), roi, raw.data(), p.seed_count, &n,
+                                    &best));
+        SeedSelectionResult out;
+        out.candidates.resize(n);
+        for (int i = 0; i < n; ++i) {
+            const dic_seed_evaluation &e = raw[i];
+            SeedEvaluation &s = out.candidates[i];
+            s.x = e.x;
+            s.y = e.y;
+            const dic_result &r = e.displacement;
+            Displacement2D &d = s.displacement;
+            d.x = r.x;
+            d.y = r.y;
+            d.u = r.p[0];
+            d.v = r.p[1];
+            d.du_dx = r.p[2];
+            d.du_dy = r.p[3];
+            d.dv_dx = r.p[4];
+            d.dv_dy = r.p[5];
+            d.correlation = r.correlation;
+            d.status = (SolverStatus)r.status;
+            d.valid = r.valid != 0;
+            s.quality = e.quality;
+            s.displacement_norm = e.displacement_norm;
+            s.quality_passed = e.quality_passed != 0;
+            s.valid = e.valid != 0;
+        }
+        if (best >= 0) {
+            out.best_seed = out.candidates[best];
+            out.found = true;
+        }
+        return out;
+    }
+
+private:
+    dic_ctx *ctx_;
+    SubsetConfig cfg_;
 };
 
 } // namespace dic_hip

@@ -29,7 +29,7 @@ namespace {
 constexpr double kSearchEps = 1e-12;    // kEpsilon, integer_search.cpp:10
 constexpr double kLocalMargin = 0.02;   // kLocalPreferenceZnccMargin, integer_search.cpp:11
 constexpr int kSearchBlock = 256;
-constexpr int kMaxSeeds = 2048;         // k-means centres held in LDS (32 KiB)
+constexpr int kMaxSeeds = 2048;         // k-means centres and their sums held in LDS (56 KiB)
 constexpr int kNearest = 50;            // seed_selector.cpp:300
 
 // One block of shifts: u = u0 + iu * step, v = v0 + iv * step, row-major.
@@ -683,29 +683,47 @@ __device__ __forceinline__ double dist2(double x0, double y0, double x1, double 
 }
 
 // run_kmeans (:70-110), assignment: nearest centre (first on ties).  The
-// coordinates are integers, so the integer sums are the reference's double
-// sums exactly, in any order.
+// coordinates are integers, so integer sums are the reference's double sums
+// exactly, in any order: each workgroup sums its kKmeansPoints points per
+// centre in LDS (int32 holds 1024 coordinates below 2^16) and adds them to the
+// global 64-bit sums once, instead of every point contending for a few
+// addresses.
+constexpr int kKmeansPoints = 1024;
+
 __global__ __launch_bounds__(256) void kmeans_assign_kernel(const int2 *sp, int ns, const double2 *centers, int k,
                                                             unsigned long long *sums, int *counts)
 {
     extern __shared__ double2 c_lds[];
+    int *acc = (int *)(c_lds + k); // per centre: sum x, sum y, count
     for (int i = threadIdx.x; i < k; i += blockDim.x) c_lds[i] = centers[i];
+    for (int i = threadIdx.x; i < 3 * k; i += blockDim.x) acc[i] = 0;
     __syncthreads();
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= ns) return;
-    const double px = (double)sp[p].x, py = (double)sp[p].y;
-    int best = 0;
-    double bd = INFINITY;
-    for (int i = 0; i < k; ++i) {
-        const double d = dist2(px, py, c_lds[i].x, c_lds[i].y);
-        if (d < bd) {
-            bd = d;
-            best = i;
+    const int first = blockIdx.x * kKmeansPoints;
+    const int last = min(ns, first + kKmeansPoints);
+    for (int p = first + threadIdx.x; p < last; p += blockDim.x) {
+        const int2 q = sp[p];
+        const double px = (double)q.x, py = (double)q.y;
+        int best = 0;
+        double bd = INFINITY;
+        for (int i = 0; i < k; ++i) {
+            const double d = dist2(px, py, c_lds[i].x, c_lds[i].y);
+            if (d < bd) {
+                bd = d;
+                best = i;
+            }
         }
+        atomicAdd(&acc[3 * best], q.x);
+        atomicAdd(&acc[3 * best + 1], q.y);
+        atomicAdd(&acc[3 * best + 2], 1);
     }
-    atomicAdd(&sums[2 * best], (unsigned long long)sp[p].x);
-    atomicAdd(&sums[2 * best + 1], (unsigned long long)sp[p].y);
-    atomicAdd(&counts[best], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        const int c = acc[3 * i + 2];
+        if (c == 0) continue;
+        atomicAdd(&sums[2 * i], (unsigned long long)acc[3 * i]);
+        atomicAdd(&sums[2 * i + 1], (unsigned long long)acc[3 * i + 1]);
+        atomicAdd(&counts[i], c);
+    }
 }
 
 __global__ void kmeans_update_kernel(double2 *centers, int k, unsigned long long *sums, int *counts)
@@ -998,8 +1016,8 @@ int run_candidates(dic_ctx *ctx, const dic_seed_params *p, const dic_image *ref,
     kmeans_init_kernel<<<(k + 255) / 256, 256, 0, s>>>(sp, ns, k, cen.as<double2>(), sums.as<unsigned long long>(),
                                                       cnts.as<int>());
     for (int it = 0; it < std::max(0, p->kmeans_iterations); ++it) {
-        kmeans_assign_kernel<<<(ns + 255) / 256, 256, sizeof(double2) * (size_t)k, s>>>(
-            sp, ns, cen.as<double2>(), k, sums.as<unsigned long long>(), cnts.as<int>());
+        kmeans_assign_kernel<<<(ns + kKmeansPoints - 1) / kKmeansPoints, 256, (sizeof(double2) + 12) * (size_t)k,
+                               s>>>(sp, ns, cen.as<double2>(), k, sums.as<unsigned long long>(), cnts.as<int>());
         kmeans_update_kernel<<<(k + 255) / 256, 256, 0, s>>>(cen.as<double2>(), k, sums.as<unsigned long long>(),
                                                             cnts.as<int>());
     }
