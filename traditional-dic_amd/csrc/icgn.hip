@@ -37,12 +37,29 @@
 #include <cmath>
 
 #include "bspline_device.hpp"
+#include <type_traits>
+
 #include "dic_internal.hpp"
 
 namespace dic {
 namespace {
 
 constexpr double kEps = 1e-12; // icgn.cpp:13
+
+// One POI per wave: the lanes of a wave exchange values through LDS, and a
+// wave's LDS operations execute in issue order, so the solver only has to keep
+// the compiler from reordering them (no workgroup barrier).  Several POIs can
+// then share a workgroup, each wave finishing on its own.
+#define WAVE_SYNC() asm volatile("" ::: "memory")
+
+// Coefficient tile in LDS shared by the POIs of a workgroup, in image cells:
+// a sample whose 6 x 6 (n x n) block starts at cell (ix, iy) reads it when
+// ix in [x0, x0 + w - n] and iy in [y0, y0 + h - n].
+struct Tile {
+    const float *t; // null: none
+    int x0, y0, w, h;
+};
+constexpr int kGroupPitch = 72, kGroupRows = 60, kGroupPois = 4;
 
 // Per-lane accumulator in the working precision.  Each lane sums at most
 // ceil(Ns / 64) terms (20 for r = 20, 45 for r = 30) and the 64 lane totals
@@ -94,6 +111,34 @@ template <int P, typename R> __device__ __forceinline__ void steepest_descent(R 
     }
 }
 
+// sum over the subset of w * sd_j with sd_j = g_t(j) * m_j(dx, dy): the FP32
+// kernels accumulate w gx and w gy against the monomials 1, x, y, x^2, xy, y^2
+// (two products per sample instead of twelve); the halves of the second-order
+// terms (icgn.cpp:142-165) are applied after the reduction, an exact scaling.
+template <int P, typename R>
+__device__ __forceinline__ void acc_sd(Acc<R> *acc, R wgx, R wgy, R lx, R ly, R xx, R xy, R yy)
+{
+    acc[0].add(wgx);
+    acc[1].add(wgy);
+    acc[2].add(wgx * lx);
+    acc[3].add(wgx * ly);
+    acc[4].add(wgy * lx);
+    acc[5].add(wgy * ly);
+    if constexpr (P == 12) {
+        acc[6].add(wgx * xx);
+        acc[7].add(wgx * xy);
+        acc[8].add(wgx * yy);
+        acc[9].add(wgy * xx);
+        acc[10].add(wgy * xy);
+        acc[11].add(wgy * yy);
+    }
+}
+__device__ __forceinline__ constexpr double sd_half(int j) { return (j == 6 || j == 8 || j == 9 || j == 11) ? 0.5 : 1.0; }
+
+__device__ __forceinline__ float buffer_f32(__amdgpu_buffer_rsrc_t rs, int voff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0));
+}
 
 struct IcgnShared {
     double H[144];    // Hessian, then its LDL^T factor (lower triangle)
@@ -261,7 +306,7 @@ template <int P> __device__ void ldlt_factor_wave(IcgnShared &sh, int lane, int 
         }
         if (lane == k) dg = akk;
     }
-    __syncthreads();
+    WAVE_SYNC();
     if (own) {
 #pragma unroll
         for (int j = 0; j < P; ++j)
@@ -270,7 +315,7 @@ template <int P> __device__ void ldlt_factor_wave(IcgnShared &sh, int lane, int 
     }
     ok_out = ret;
     pos_out = (sign == 0 || sign == 2);
-    __syncthreads();
+    WAVE_SYNC();
 }
 
 // Column `lane` of the LDLT solve operator (Eigen LDLT::_solve_impl:
@@ -430,7 +475,7 @@ __device__ bool second_order_solve_wave(IcgnShared &sh, int lane)
         const double pr = __shfl(a, k * 8 + c, 64);
         if (active) a = (r == k) ? a / piv : a - (fr / piv) * pr;
     }
-    __syncthreads();
+    WAVE_SYNC();
     int rank = 0;
 #pragma unroll 1
     for (int k = 0; k < nonzero; ++k)
@@ -439,7 +484,7 @@ __device__ bool second_order_solve_wave(IcgnShared &sh, int lane)
     // row k of the right-hand part holds unknown perm(column k) of X^T
     const int unknown = __shfl(perm, r * 8 + r, 64);
     if (active && c >= 6) sh.X[c - 6][unknown] = a;
-    __syncthreads();
+    WAVE_SYNC();
     return true;
 }
 
@@ -479,7 +524,9 @@ template <int D, typename R>
 __device__ __forceinline__ R sample_at(__amdgpu_buffer_rsrc_t rs, int pitch, int b, int ix, int iy, R tx, R ty)
 {
     using S = Spline<D>;
-    const int voff = ((iy + b - S::off) * pitch + (ix + b - S::off)) * (int)sizeof(R);
+    // row x pitch with the full-rate 24-bit multiply (both below 2^23: the
+    // padded image has fewer than 2^31 bytes)
+    const int voff = (__mul24(iy + b - S::off, pitch) + (ix + b - S::off)) * (int)sizeof(R);
     const int rowb = pitch * (int)sizeof(R);
     R wx[S::n], wy[S::n];
     R v = R(0);
@@ -512,6 +559,26 @@ __device__ __forceinline__ R sample_at(__amdgpu_buffer_rsrc_t rs, int pitch, int
     return v;
 }
 
+// The value of sample_at<D, float> from the tile: same weights, same order of
+// the products, so the same float.  Row offsets are immediates from one base.
+template <int D>
+__device__ __forceinline__ float sample_tile(const float *blk, float tx, float ty)
+{
+    using S = Spline<D>;
+    float wx[S::n], wy[S::n];
+    S::weights(tx, wx);
+    S::weights(ty, wy);
+    float v = 0.0f;
+#pragma unroll
+    for (int j = 0; j < S::n; ++j) {
+        float rv = 0.0f;
+#pragma unroll
+        for (int k = 0; k < S::n; ++k) rv += wx[k] * blk[j * kGroupPitch + k];
+        v += wy[j] * rv;
+    }
+    return v;
+}
+
 __device__ __forceinline__ void decode(int t, int &dx, int &dy)
 {
     dx = (int)(short)(t & 0xffff);
@@ -521,18 +588,34 @@ __device__ __forceinline__ void decode(int t, int &dx, int &dy)
 // One POI on the calling wave (64 lanes, the whole workgroup): sh is the
 // workgroup's LDS record, gbuf (FP64 ZNSSD only) holds nfull doubles.
 template <int P, int OBJ, int D, typename R, bool MASKED>
-__device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, IcgnShared &sh, double *gbuf)
+__device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, IcgnShared &sh, double *gbuf,
+                                          const Tile tile = Tile{nullptr, 0, 0, 0, 0})
 {
     constexpr bool ZNSSD = OBJ == DIC_OBJECTIVE_ZNSSD;
     constexpr bool FP64 = sizeof(R) == 8;
     constexpr bool TWOPASS = ZNSSD && FP64;
+    // first-order FP32 kernels read a shared tile (they are bound by the load
+    // data path; the second-order ones by the vector ALU)
+    constexpr bool TILED = !FP64 && !MASKED && P == 6;
 
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63; // one POI per wave
     const R *__restrict__ gxp = (const R *)a.gx;
     const R *__restrict__ gyp = (const R *)a.gy;
     const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<void *>(a.coeffs), (short)0, a.coeff_bytes, 0x00020000);
     const int W = a.W, Hh = a.H;
+    // FP32 kernels: reference, gradient planes and sample table through buffer
+    // views (one 32-bit offset per sample for all three images; W * H * 4 is
+    // below the coefficient image's size, itself < 2^31 bytes)
+    const int pix_bytes = W * Hh * 4;
+    const __amdgpu_buffer_rsrc_t grx = __builtin_amdgcn_make_buffer_rsrc(const_cast<R *>(gxp), (short)0, pix_bytes,
+                                                                         0x00020000);
+    const __amdgpu_buffer_rsrc_t gry = __builtin_amdgcn_make_buffer_rsrc(const_cast<R *>(gyp), (short)0, pix_bytes,
+                                                                         0x00020000);
+    const __amdgpu_buffer_rsrc_t frs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(a.ref), (short)0,
+                                                                         pix_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc(const_cast<int *>(a.table), (short)0,
+                                                                         a.nfull * 4, 0x00020000);
     const double px = a.xy[2 * poi], py = a.xy[2 * poi + 1];
     const int cx = (int)round(px), cy = (int)round(py);
 
@@ -601,7 +684,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
     // for P = 12; the one-pass variant also needs sum sd, sum fhat sd,
     // sum fhat, sum fhat^2 ----
     if (!early) {
-        __syncthreads();
+        WAVE_SYNC();
         const double fmean = sh.fmean, fnorm = sh.fnorm;
         const double hscale = ZNSSD ? 2.0 / (fnorm * fnorm) : 1.0;
         if constexpr (P == 6) {
@@ -657,7 +740,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                     if (lane == 0) sh.mom[T][k] = v;
                 }
             }
-            __syncthreads();
+            WAVE_SYNC();
             // parameter a: gradient type (0 gx, 1 gy), monomial exponents, coefficient
             constexpr int ty[12] = {0, 1, 0, 0, 1, 1, 0, 0, 0, 1, 1, 1};
             constexpr int pe[12] = {0, 0, 1, 0, 1, 0, 2, 1, 0, 2, 1, 0};
@@ -669,28 +752,26 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 sh.H[e] = hscale * ce[ia] * ce[ib] * sh.mom[ty[ia] + ty[ib]][d * (d + 1) / 2 + qq];
             }
         }
-        if constexpr (!TWOPASS && ZNSSD) {
+        if constexpr (!TWOPASS && ZNSSD) { // FP32 only (FP64 ZNSSD is two-pass)
             Acc<R> a_sd[P], a_fsd[P], a_f, a_f2;
             const R fm = (R)fmean, fin = (R)(1.0 / fnorm);
             for (int s = lane; s < a.nfull; s += 64) {
                 int dx, dy;
-                decode(a.table[s], dx, dy);
+                decode((int)__builtin_amdgcn_raw_buffer_load_b32(trs, 4 * s, 0, 0), dx, dy);
                 const int x = cx + dx, y = cy + dy;
                 if (!usable(x, y)) continue;
-                R sd[P];
-                steepest_descent<P, R>(gxp[(size_t)y * W + x], gyp[(size_t)y * W + x], (R)dx, (R)dy, sd);
-                const R fh = ((R)a.ref[(size_t)y * W + x] - fm) * fin;
+                const int vo = (__mul24(y, W) + x) * 4;
+                const R gx = buffer_f32(grx, vo), gy = buffer_f32(gry, vo);
+                const R fh = (buffer_f32(frs, vo) - fm) * fin;
+                const R lx = (R)dx, ly = (R)dy;
                 a_f.add(fh);
                 a_f2.add(fh * fh);
-#pragma unroll
-                for (int j = 0; j < P; ++j) {
-                    a_sd[j].add(sd[j]);
-                    a_fsd[j].add(fh * sd[j]);
-                }
+                acc_sd<P, R>(a_sd, gx, gy, lx, ly, lx * lx, lx * ly, ly * ly);
+                acc_sd<P, R>(a_fsd, fh * gx, fh * gy, lx, ly, lx * lx, lx * ly, ly * ly);
             }
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-                const double v1 = wave_sum(a_sd[j].value()), v2 = wave_sum(a_fsd[j].value());
+                const double v1 = wave_sum(a_sd[j].value()) * sd_half(j), v2 = wave_sum(a_fsd[j].value()) * sd_half(j);
                 if (lane == 0) {
                     sh.ssd[j] = v1;
                     sh.sfsd[j] = v2;
@@ -702,7 +783,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 sh.sfh2 = v2;
             }
         }
-        __syncthreads();
+        WAVE_SYNC();
         int ok, pos;
         if constexpr (P == 12) {
             ldlt_factor_wave<P>(sh, lane, ok, pos);
@@ -711,7 +792,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 ldlt_factor<P>(sh.H, sh.tr, sh.x, &ok, &pos);
                 sh.state = ok && pos;
             }
-            __syncthreads();
+            WAVE_SYNC();
             ok = pos = sh.state;
         }
         if (!(ok && pos)) {
@@ -719,7 +800,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
             status = DIC_NUMERICAL_FAILURE;
         } else {
             ldlt_operator_column<P>(sh, lane);
-            __syncthreads();
+            WAVE_SYNC();
         }
     }
 
@@ -737,7 +818,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
 #pragma unroll
             for (int j = 6; j < 12; ++j) sh.p[j] = 0.0;
         }
-        __syncthreads();
+        WAVE_SYNC();
         const double Wm1 = (double)(W - 1), Hm1 = (double)(Hh - 1);
         const int ns = sh.ns;
         const double fmean = uniform_f64(sh.fmean), fnorm = uniform_f64(sh.fnorm);
@@ -756,12 +837,13 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
             const int bx = cx + (int)ui, by = cy + (int)vi;
             R q[12];
 #pragma unroll
-            for (int j = 2; j < 12; ++j) q[j] = (R)p[j];
+            for (int j = 2; j < 12; ++j) q[j] = (R)(p[j] * sd_half(j)); // halves folded: q6, q8, q9, q11 meet x^2, y^2
 
             // whole warped subset provably inside [0, W-1] x [0, H-1]: the
             // per-sample bounds test can be skipped (the extent bounds the
             // linear part over the disc and the quadratic part by r^2 / 2)
             bool inside;
+            bool staged = false;
             {
                 const double rr = (double)a.radius;
                 double ex = rr * sqrt((1.0 + p[2]) * (1.0 + p[2]) + p[3] * p[3]);
@@ -772,16 +854,31 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 }
                 const double x0 = (double)cx + p[0], y0 = (double)cy + p[1], m = 0.01;
                 inside = !wild && x0 - ex - m >= 0.0 && x0 + ex + m <= Wm1 && y0 - ey - m >= 0.0 && y0 + ey + m <= Hm1;
+                if constexpr (TILED) {
+                    // every sample's block inside the tile (the same extent bound)
+                    if (inside && tile.t) {
+                        const int xlo = (int)floor(x0 - ex - m), xhi = (int)floor(x0 + ex + m);
+                        const int ylo = (int)floor(y0 - ey - m), yhi = (int)floor(y0 + ey + m);
+                        staged = xlo >= tile.x0 && ylo >= tile.y0 && xhi + Spline<D>::n <= tile.x0 + tile.w &&
+                                 yhi + Spline<D>::n <= tile.y0 + tile.h;
+                    }
+                }
             }
             bool oob = wild;
             Acc<R> s1, s2, sfg, sg[P];
+            // the sample pass, instantiated once per source of the coefficients
+            auto sample_pass = [&](auto from_tile) {
             for (int s = wild ? a.nfull : lane; s < a.nfull; s += 64) {
                 int dx, dy;
-                decode(a.table[s], dx, dy);
+                if constexpr (FP64)
+                    decode(a.table[s], dx, dy);
+                else
+                    decode((int)__builtin_amdgcn_raw_buffer_load_b32(trs, 4 * s, 0, 0), dx, dy);
                 const int x = cx + dx, y = cy + dy;
                 if (!usable(x, y)) continue;
                 int ix, iy;
                 R tx, ty;
+                R xx = R(0), xy = R(0), yy = R(0); // FP32: monomials shared by the warp and the accumulation
                 if constexpr (FP64) { // warp + bounds exactly as icgn.cpp:35-40, :559-575
                     const double lx = (double)dx, ly = (double)dy;
                     double wx = (double)cx + lx + p[0] + p[2] * lx + p[3] * ly;
@@ -804,9 +901,11 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                     R xl = lx + uf + q[2] * lx + q[3] * ly;
                     R yl = ly + vf + q[4] * lx + q[5] * ly;
                     if constexpr (P == 12) {
-                        const R dx22 = lx * lx * R(0.5), dy22 = ly * ly * R(0.5), dxy = lx * ly;
-                        xl += q[6] * dx22 + q[7] * dxy + q[8] * dy22;
-                        yl += q[9] * dx22 + q[10] * dxy + q[11] * dy22;
+                        xx = lx * lx;
+                        xy = lx * ly;
+                        yy = ly * ly;
+                        xl += q[6] * xx + q[7] * xy + q[8] * yy;
+                        yl += q[9] * xx + q[10] * xy + q[11] * yy;
                     }
                     // clamp before the int conversion; anything clamped is out of bounds
                     const R fx = fmin(fmax(floor(xl), R(-1e7)), R(1e7));
@@ -824,14 +923,39 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 }
                 R gxv = R(0), gyv = R(0), fv = R(0);
                 if constexpr (!TWOPASS) { // independent loads, issued before the gathers
-                    gxv = gxp[(size_t)y * W + x];
-                    gyv = gyp[(size_t)y * W + x];
-                    fv = (R)a.ref[(size_t)y * W + x];
+                    if constexpr (FP64) {
+                        gxv = gxp[(size_t)y * W + x];
+                        gyv = gyp[(size_t)y * W + x];
+                        fv = (R)a.ref[(size_t)y * W + x];
+                    } else {
+                        const int vo = (__mul24(y, W) + x) * 4;
+                        gxv = buffer_f32(grx, vo);
+                        gyv = buffer_f32(gry, vo);
+                        fv = buffer_f32(frs, vo);
+                    }
                 }
-                const R g = sample_at<D, R>(crs, a.pitch, a.b, ix, iy, tx, ty);
+                R g;
+                if constexpr (decltype(from_tile)::value)
+                    g = sample_tile<D>(tile.t + (iy - tile.y0) * kGroupPitch + (ix - tile.x0), tx, ty);
+                else
+                    g = sample_at<D, R>(crs, a.pitch, a.b, ix, iy, tx, ty);
                 if constexpr (TWOPASS) {
                     gbuf[s] = g;
                     s1.add(g);
+                } else if constexpr (!FP64) {
+                    const R lx = (R)dx, ly = (R)dy;
+                    R w;
+                    if constexpr (ZNSSD) {
+                        const R fh = (fv - fm) * fin;
+                        w = g - fm;
+                        s1.add(w);
+                        s2.add(w * w);
+                        sfg.add(fh * w);
+                    } else { // SSD residual e = f - g, icgn.cpp:945-950
+                        w = fv - g;
+                        s2.add(w * w);
+                    }
+                    acc_sd<P, R>(sg, w * gxv, w * gyv, lx, ly, xx, xy, yy);
                 } else {
                     R sd[P];
                     steepest_descent<P, R>(gxv, gyv, (R)dx, (R)dy, sd);
@@ -850,6 +974,15 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                         for (int j = 0; j < P; ++j) sg[j].add(e * sd[j]);
                     }
                 }
+            }
+            };
+            if constexpr (TILED) {
+                if (staged)
+                    sample_pass(std::integral_constant<bool, true>{});
+                else
+                    sample_pass(std::integral_constant<bool, false>{});
+            } else {
+                sample_pass(std::integral_constant<bool, false>{});
             }
             if (__any(oob)) {
                 status = DIC_INVALID_INPUT;
@@ -913,21 +1046,21 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 }
                 const double gnorm = sqrt(var), sc = 2.0 / fnorm;
 #pragma unroll
-                for (int j = 0; j < P; ++j) {
-                    const double v = wave_sum(sg[j].value());
+                for (int j = 0; j < P; ++j) { // FP32 only: sg holds the monomial sums
+                    const double v = wave_sum(sg[j].value()) * sd_half(j);
                     if (lane == 0) sh.red[j] = sc * (sh.sfsd[j] - (v - md * sh.ssd[j]) / gnorm);
                 }
                 if (lane == 0) sh.red[P] = sh.sfh2 - 2.0 * (SFG - md * sh.sfh) / gnorm + var / (gnorm * gnorm);
             } else {
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
-                    const double v = wave_sum(sg[j].value());
+                    const double v = wave_sum(sg[j].value()) * (FP64 ? 1.0 : sd_half(j));
                     if (lane == 0) sh.red[j] = v;
                 }
                 const double v = wave_sum(s2.value());
                 if (lane == 0) sh.red[P] = v;
             }
-            __syncthreads();
+            WAVE_SYNC();
             corr = sh.red[P];
             // ---- dp = -H+ G (icgn.cpp:594): lane i forms row i ----
             if (lane < P) {
@@ -936,7 +1069,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 for (int j = 0; j < P; ++j) d += sh.Hinv[lane * P + j] * sh.red[j];
                 sh.dp[lane] = -d;
             }
-            __syncthreads();
+            WAVE_SYNC();
             bool fin = true;
             double nrm = 0.0;
 #pragma unroll
@@ -983,7 +1116,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 }
                 sh.state = ok;
             }
-            __syncthreads();
+            WAVE_SYNC();
             if (!sh.state) {
                 status = DIC_NUMERICAL_FAILURE;
                 failed = true;
@@ -1043,6 +1176,105 @@ __global__ __launch_bounds__(64, (kMinWaves<P, R>)) void icgn_kernel(IcgnArgs a)
     const int poi = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
     if (poi >= a.n) return;
     solve_poi<P, OBJ, D, R, MASKED>(a, poi, sh, gbuf);
+}
+
+// ---- first-order FP32: kGroupPois neighbouring POIs per workgroup ----
+// The workgroup stages the coefficient cells around the POIs' starting
+// subsets (integer guess, strain up to r/10, 3 px margin) in LDS once; each
+// wave then solves its POI on its own and reads the tile in every iteration
+// whose warped subset lies inside it (the global gather otherwise), so the
+// 36 reads of a sample come from LDS instead of the load data path.
+
+// Grows the box [xlo, xhi] x [ylo, yhi] by POI poi's starting subset.
+__device__ __forceinline__ void grow_box(const IcgnArgs &a, int poi, int &xlo, int &ylo, int &xhi, int &yhi)
+{
+    const dic_initial ini = a.init[poi];
+    if (!(fabs(ini.u) < 1e6 && fabs(ini.v) < 1e6)) return;
+    const int x = (int)round(a.xy[2 * poi]) + (int)floor(ini.u), y = (int)round(a.xy[2 * poi + 1]) + (int)floor(ini.v);
+    const int e = a.radius + a.radius / 10 + 3;
+    xlo = min(xlo, x - e);
+    xhi = max(xhi, x + e);
+    ylo = min(ylo, y - e);
+    yhi = max(yhi, y + e);
+}
+
+// All threads of the workgroup: loads the box's cells into tile when it fits.
+template <int D>
+__device__ Tile stage_tile(const IcgnArgs &a, int xlo, int ylo, int xhi, int yhi, float *tile)
+{
+    using S = Spline<D>;
+    xlo = max(xlo, 0);
+    ylo = max(ylo, 0);
+    xhi = min(xhi, a.W - 1);
+    yhi = min(yhi, a.H - 1);
+    const int tw = xhi - xlo + S::n, th = yhi - ylo + S::n;
+    if (!(xhi >= xlo && yhi >= ylo && tw <= kGroupPitch && th <= kGroupRows)) return Tile{nullptr, 0, 0, 0, 0};
+    const __amdgpu_buffer_rsrc_t crs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a.coeffs), (short)0, a.coeff_bytes, 0x00020000);
+    const int c0 = xlo + a.b - S::off, r0 = ylo + a.b - S::off;
+    for (int e = threadIdx.x; e < kGroupPitch * th; e += blockDim.x) {
+        const int row = e / kGroupPitch, col = e - row * kGroupPitch;
+        if (col < tw) tile[e] = buffer_f32(crs, (__mul24(r0 + row, a.pitch) + c0 + col) * 4);
+    }
+    return Tile{tile, xlo, ylo, tw, th};
+}
+
+constexpr int kBigBox = 1 << 30;
+
+template <int OBJ, int D>
+__global__ __launch_bounds__(64 * kGroupPois, (kMinWaves<6, float>)) void icgn_group_kernel(IcgnArgs a)
+{
+    __shared__ IcgnShared sh[kGroupPois];
+    __shared__ float tile[kGroupPitch * kGroupRows];
+    const int ngroups = (a.n + kGroupPois - 1) / kGroupPois;
+    const int per = (ngroups + 7) >> 3; // XCD-blocked, as icgn_kernel
+    const int g = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    if (g >= ngroups) return;
+    int xlo = kBigBox, ylo = kBigBox, xhi = -kBigBox, yhi = -kBigBox;
+    for (int q = 0; q < kGroupPois; ++q)
+        if (g * kGroupPois + q < a.n) grow_box(a, g * kGroupPois + q, xlo, ylo, xhi, yhi);
+    const Tile t = stage_tile<D>(a, xlo, ylo, xhi, yhi, tile);
+    __syncthreads();
+    const int poi = g * kGroupPois + (int)(threadIdx.x >> 6);
+    if (poi >= a.n) return;
+    solve_poi<6, OBJ, D, float, false>(a, poi, sh[threadIdx.x >> 6], nullptr, t);
+}
+
+// Serving ticks, first order FP32: POIs g .. g + 3 of the tick's contiguous
+// ranges; the tile is staged when they all belong to one descriptor.
+template <int OBJ, int D>
+__global__ __launch_bounds__(64 * kGroupPois, (kMinWaves<6, float>)) void icgn_range_group_kernel(
+    const BatchPair *__restrict__ pairs, const int *__restrict__ offsets, int npairs)
+{
+    __shared__ IcgnShared sh[kGroupPois];
+    __shared__ float tile[kGroupPitch * kGroupRows];
+    const int total = offsets[npairs];
+    const int g0 = (int)blockIdx.x * kGroupPois;
+    if (g0 >= total) return;
+    auto pair_of = [&](int g) {
+        int lo = 0, hi = npairs;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (offsets[mid] <= g) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    };
+    const int gl = min(g0 + kGroupPois, total) - 1;
+    const int p0 = pair_of(g0), pl = pair_of(gl);
+    Tile t{nullptr, 0, 0, 0, 0};
+    if (p0 == pl) {
+        const IcgnArgs &a = pairs[p0].a;
+        int xlo = kBigBox, ylo = kBigBox, xhi = -kBigBox, yhi = -kBigBox;
+        for (int q = g0; q <= gl; ++q) grow_box(a, q - offsets[p0], xlo, ylo, xhi, yhi);
+        t = stage_tile<D>(a, xlo, ylo, xhi, yhi, tile);
+    }
+    __syncthreads();
+    const int g = g0 + (int)(threadIdx.x >> 6);
+    if (g >= total) return;
+    const int pid = __builtin_amdgcn_readfirstlane(pair_of(g));
+    solve_poi<6, OBJ, D, float, false>(pairs[pid].a, g - offsets[pid], sh[threadIdx.x >> 6], nullptr,
+                                       pid == p0 ? t : Tile{nullptr, 0, 0, 0, 0});
 }
 
 // Batched solving over many (pair, radius) descriptors: one launch per group
@@ -1233,7 +1465,7 @@ __device__ __forceinline__ void sample_vg(__amdgpu_buffer_rsrc_t rs, int pitch, 
                                           R &v, R &gx, R &gy)
 {
     using S = Spline<D>;
-    const int voff = ((iy + b - S::off) * pitch + (ix + b - S::off)) * (int)sizeof(R);
+    const int voff = (__mul24(iy + b - S::off, pitch) + (ix + b - S::off)) * (int)sizeof(R);
     const int rowb = pitch * (int)sizeof(R);
     R wx[S::n], wy[S::n], dwx[S::n], dwy[S::n];
     S::weights(tx, wx);
@@ -1636,6 +1868,12 @@ hipError_t launch_variant(const IcgnArgs &args, hipStream_t s)
 {
     if (args.fgn) return launch_fgn_variant<P, OBJ, D, R, MASKED>(args, s);
     constexpr bool TWOPASS = OBJ == DIC_OBJECTIVE_ZNSSD && sizeof(R) == 8;
+    if constexpr (P == 6 && sizeof(R) == 4 && !MASKED) {
+        const int groups = (args.n + kGroupPois - 1) / kGroupPois;
+        hipLaunchKernelGGL((icgn_group_kernel<OBJ, D>), dim3(8 * ((groups + 7) / 8)), dim3(64 * kGroupPois), 0, s,
+                           args);
+        return hipGetLastError();
+    }
     const size_t lds = TWOPASS ? sizeof(double) * (size_t)args.nfull : 0;
     auto kern = icgn_kernel<P, OBJ, D, R, MASKED>;
     if (lds > 48 * 1024) {
@@ -1844,6 +2082,8 @@ namespace {
 template <int P, int OBJ, int D>
 const void *range_func_prec(int precision)
 {
+    if constexpr (P == 6)
+        if (precision != DIC_PRECISION_FP64) return (const void *)icgn_range_group_kernel<OBJ, D>;
     return precision == DIC_PRECISION_FP64 ? (const void *)icgn_range_kernel<P, OBJ, D, double>
                                            : (const void *)icgn_range_kernel<P, OBJ, D, float>;
 }
@@ -1881,6 +2121,9 @@ hipError_t launch_range(const RangeLaunch &L, hipStream_t s)
     const int *offsets = L.offsets;
     int npairs = L.npairs;
     void *args[] = {(void *)&pairs, (void *)&offsets, (void *)&npairs};
+    if (L.shape_order == 1 && L.precision != DIC_PRECISION_FP64) // icgn_range_group_kernel
+        return hipLaunchKernel(func, dim3((unsigned)((L.total + kGroupPois - 1) / kGroupPois)), dim3(64 * kGroupPois),
+                               args, 0, s);
     return hipLaunchKernel(func, dim3((unsigned)L.total), dim3(64), args, lds, s);
 }
 
