@@ -96,6 +96,13 @@ struct IcgnArgs {
     int n;
 };
 
+// First-order FP32 solves of the per-pair and serving launches run this many
+// POIs per workgroup (one per wave) over a shared LDS tile of the
+// coefficients (72 x 60 cells); with a subset radius above 24 four POIs never
+// fit one tile, and the per-pair launch keeps one POI per workgroup.
+constexpr int kGroupPois = 4;
+inline bool grouped_radius(int radius) { return radius <= 24; }
+
 // One descriptor of the persistent batch kernel.
 struct BatchPair {
     IcgnArgs a;

@@ -59,7 +59,7 @@ struct Tile {
     const float *t; // null: none
     int x0, y0, w, h;
 };
-constexpr int kGroupPitch = 72, kGroupRows = 60, kGroupPois = 4;
+constexpr int kGroupPitch = 72, kGroupRows = 60;
 
 // Per-lane accumulator in the working precision.  Each lane sums at most
 // ceil(Ns / 64) terms (20 for r = 20, 45 for r = 30) and the 64 lane totals
@@ -1868,7 +1868,7 @@ hipError_t launch_variant(const IcgnArgs &args, hipStream_t s)
 {
     if (args.fgn) return launch_fgn_variant<P, OBJ, D, R, MASKED>(args, s);
     constexpr bool TWOPASS = OBJ == DIC_OBJECTIVE_ZNSSD && sizeof(R) == 8;
-    if constexpr (P == 6 && sizeof(R) == 4 && !MASKED) {
+    if (P == 6 && sizeof(R) == 4 && !MASKED && grouped_radius(args.radius)) {
         const int groups = (args.n + kGroupPois - 1) / kGroupPois;
         hipLaunchKernelGGL((icgn_group_kernel<OBJ, D>), dim3(8 * ((groups + 7) / 8)), dim3(64 * kGroupPois), 0, s,
                            args);
