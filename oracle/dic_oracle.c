@@ -1487,9 +1487,12 @@ static void integer_search_one(const orc_search_params *p, const float *ref, con
     const int scale = imax(1, p->pyramid_scale);
     const int refine_param = imax(1, p->pyramid_refinement_radius);
     const uint8_t *mroi = p->masked ? roi : NULL;
+    /* a point or prior that is not a finite image coordinate is invalid (the
+     * reference's integer conversion of it is undefined) */
+    if (!(fabs(px) < 1e9 && fabs(py) < 1e9)) return;
     if (p->around_prior) {
         /* estimate_around_displacement :519-579 / _with_mask_ :581-645 */
-        if (!isfinite(prior[0]) || !isfinite(prior[1])) return;
+        if (!(fabs(prior[0]) < 1e9 && fabs(prior[1]) < 1e9)) return;
         const int cx = (int)llround(px), cy = (int)llround(py);
         const int n = collect_samples(ref, W, H, mroi, cx, cy, r, 1, s);
         if (n < min_required(r, 1)) return;

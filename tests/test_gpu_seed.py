@@ -35,7 +35,9 @@ def _points():
     xs = np.concatenate([np.linspace(4, 250, 9), [30.5, 31.5, 128.49, 128.5]])
     ys = np.concatenate([np.linspace(3, 204, 5), [40.5, 41.5]])
     X, Y = np.meshgrid(xs, ys)
-    return np.stack([X.ravel(), Y.ravel()], 1)
+    pts = np.stack([X.ravel(), Y.ravel()], 1)
+    # points that are no image coordinate: reported invalid on both sides
+    return np.concatenate([pts, [[np.nan, 50.0], [60.0, np.inf], [3e9, 40.0], [-2e10, -5.0]]])
 
 
 SEARCH_CASES = {

@@ -217,10 +217,16 @@ __device__ void prep_point(const SearchArgs &a, int i)
     s.task[1] = empty_region();
     const int r = a.r;
     const double px = a.xy[2 * i], py = a.xy[2 * i + 1];
+    // a point that is not a finite image coordinate is invalid (its integer
+    // conversion would be undefined)
+    if (!(fabs(px) < 1e9 && fabs(py) < 1e9)) {
+        a.st[i] = s;
+        return;
+    }
     if (a.around_prior) {
         // estimate_around_displacement :519-579, _with_mask_ :581-645
         const double pu = a.prior[2 * i], pv = a.prior[2 * i + 1];
-        if (isfinite(pu) && isfinite(pv)) {
+        if (fabs(pu) < 1e9 && fabs(pv) < 1e9) {
             s.cx = (int)llround(px);
             s.cy = (int)llround(py);
             const bool inside = subset_stats<MASKED>(a.ref, a.roi, a.W, a.H, s.cx, s.cy, r, 1, s.n_full,
@@ -1120,8 +1126,9 @@ extern "C" int dic_seed_candidates(dic_ctx *ctx, const dic_seed_params *params, 
     if (rc == DIC_OK) rc = run_candidates(ctx, params, ref, roib.as<uint8_t>(), xyb.as<double>(), &k);
     k = std::min(k, capacity);
     if (rc == DIC_OK && k > 0)
-        rc = check_hip(ctx, hipMemcpy(xy_out, xyb.p, sizeof(double) * 2 * (size_t)k, hipMemcpyDeviceToHost),
-                       "candidate download");
+        rc = check_hip(ctx, hipMemcpyAsync(xy_out, xyb.p, sizeof(double) * 2 * (size_t)k, hipMemcpyDeviceToHost,
+                                           ctx->stream), "candidate download");
+    if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "candidate download");
     if (rc == DIC_OK) *n_out = k;
     return rc;
 }
