@@ -868,12 +868,14 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
             Acc<R> s1, s2, sfg, sg[P];
             // the sample pass, instantiated once per source of the coefficients
             auto sample_pass = [&](auto from_tile) {
+            // the next sample's table entry is loaded one sample ahead (reads
+            // past the table return 0), so a sample waits for its gather only
+            int tnext = (int)__builtin_amdgcn_raw_buffer_load_b32(trs, 4 * lane, 0, 0);
             for (int s = wild ? a.nfull : lane; s < a.nfull; s += 64) {
                 int dx, dy;
-                if constexpr (FP64)
-                    decode(a.table[s], dx, dy);
-                else
-                    decode((int)__builtin_amdgcn_raw_buffer_load_b32(trs, 4 * s, 0, 0), dx, dy);
+                const int tcur = tnext;
+                tnext = (int)__builtin_amdgcn_raw_buffer_load_b32(trs, 4 * (s + 64), 0, 0);
+                decode(tcur, dx, dy);
                 const int x = cx + dx, y = cy + dy;
                 if (!usable(x, y)) continue;
                 int ix, iy;
