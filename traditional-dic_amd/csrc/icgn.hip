@@ -85,6 +85,14 @@ __device__ __forceinline__ int wave_sum_i(int v)
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
     return v;
 }
+// t <= n for an integer n (NaN: false), compared in the integer domain: a
+// double copy of a uniform bound kept live across the iteration loop is a
+// VGPR pair, spilled to scratch memory in the FP32 second-order kernels
+__device__ __forceinline__ bool le_int(double t, int n)
+{
+    return t <= -1e9 || (t < 1e9 && (int)ceil(t) <= n);
+}
+
 __device__ __forceinline__ double uniform_f64(double v)
 {
     const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
@@ -134,6 +142,27 @@ __device__ __forceinline__ void acc_sd(Acc<R> *acc, R wgx, R wgy, R lx, R ly, R 
     }
 }
 __device__ __forceinline__ constexpr double sd_half(int j) { return (j == 6 || j == 8 || j == 9 || j == 11) ? 0.5 : 1.0; }
+
+// Parameter j of the warp as g_t(j) * dx^p(j) * dy^q(j) * sd_half(j) (the
+// steepest-descent images above, t = 0 for gx, 1 for gy).  The three tables
+// are packed two bits per parameter so that a run-time j is a shift and a
+// mask: a constexpr array indexed at run time is materialised in per-lane
+// scratch memory, once per POI.
+constexpr int kParamT[12] = {0, 1, 0, 0, 1, 1, 0, 0, 0, 1, 1, 1};
+constexpr int kParamP[12] = {0, 0, 1, 0, 1, 0, 2, 1, 0, 2, 1, 0};
+constexpr int kParamQ[12] = {0, 0, 0, 1, 0, 1, 0, 1, 2, 0, 1, 2};
+constexpr unsigned pack2(const int (&v)[12])
+{
+    unsigned r = 0;
+    for (int j = 0; j < 12; ++j) r |= unsigned(v[j]) << (2 * j);
+    return r;
+}
+struct ParamMonomial { int t, p, q; };
+__device__ __forceinline__ ParamMonomial param_monomial(int j)
+{
+    constexpr unsigned kT = pack2(kParamT), kP = pack2(kParamP), kQ = pack2(kParamQ);
+    return {int(kT >> (2 * j)) & 3, int(kP >> (2 * j)) & 3, int(kQ >> (2 * j)) & 3};
+}
 
 __device__ __forceinline__ float buffer_f32(__amdgpu_buffer_rsrc_t rs, int voff)
 {
@@ -741,15 +770,11 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 }
             }
             WAVE_SYNC();
-            // parameter a: gradient type (0 gx, 1 gy), monomial exponents, coefficient
-            constexpr int ty[12] = {0, 1, 0, 0, 1, 1, 0, 0, 0, 1, 1, 1};
-            constexpr int pe[12] = {0, 0, 1, 0, 1, 0, 2, 1, 0, 2, 1, 0};
-            constexpr int qe[12] = {0, 0, 0, 1, 0, 1, 0, 1, 2, 0, 1, 2};
-            constexpr double ce[12] = {1, 1, 1, 1, 1, 1, 0.5, 1, 0.5, 0.5, 1, 0.5};
             for (int e = lane; e < 144; e += 64) {
                 const int ia = e / 12, ib = e % 12;
-                const int pp = pe[ia] + pe[ib], qq = qe[ia] + qe[ib], d = pp + qq;
-                sh.H[e] = hscale * ce[ia] * ce[ib] * sh.mom[ty[ia] + ty[ib]][d * (d + 1) / 2 + qq];
+                const ParamMonomial ma = param_monomial(ia), mb = param_monomial(ib);
+                const int qq = ma.q + mb.q, d = ma.p + mb.p + qq;
+                sh.H[e] = hscale * sd_half(ia) * sd_half(ib) * sh.mom[ma.t + mb.t][d * (d + 1) / 2 + qq];
             }
         }
         if constexpr (!TWOPASS && ZNSSD) { // FP32 only (FP64 ZNSSD is two-pass)
@@ -852,8 +877,12 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                     ex += 0.5 * rr * rr * (fabs(p[6]) + fabs(p[7]) + fabs(p[8]));
                     ey += 0.5 * rr * rr * (fabs(p[9]) + fabs(p[10]) + fabs(p[11]));
                 }
-                const double x0 = (double)cx + p[0], y0 = (double)cy + p[1], m = 0.01;
-                inside = !wild && x0 - ex - m >= 0.0 && x0 + ex + m <= Wm1 && y0 - ey - m >= 0.0 && y0 + ey + m <= Hm1;
+                // cx + p0 as bx + (p0 - ui): the same real number (p0 - ui is exact), rounded
+                // once; built from this iteration's values, no double copy of the POI
+                // centre is kept live across the loop (only read when !wild)
+                const double x0 = (double)bx + (p[0] - ui), y0 = (double)by + (p[1] - vi), m = 0.01;
+                inside = !wild && x0 - ex - m >= 0.0 && le_int(x0 + ex + m, W - 1) && y0 - ey - m >= 0.0 &&
+                         le_int(y0 + ey + m, Hh - 1);
                 if constexpr (TILED) {
                     // every sample's block inside the tile (the same extent bound)
                     if (inside && tile.t) {
@@ -1725,15 +1754,12 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
             }
             __syncthreads();
             {
-                constexpr int ty[12] = {0, 1, 0, 0, 1, 1, 0, 0, 0, 1, 1, 1};
-                constexpr int pe[12] = {0, 0, 1, 0, 1, 0, 2, 1, 0, 2, 1, 0};
-                constexpr int qe[12] = {0, 0, 0, 1, 0, 1, 0, 1, 2, 0, 1, 2};
-                constexpr double ce[12] = {1, 1, 1, 1, 1, 1, 0.5, 1, 0.5, 0.5, 1, 0.5};
                 const double hsc = ZNSSD ? 2.0 / (fnorm * fnorm) : 1.0;
                 for (int e = lane; e < P * P; e += 64) {
                     const int ia = e / P, ib = e % P;
-                    const int pp = pe[ia] + pe[ib], qq = qe[ia] + qe[ib], d = pp + qq;
-                    sh.H[e] = hsc * ce[ia] * ce[ib] * sh.mom[ty[ia] + ty[ib]][d * (d + 1) / 2 + qq];
+                    const ParamMonomial ma = param_monomial(ia), mb = param_monomial(ib);
+                    const int qq = ma.q + mb.q, d = ma.p + mb.p + qq;
+                    sh.H[e] = hsc * sd_half(ia) * sd_half(ib) * sh.mom[ma.t + mb.t][d * (d + 1) / 2 + qq];
                 }
             }
             __syncthreads();
