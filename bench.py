@@ -250,6 +250,63 @@ def stereo_leg(pkg, ctx, synth, steps=5):
     return out
 
 
+def fgn_leg(pkg, ctx, wl, steps=5, stride=16):
+    """Forward Gauss-Newton (optimizer = FGN) on the 2048^2 large-field
+    workload: second order, radius 20, quintic, the same POIs and guesses as
+    the IC-GN headline, both precisions, inputs in HBM.  The oracle's FGN
+    solves every `stride`-th POI on the host threads (parity at full size)."""
+    import os
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as orc
+    n = wl.poi_xy.shape[0]
+    dref, ddef = ctx.upload_image(wl.reference), ctx.upload_image(wl.deformed)
+    dxy, dini = ctx.to_device(np.ascontiguousarray(wl.poi_xy)), ctx.to_device(wl.init)
+    cfg = pkg.SubsetConfig(subset_radius=20, shape_function="second_order", max_iterations=30,
+                           convergence_threshold=1e-3, optimizer="forward_gauss_newton")
+    cfg.image_precompute.degree = 5
+    res, ms = {}, {}
+    for name, prec in (("fp64", pkg.PRECISION_FP64), ("mixed", pkg.PRECISION_MIXED)):
+        grad = ctx.ref_gradient(dref, pkg.GRADIENT_CENTRAL, 5, 3, prec)
+        co = ctx.prefilter(ddef, 5, 3, True, prec)
+        out = ctx.alloc(n * pkg.RESULT_DTYPE.itemsize)
+        params = pkg.make_params(cfg, prec)
+        ms[name] = _median_ms(ctx, lambda: ctx.solve_device(params, dref, grad, co, dxy.ptr, dini.ptr, n, out.ptr),
+                              steps)
+        res[name] = out.download(pkg.RESULT_DTYPE)
+    fp, mx = res["fp64"], res["mixed"]
+    ok64 = (fp["status"] == pkg.SUCCESS) & (fp["valid"] == 1)
+    d = np.maximum(np.abs(mx["p"][:, 0] - fp["p"][:, 0]), np.abs(mx["p"][:, 1] - fp["p"][:, 1]))
+    okmx = (mx["status"] == pkg.SUCCESS) & (mx["valid"] == 1) & ok64 & (d <= 1e-3)
+    tu, tv, _ = wl.truth()
+    truth = np.hypot(fp["p"][ok64, 0] - tu[ok64], fp["p"][ok64, 1] - tv[ok64])
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    sel = np.arange(0, n, stride)
+    pre = orc.Precomputed(wl.deformed, degree=5)
+    c0 = time.perf_counter()
+    cres = orc.fgn_solve(wl.reference, pre, wl.poi_xy[sel], wl.init[sel], radius=20, max_iterations=30,
+                         threshold=1e-3, shape_order=2, threads=threads)
+    t_cpu = time.perf_counter() - c0
+    okc = cres["status"] == pkg.SUCCESS
+    dcpu = np.maximum(np.abs(fp["p"][sel, 0] - cres["p"][:, 0]), np.abs(fp["p"][sel, 1] - cres["p"][:, 1]))
+    out = {"workload": "large_field_2048 with optimizer FGN: 2048x2048, 161604 POIs, radius 20, second order, "
+                       "quintic, ZNSSD, max 30 iterations, threshold 1e-3",
+           "pois": int(n), "fp64_ms": round(ms["fp64"], 3), "mixed_ms": round(ms["mixed"], 3),
+           "converged": int(okmx.sum()), "converged_fp64": int(ok64.sum()),
+           "mean_iterations_fp64": round(float(fp["iterations"].mean()), 3),
+           "value": round(okmx.sum() / (ms["mixed"] * 1e-3), 1),
+           "fp64_value": round(ok64.sum() / (ms["fp64"] * 1e-3), 1),
+           "mixed_vs_fp64_max_px": float(np.max(d[ok64], initial=0.0)),
+           "fp64_rms_error_vs_truth_px": round(float(np.sqrt(np.mean(truth ** 2))), 5),
+           "cpu_oracle": {"pois": int(sel.size), "threads": threads, "s": round(t_cpu, 3),
+                          "converged_pois_per_s": round(okc.sum() / t_cpu, 1),
+                          "status_agreement": float(np.mean(cres["status"] == fp["status"][sel])),
+                          "fp64_gpu_vs_oracle_max_px": float(np.max(dcpu[okc], initial=0.0))}}
+    for h in (dref, ddef, dxy, dini):
+        h.free()
+    log(f"fgn: {out}")
+    return out
+
+
 def propagation_leg(pkg, ctx, reps=3):
     """Reliability-guided propagation over the reference's stereo-plate case
     (1440x1080, ROI, radius 41, quintic, ||dp|| < 1e-6, spacing 3, one seed),
@@ -416,6 +473,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream serving leg")
     ap.add_argument("--serving-seconds", type=float, default=6.0)
     ap.add_argument("--no-stereo", action="store_true", help="skip the stereo matching leg")
+    ap.add_argument("--no-fgn", action="store_true", help="skip the forward Gauss-Newton leg")
     ap.add_argument("--no-propagation", action="store_true", help="skip the propagation leg")
     ap.add_argument("--no-seeding", action="store_true", help="skip the seed-selection leg")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -549,6 +607,7 @@ def main():
     batched = None if args.no_batched else batched_leg(pkg, ctx, synth)
     serving = None if args.no_serving else serving_leg(pkg, ctx, synth, duration_s=args.serving_seconds)
     stereo = None if args.no_stereo else stereo_leg(pkg, ctx, synth)
+    fgn = None if args.no_fgn else fgn_leg(pkg, ctx, wl)
     propagation = None if args.no_propagation else propagation_leg(pkg, ctx)
     seeding = None if args.no_seeding else seeding_leg(pkg, ctx, wl)
 
@@ -582,6 +641,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "stereo": stereo,
+        "fgn": fgn,
         "propagation": propagation,
         "seeding": seeding,
     }
