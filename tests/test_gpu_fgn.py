@@ -84,3 +84,35 @@ def test_fgn_step_cap_one_step(pkg, ctx, orc, pair):
     ok = gpu["status"] == 0
     step = np.linalg.norm(gpu["p"][ok, :6] - np.array([2.5, 0, 0, 0, 0, 0]), axis=1)
     assert ok.any() and np.all(step <= 0.5 + 1e-12) and np.any(np.abs(step - 0.5) < 1e-12)
+
+
+def test_subset_beyond_lds_is_refused_and_leaves_no_error_behind(pkg, ctx, pair):
+    """FGN in FP64 keeps three doubles per sample in LDS beside the per-POI
+    block: radius 46 (6625 samples, 162504 B) fits the 160 KiB of one
+    workgroup, radius 47 (6921 samples) does not and is refused before any
+    launch, as is FP64 ZNSSD IC-GN (one double per sample) at radius 80; the
+    solves after a refusal run and give the same results as before it."""
+    ref, dfm, f, xy = pair
+    c = np.array([[96.0, 96.0]])
+    u, v, _ = f.displacement(c[:, 0], c[:, 1])
+    ini = pkg.initial_guesses(np.round(u), np.round(v))
+    small = _gpu(pkg, ctx, ref, dfm, xy[:20], pkg.initial_guesses(np.zeros(20), np.zeros(20)), 2, "znssd",
+                 pkg.PRECISION_FP64)
+    big = _gpu(pkg, ctx, ref, dfm, c, ini, 2, "znssd", pkg.PRECISION_FP64, max_it=5)
+    _report(f"radius 46 FGN FP64: status {big['status'].tolist()}")
+    assert big["status"][0] in (pkg.SUCCESS, pkg.NOT_CONVERGED)
+    cases = [("forward_gauss_newton", 47), ("icgn", 80)]
+    for optimizer, radius in cases:
+        cfg = pkg.SubsetConfig(subset_radius=radius, shape_function="second_order", optimizer=optimizer)
+        cfg.image_precompute.degree = 3
+        params = pkg.make_params(cfg, pkg.PRECISION_FP64)
+        dref, ddef = ctx.upload_image(ref), ctx.upload_image(dfm)
+        co = ctx.prefilter(ddef, 3, 3, True, pkg.PRECISION_FP64)
+        grad = ctx.ref_gradient(dref, 0, 3, 3, pkg.PRECISION_FP64)
+        with pytest.raises(pkg.DicError) as err:
+            ctx.solve(params, dref, grad, co, c, ini)
+        _report(f"{optimizer} radius {radius}: {err.value}")
+        assert err.value.code == pkg._lib.DIC_ERR_UNSUPPORTED and "LDS" in str(err.value)
+        again = _gpu(pkg, ctx, ref, dfm, xy[:20], pkg.initial_guesses(np.zeros(20), np.zeros(20)), 2, "znssd",
+                     pkg.PRECISION_FP64)
+        assert again.tobytes() == small.tobytes()

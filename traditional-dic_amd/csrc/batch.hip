@@ -226,6 +226,7 @@ int dic_batch_create(dic_ctx *ctx, const dic_batch_item *items, int n_items, int
         const int *table = nullptr;
         int count = 0;
         rc = get_subset_table(ctx, it.subset_radius, &table, &count);
+        if (rc == DIC_OK) rc = check_solve_lds(ctx, false, c->precision, objective, it.subset_radius, count);
         if (rc != DIC_OK) break;
         BatchPair bp{};
         IcgnArgs &a = bp.a;

@@ -20,7 +20,29 @@ int set_error(dic_ctx *ctx, int code, const std::string &msg)
 int check_hip(dic_ctx *ctx, hipError_t e, const char *what)
 {
     if (e == hipSuccess) return DIC_OK;
+    // a failed runtime call also leaves its code as the thread's last error;
+    // consume it so that the next launch's hipGetLastError does not report it
+    (void)hipGetLastError();
     return set_error(ctx, DIC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+size_t solve_dynamic_lds(bool fgn, int precision, int objective, int nfull)
+{
+    const size_t elem = precision == DIC_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+    if (fgn) return 3 * elem * (size_t)nfull;
+    const bool twopass = precision == DIC_PRECISION_FP64 && objective == DIC_OBJECTIVE_ZNSSD;
+    return twopass ? sizeof(double) * (size_t)nfull : 0;
+}
+
+int check_solve_lds(dic_ctx *ctx, bool fgn, int precision, int objective, int radius, int nfull)
+{
+    const size_t need = solve_static_lds() + solve_dynamic_lds(fgn, precision, objective, nfull);
+    if (ctx->max_lds == 0 || need <= ctx->max_lds) return DIC_OK;
+    return set_error(ctx, DIC_ERR_UNSUPPORTED,
+                     "subset radius " + std::to_string(radius) + " (" + std::to_string(nfull) + " samples) needs " +
+                         std::to_string(need) + " bytes of LDS per POI with " + (fgn ? "FGN" : "IC-GN") + " in " +
+                         (precision == DIC_PRECISION_FP64 ? "FP64" : "mixed precision") + "; the device allows " +
+                         std::to_string(ctx->max_lds));
 }
 
 hipError_t ctx_malloc(dic_ctx *ctx, void **ptr, size_t bytes)
@@ -132,6 +154,9 @@ int dic_ctx_create(int device, uint64_t hbm_budget_bytes, dic_ctx **out)
     if (!ctx) return DIC_ERR_HIP;
     ctx->device = device;
     ctx->budget = hbm_budget_bytes;
+    int max_lds = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && max_lds > 0)
+        ctx->max_lds = (size_t)max_lds;
     if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return DIC_ERR_HIP;
@@ -569,6 +594,9 @@ int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref,
     int count = 0;
     if (params->subset_radius >= 1) {
         int rc = get_subset_table(ctx, params->subset_radius, &table, &count);
+        if (rc != DIC_OK) return rc;
+        rc = check_solve_lds(ctx, params->optimizer == DIC_OPTIMIZER_FGN, params->precision, params->objective,
+                             params->subset_radius, count);
         if (rc != DIC_OK) return rc;
     }
     IcgnLaunch L;

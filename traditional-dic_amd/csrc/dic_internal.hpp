@@ -26,6 +26,8 @@ struct dic_ctx {
     // scratch used by the host-pointer entry points
     void *staging = nullptr;
     size_t staging_bytes = 0;
+    // LDS one workgroup may use on this device (hipDeviceAttributeMaxSharedMemoryPerBlock)
+    size_t max_lds = 0;
 };
 
 struct dic_image {
@@ -65,6 +67,14 @@ int validate_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image 
 int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
               const dic_coeffs *def, const uint8_t *roi_dev, const double *xy, const dic_initial *init, int n,
               dic_result *out, hipStream_t s);
+
+// LDS of one solve workgroup: the static per-POI block (icgn.hip) plus the
+// per-sample buffer of FP64 two-pass ZNSSD IC-GN (one value) or of FGN (value
+// and two gradient terms); check_solve_lds turns a subset too large for the
+// device into DIC_ERR_UNSUPPORTED before any launch
+size_t solve_static_lds();
+size_t solve_dynamic_lds(bool fgn, int precision, int objective, int nfull);
+int check_solve_lds(dic_ctx *ctx, bool fgn, int precision, int objective, int radius, int nfull);
 
 // one-line sample table of a circular subset (dy outer, dx inner, dx^2+dy^2 <= r^2)
 int get_subset_table(dic_ctx *ctx, int radius, const int **table, int *count);

@@ -1938,6 +1938,8 @@ hipError_t dispatch_degree(const IcgnArgs &args, int degree, int precision, bool
 
 } // namespace
 
+size_t solve_static_lds() { return sizeof(IcgnShared); }
+
 hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s)
 {
     if (L.n <= 0) return hipSuccess;

@@ -116,6 +116,7 @@ int fail(dic_server *s, int code, const std::string &msg)
 int hipcheck(dic_server *s, hipError_t e, const char *what)
 {
     if (e == hipSuccess) return DIC_OK;
+    (void)hipGetLastError(); // consumed, as check_hip does
     return fail(s, DIC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
@@ -493,9 +494,7 @@ int dic_server_create(dic_ctx *ctx, const dic_server_config *cfg, dic_server **o
     s->border = b;
     s->t0 = std::chrono::steady_clock::now();
     int rc = get_subset_table(ctx, c.subset_radius, &s->table, &s->nfull);
-    const bool twopass = c.objective == DIC_OBJECTIVE_ZNSSD && c.precision == DIC_PRECISION_FP64;
-    if (rc == DIC_OK && twopass && (size_t)s->nfull * sizeof(double) > 128 * 1024)
-        rc = set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "server: FP64 ZNSSD subset too large for LDS");
+    if (rc == DIC_OK) rc = check_solve_lds(ctx, false, c.precision, c.objective, c.subset_radius, s->nfull);
     auto hc = [&](hipError_t e, const char *w) {
         if (rc == DIC_OK) rc = check_hip(ctx, e, w);
     };
