@@ -190,6 +190,43 @@ def test_icgn_failure_statuses(pkg, ctx, orc, synth):
     assert (r["status"] == 2).all() and np.array_equal(r["p"][:, 0], ini["u"])
 
 
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_icgn_edge_inputs(pkg, ctx, orc, small_pair, order, precision):
+    """No POI at all; POIs whose subset touches the image border exactly or
+    crosses it by one pixel; guesses that are not finite or lie far outside
+    the image (InvalidInput, the guess kept, as icgn.cpp:35-40 and :464-470
+    give); a single POI solved alone equals its row of the whole batch."""
+    ref, dfm, _, u0, v0 = small_pair
+    r, W = 15, ref.shape[1]
+    xy = np.array([[r, 96], [W - 1 - r, 96], [96, r], [96, W - 1 - r], [r - 1, 96], [W - r, 96], [96, 96],
+                   [96, 96], [96, 96], [96, 96], [96, 96], [96, 96]], float)
+    nan, inf = np.nan, np.inf
+    u = np.array([1, 1, 1, 1, 1, 1, 1, nan, inf, 1e8, 1, -3e7])
+    v = np.array([-1, -1, -1, -1, -1, -1, -1, 0, 0, 0, -1, 0])
+    dudx = np.array([0, 0, 0, 0, 0, 0, 0, 0, 0, 0, nan, 0])
+    ini = _ini(pkg, u, v, dudx, conf=0.3)
+    s = _solver(pkg, ctx, r, order=order, precision=precision)
+    gpu = s.solve_batch(ref, dfm, xy, ini)
+    cpu = _oracle(orc, ref, dfm, xy, ini, r, order=order)
+    _report(f"edge inputs order={order} {precision}: gpu {gpu['status'].tolist()} oracle {cpu['status'].tolist()}")
+    assert np.array_equal(gpu["status"], cpu["status"])
+    assert np.array_equal(gpu["valid"], cpu["valid"])
+    if precision == "fp64":
+        assert np.array_equal(gpu["iterations"], cpu["iterations"])
+    assert (cpu["status"][[4, 5, 7, 8, 9, 11]] == pkg.INVALID_INPUT).all()
+    ok = cpu["status"] == pkg.SUCCESS
+    assert ok[6]
+    tol = TOL_UV if precision == "fp64" else 1e-3
+    assert np.abs(gpu["p"][ok, :2] - cpu["p"][ok, :2]).max() <= tol
+    # failure exits keep the initial u, v (NaN and infinity included)
+    np.testing.assert_array_equal(gpu["p"][~ok, :2], cpu["p"][~ok, :2])
+    empty = s.solve_batch(ref, dfm, xy[:0], ini[:0])
+    assert empty.shape == (0,)
+    one = s.solve_batch(ref, dfm, xy[6:7], ini[6:7])
+    assert one.tobytes() == gpu[6:7].tobytes()
+
+
 def test_plain_512_parity_and_accuracy(pkg, ctx, orc, synth):
     """512^2 8-bit speckle, affine, radius 15, first order, bicubic (2304 POIs)."""
     wl = synth.plain_512()
