@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdic_oracle.so")
+# DIC_ORACLE_LIB selects another build of dic_oracle.c, e.g. the sanitizer
+# build of `make -C oracle sanitize`
+LIB_PATH = os.environ.get("DIC_ORACLE_LIB") or os.path.join(HERE, "libdic_oracle.so")
 
 RESULT_DTYPE = np.dtype([
     ("x", "<f8"), ("y", "<f8"), ("p", "<f8", (12,)), ("correlation", "<f8"),
