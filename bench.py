@@ -119,10 +119,19 @@ def batched_leg(pkg, ctx, synth, reps=3):
     t_mix_pp = _median_ms(ctx, lambda: per_pair(st["mixed"][0]), reps)
     b = ctx.batch(st["mixed"][0])
     a0 = time.time()
-    cands = [(pkg.BATCH_GRID, 0, 1, pkg.BATCH_ORDER_NATURAL), (pkg.BATCH_GRID, 0, 1, pkg.BATCH_ORDER_COST),
-             (pkg.BATCH_GRID, 0, 1, pkg.BATCH_ORDER_INTERLEAVE), (pkg.BATCH_GRID, 0, 1, pkg.BATCH_ORDER_XCD),
-             (pkg.BATCH_PERSISTENT, 16, 1, pkg.BATCH_ORDER_COST), (pkg.BATCH_PERSISTENT, 8, 4, pkg.BATCH_ORDER_COST)]
-    best, best_ms = b.autotune(cands, reps=2)
+    cands = {"grid": [(pkg.BATCH_GRID, 0, 1, pkg.BATCH_ORDER_NATURAL), (pkg.BATCH_GRID, 0, 1, pkg.BATCH_ORDER_COST),
+                      (pkg.BATCH_GRID, 0, 1, pkg.BATCH_ORDER_INTERLEAVE), (pkg.BATCH_GRID, 0, 1, pkg.BATCH_ORDER_XCD)],
+             "persistent": [(pkg.BATCH_PERSISTENT, 16, 1, pkg.BATCH_ORDER_COST),
+                            (pkg.BATCH_PERSISTENT, 8, 4, pkg.BATCH_ORDER_COST)],
+             "tile": [(pkg.BATCH_TILE, 0, 1, pkg.BATCH_ORDER_NATURAL), (pkg.BATCH_TILE, 0, 1, pkg.BATCH_ORDER_COST),
+                      (pkg.BATCH_TILE, 0, 1, pkg.BATCH_ORDER_XCD)]}
+    per_mapping, best, best_ms = {}, None, None
+    for name, cs in cands.items():  # the autotuner over each mapping's candidates, the fastest kept
+        t, ms = b.autotune(cs, reps=2)
+        per_mapping[name] = round(ms, 3)
+        if best is None or ms < best_ms:
+            best, best_ms = t, ms
+    b.set_tuning(best.mapping, best.waves_per_cu, best.chunk, best.order)
     tune_s = time.time() - a0
     t_batch = _median_ms(ctx, b.run, reps)
     mx = [it["out"].download(pkg.RESULT_DTYPE) for it in st["mixed"][0]]
@@ -141,6 +150,7 @@ def batched_leg(pkg, ctx, synth, reps=3):
            "pois": total, "step": "IC-GN over all POIs of all pairs, inputs in HBM",
            "fp64_per_pair_ms": round(t_fp64, 3), "mixed_per_pair_ms": round(t_mix_pp, 3),
            "mixed_batch_ms": round(t_batch, 3), "autotuned": best.as_dict(), "autotune_s": round(tune_s, 1),
+           "autotune_best_ms_by_mapping": per_mapping,
            "converged": conv, "converged_fp64": conv64,
            "value": round(conv / (t_batch * 1e-3), 1), "fp64_value": round(conv64 / (t_fp64 * 1e-3), 1),
            "vs_fp64_drop_in": round((conv / t_batch) / (conv64 / t_fp64), 3),

@@ -83,6 +83,7 @@ def _assert_equivalent(g, r, tol):
 TUNINGS = [
     (0, 0, 1, 0), (0, 0, 1, 1), (0, 0, 1, 2), (0, 0, 1, 3),  # grid: natural, cost, interleave, xcd
     (1, 4, 1, 0), (1, 8, 4, 1), (1, 16, 16, 2),    # persistent
+    (2, 0, 1, 0), (2, 0, 1, 1), (2, 0, 1, 3),      # tile (first-order FP32 groups): natural, cost, xcd
 ]
 
 
@@ -105,7 +106,7 @@ def test_batch_equals_per_pair_launches(pkg, ctx, cases, precision):
         _report(f"  tuning {b.tuning().as_dict()}: " +
                 ("bit-identical to per-pair launches" if same else str([_diff(g, r) for g, r in zip(got, ref)])))
         for k, (g, r) in enumerate(zip(got, ref)):
-            if t[0] == 0:
+            if t[0] in (0, 2):  # grid and tile run the per-pair code: same bits
                 assert _same_bits(g, r), (t, k, _diff(g, r))
             else:
                 _assert_equivalent(g, r, 1e-9 if precision == "fp64" else 1e-5)
@@ -204,8 +205,9 @@ def test_first_order_tile_and_fallback_match_the_gather(pkg, ctx, synth):
     warped subset leaves it.  Guesses 3-5 px off the truth walk many POIs out
     of the tile.  The same POIs solved in a random order (the four POIs of a
     workgroup then lie far apart, the tile does not fit, every sample is
-    gathered) and the batch kernel (one POI per workgroup, gather only) must
-    give the same results bit for bit; edge POIs and a POI count that is not a
+    gathered), the batch kernel (one POI per workgroup, gather only) and the
+    batch tile mapping (four items per workgroup) must give the same results
+    bit for bit; edge POIs and a POI count that is not a
     multiple of four are included."""
     c = synth.batched_pairs(n_pairs=1, sizes=(512,), seed0=3300)[0]
     rng = np.random.default_rng(7)
@@ -233,16 +235,21 @@ def test_first_order_tile_and_fallback_match_the_gather(pkg, ctx, synth):
         b.set_tuning(0, 0, 1, 3)  # grid mapping, XCD order
         b.run()
         batched = out.download(pkg.RESULT_DTYPE)
+        out.upload(np.zeros(n, pkg.RESULT_DTYPE))
+        b.set_tuning(2, 0, 1, 0)  # tile mapping
+        b.run()
+        batched_tile = out.download(pkg.RESULT_DTYPE)
         b.destroy()
-        res = np.zeros(n, pkg.RESULT_DTYPE), np.zeros(n, pkg.RESULT_DTYPE)
-        res[0][order], res[1][order] = per_pair, batched
+        res = np.zeros(n, pkg.RESULT_DTYPE), np.zeros(n, pkg.RESULT_DTYPE), np.zeros(n, pkg.RESULT_DTYPE)
+        res[0][order], res[1][order], res[2][order] = per_pair, batched, batched_tile
         return res
 
-    tiled_pp, tiled_b = solve(np.arange(n))
-    gathered_pp, gathered_b = solve(perm)
+    tiled_pp, tiled_b, tiled_bt = solve(np.arange(n))
+    gathered_pp, gathered_b, gathered_bt = solve(perm)
     moved = int((np.hypot(tiled_pp["p"][:, 0] - init["u"], tiled_pp["p"][:, 1] - init["v"]) > 3.0).sum())
     _report(f"tile/fallback: {n} POIs, {far.sum()} guesses 3-5 px off, {moved} moved > 3 px, "
             f"{int((tiled_pp['status'] == 0).sum())} Success")
     assert moved > 0.2 * n
     assert tiled_pp.tobytes() == gathered_pp.tobytes()
     assert tiled_b.tobytes() == gathered_b.tobytes() == tiled_pp.tobytes()
+    assert tiled_bt.tobytes() == gathered_bt.tobytes() == tiled_pp.tobytes()

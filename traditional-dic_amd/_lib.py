@@ -74,7 +74,7 @@ class IcgnParams(ctypes.Structure):
     ]
 
 
-BATCH_GRID, BATCH_PERSISTENT = 0, 1
+BATCH_GRID, BATCH_PERSISTENT, BATCH_TILE = 0, 1, 2
 BATCH_ORDER_NATURAL, BATCH_ORDER_COST, BATCH_ORDER_INTERLEAVE, BATCH_ORDER_XCD = 0, 1, 2, 3
 
 
@@ -104,7 +104,7 @@ class BatchTuning(ctypes.Structure):
     ]
 
     def as_dict(self):
-        return {"mapping": ("grid", "persistent")[self.mapping], "waves_per_cu": self.waves_per_cu,
+        return {"mapping": ("grid", "persistent", "tile")[self.mapping], "waves_per_cu": self.waves_per_cu,
                 "chunk": self.chunk, "order": ("natural", "cost", "interleave", "xcd")[self.order]}
 
 

@@ -24,7 +24,7 @@ struct dic_batch {
     int64_t total = 0;
     int max_nfull = 0;
     dic_batch_tuning tuning{DIC_BATCH_GRID, 8, 1, DIC_BATCH_ORDER_COST};
-    int items_order = -1;          // order the device item table holds
+    int items_key = -1;            // order (and tile layout) the device item table holds
     std::vector<BatchGroup> groups; // launch groups of the current item table
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -46,8 +46,8 @@ void drop_graph(dic_batch *b)
 
 int valid_tuning(dic_ctx *ctx, const dic_batch_tuning &t)
 {
-    if (t.mapping != DIC_BATCH_GRID && t.mapping != DIC_BATCH_PERSISTENT)
-        return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "batch mapping must be GRID or PERSISTENT");
+    if (t.mapping != DIC_BATCH_GRID && t.mapping != DIC_BATCH_PERSISTENT && t.mapping != DIC_BATCH_TILE)
+        return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "batch mapping must be GRID, PERSISTENT or TILE");
     if (t.order < DIC_BATCH_ORDER_NATURAL || t.order > DIC_BATCH_ORDER_XCD)
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "unknown batch order");
     if (t.mapping == DIC_BATCH_PERSISTENT && (t.waves_per_cu < 1 || t.waves_per_cu > 64 || t.chunk < 1 ||
@@ -75,8 +75,17 @@ std::vector<std::vector<int>> make_groups(const dic_batch *b)
     return g;
 }
 
-// (descriptor, POI) list, group after group, each group in the requested order
-std::vector<int2> make_items(const dic_batch *b, int order, std::vector<BatchGroup> &groups)
+// Whether group G runs the tile kernel under the TILE mapping (first order, FP32).
+bool tile_group(const dic_batch *b, int shape_order, int mapping)
+{
+    return mapping == DIC_BATCH_TILE && shape_order == 1 && b->precision != DIC_PRECISION_FP64;
+}
+
+// (descriptor, POI) list, group after group, each group in the requested
+// order; a tile group keeps each descriptor's POIs in their natural order
+// (four consecutive items are neighbouring POIs; the kernel deals the quads
+// over the XCDs) and ranks the descriptors by cost under ORDER_COST only
+std::vector<int2> make_items(const dic_batch *b, int order, int mapping, std::vector<BatchGroup> &groups)
 {
     const int np = (int)b->pairs.size();
     std::vector<int2> items;
@@ -86,6 +95,15 @@ std::vector<int2> make_items(const dic_batch *b, int order, std::vector<BatchGro
         const size_t start = items.size();
         BatchGroup G{b->pairs[ids[0]].shape_order, 0, (int)start, 0};
         for (int p : ids) G.max_nfull = std::max(G.max_nfull, b->pairs[p].a.nfull);
+        if (tile_group(b, G.shape_order, mapping)) {
+            if (order == DIC_BATCH_ORDER_COST)
+                std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) { return b->cost[x] > b->cost[y]; });
+            for (int p : ids)
+                for (int i = 0; i < b->pairs[p].a.n; ++i) items.push_back(make_int2(p, i));
+            G.count = (int)(items.size() - start);
+            if (G.count > 0) groups.push_back(G);
+            continue;
+        }
         if (order == DIC_BATCH_ORDER_COST)
             std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) { return b->cost[x] > b->cost[y]; });
         if (order == DIC_BATCH_ORDER_INTERLEAVE) {
@@ -132,8 +150,9 @@ std::vector<int2> make_items(const dic_batch *b, int order, std::vector<BatchGro
 int prepare(dic_batch *b)
 {
     dic_ctx *ctx = b->ctx;
-    if (b->items_order != b->tuning.order) {
-        const std::vector<int2> items = make_items(b, b->tuning.order, b->groups);
+    const int key = b->tuning.order * 4 + b->tuning.mapping;
+    if (b->items_key != key) {
+        const std::vector<int2> items = make_items(b, b->tuning.order, b->tuning.mapping, b->groups);
         if (b->groups.size() > (size_t)kMaxBatchGroups)
             return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "too many launch groups in one batch");
         if (!items.empty()) {
@@ -143,7 +162,7 @@ int prepare(dic_batch *b)
             if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch item upload");
             if (rc != DIC_OK) return rc;
         }
-        b->items_order = b->tuning.order;
+        b->items_key = key;
     }
     if (b->exec) return DIC_OK;
     BatchLaunch L;
@@ -155,6 +174,7 @@ int prepare(dic_batch *b)
     L.groups = b->groups.data();
     L.ngroups = (int)b->groups.size();
     L.persistent = b->tuning.mapping == DIC_BATCH_PERSISTENT;
+    L.tile = b->tuning.mapping == DIC_BATCH_TILE;
     L.grid = std::max(1, b->tuning.waves_per_cu * b->cus);
     L.chunk = std::max(1, b->tuning.chunk);
     L.counter = b->d_counter;
@@ -338,6 +358,11 @@ int dic_batch_autotune(dic_batch *b, const dic_batch_tuning *candidates, int n_c
             for (int chunk : {1, 4, 16})
                 for (int order : {DIC_BATCH_ORDER_NATURAL, DIC_BATCH_ORDER_COST})
                     cand.push_back({DIC_BATCH_PERSISTENT, wpc, chunk, order});
+        bool first_fp32 = false;
+        for (const BatchPair &p : b->pairs) first_fp32 = first_fp32 || tile_group(b, p.shape_order, DIC_BATCH_TILE);
+        if (first_fp32)
+            for (int order : {DIC_BATCH_ORDER_NATURAL, DIC_BATCH_ORDER_COST, DIC_BATCH_ORDER_XCD})
+                cand.push_back({DIC_BATCH_TILE, 0, 1, order});
     }
     for (const auto &c : cand) {
         int rc = valid_tuning(ctx, c);

@@ -136,6 +136,7 @@ struct BatchLaunch {
     const BatchGroup *groups;
     int ngroups;
     bool persistent;   // work-queue kernels on a fixed grid, else one workgroup per item
+    bool tile;         // first-order FP32 groups: four items per workgroup over an LDS tile
     int grid;          // persistent workgroups
     int chunk;         // items per queue grab
     unsigned *counter; // one queue counter per group (persistent)

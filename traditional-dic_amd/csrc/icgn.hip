@@ -1351,6 +1351,42 @@ __global__ __launch_bounds__(64, (kMinWaves<P, R>)) void icgn_batch_kernel(
     }
 }
 
+// Batch tile mapping, first order FP32: workgroup q takes items 4q .. 4q+3
+// (quads dealt over the XCDs in contiguous runs, as icgn_group_kernel deals
+// its POIs), one per wave; when the four belong to one descriptor the
+// coefficient cells around their starting subsets are staged in LDS first.
+// The item table keeps each descriptor's POIs in their natural order, so the
+// four are neighbours.
+template <int OBJ, int D>
+__global__ __launch_bounds__(64 * kGroupPois, (kMinWaves<6, float>)) void icgn_batch_group_kernel(
+    const BatchPair *__restrict__ pairs, const int2 *__restrict__ items, int nitems, unsigned *, int)
+{
+    __shared__ IcgnShared sh[kGroupPois];
+    __shared__ float tile[kGroupPitch * kGroupRows];
+    const int nq = (nitems + kGroupPois - 1) / kGroupPois;
+    const int per = (nq + 7) >> 3;
+    const int q = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    if (q >= nq) return;
+    const int u0 = q * kGroupPois, ul = min(u0 + kGroupPois, nitems) - 1;
+    const int p0 = items[u0].x;
+    bool one = true;
+    for (int u = u0 + 1; u <= ul; ++u) one = one && items[u].x == p0;
+    Tile t{nullptr, 0, 0, 0, 0};
+    if (one) {
+        const IcgnArgs &a = pairs[p0].a;
+        int xlo = kBigBox, ylo = kBigBox, xhi = -kBigBox, yhi = -kBigBox;
+        for (int u = u0; u <= ul; ++u) grow_box(a, items[u].y, xlo, ylo, xhi, yhi);
+        t = stage_tile<D>(a, xlo, ylo, xhi, yhi, tile);
+    }
+    __syncthreads();
+    const int u = u0 + (int)(threadIdx.x >> 6);
+    if (u > ul) return;
+    const int2 it = items[u];
+    const int pid = __builtin_amdgcn_readfirstlane(it.x);
+    const int poi = __builtin_amdgcn_readfirstlane(it.y);
+    solve_poi<6, OBJ, D, float, false>(pairs[pid].a, poi, sh[threadIdx.x >> 6], nullptr, t);
+}
+
 // Serving ticks: descriptors own contiguous POI ranges [offsets[k],
 // offsets[k+1]); workgroup g finds its descriptor by a binary search over
 // the (npairs + 1) prefix offsets, so a tick needs no per-POI item table.
@@ -2015,24 +2051,48 @@ const void *batch_func(int order, bool persistent, int objective, int degree, in
                       : batch_func_obj<6, false>(objective, degree, precision);
 }
 
+template <int OBJ>
+const void *batch_group_func_deg(int degree)
+{
+    switch (degree) {
+    case 1: return (const void *)icgn_batch_group_kernel<OBJ, 1>;
+    case 3: return (const void *)icgn_batch_group_kernel<OBJ, 3>;
+    case 5: return (const void *)icgn_batch_group_kernel<OBJ, 5>;
+    default: return nullptr;
+    }
+}
+
 struct Launch1 { // one group's launch
     const void *func;
     size_t lds;
     const int2 *items;
     int nitems;
     unsigned *counter;
+    unsigned grid, block;
 };
 
 hipError_t batch_launch_of(const BatchLaunch &L, int g, Launch1 &o)
 {
     const BatchGroup &G = L.groups[g];
     const bool twopass = L.objective == DIC_OBJECTIVE_ZNSSD && L.precision == DIC_PRECISION_FP64;
-    o.func = batch_func(G.shape_order, L.persistent, L.objective, L.degree, L.precision);
-    if (!o.func) return hipErrorInvalidValue;
-    o.lds = twopass ? sizeof(double) * (size_t)G.max_nfull : 0;
     o.items = L.items + G.first;
     o.nitems = G.count;
     o.counter = L.counter + g;
+    if (L.tile && G.shape_order == 1 && L.precision != DIC_PRECISION_FP64) {
+        o.func = L.objective == DIC_OBJECTIVE_SSD ? batch_group_func_deg<DIC_OBJECTIVE_SSD>(L.degree)
+                                                  : batch_group_func_deg<DIC_OBJECTIVE_ZNSSD>(L.degree);
+        if (!o.func) return hipErrorInvalidValue;
+        o.lds = 0;
+        const unsigned quads = (unsigned)((G.count + kGroupPois - 1) / kGroupPois);
+        o.grid = 8u * ((quads + 7u) / 8u);
+        o.block = 64u * kGroupPois;
+        return hipSuccess;
+    }
+    o.func = batch_func(G.shape_order, L.persistent, L.objective, L.degree, L.precision);
+    if (!o.func) return hipErrorInvalidValue;
+    o.lds = twopass ? sizeof(double) * (size_t)G.max_nfull : 0;
+    o.grid = L.persistent ? (unsigned)L.grid : (unsigned)G.count;
+    o.block = 64u;
     if (o.lds > 48 * 1024) return hipFuncSetAttribute(o.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)o.lds);
     return hipSuccess;
 }
@@ -2056,8 +2116,7 @@ hipError_t launch_batch(const BatchLaunch &L, hipStream_t s)
         int nitems = l.nitems, chunk = L.chunk;
         unsigned *counter = l.counter;
         void *args[] = {(void *)&pairs, (void *)&items, (void *)&nitems, (void *)&counter, (void *)&chunk};
-        const unsigned grid = L.persistent ? (unsigned)L.grid : (unsigned)nitems;
-        e = hipLaunchKernel(l.func, dim3(grid), dim3(64), args, l.lds, s);
+        e = hipLaunchKernel(l.func, dim3(l.grid), dim3(l.block), args, l.lds, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -2094,8 +2153,8 @@ hipError_t add_batch_nodes(const BatchLaunch &L, hipGraph_t g)
         void *kargs[] = {(void *)&pairs, (void *)&items, (void *)&nitems, (void *)&counter, (void *)&chunk};
         hipKernelNodeParams kp{};
         kp.func = const_cast<void *>(l.func);
-        kp.gridDim = dim3(L.persistent ? (unsigned)L.grid : (unsigned)nitems);
-        kp.blockDim = dim3(64);
+        kp.gridDim = dim3(l.grid);
+        kp.blockDim = dim3(l.block);
         kp.sharedMemBytes = (unsigned)l.lds;
         kp.kernelParams = kargs;
         kp.extra = nullptr;
