@@ -283,6 +283,32 @@ def test_device_entry_matches_host_entry(pkg, ctx, synth):
         assert dev.tobytes() == host.tobytes()
 
 
+@pytest.mark.parametrize("step", [3, 16])
+def test_first_order_mixed_launch_shapes_agree(pkg, ctx, synth, step):
+    """First-order mixed solves run four POIs per workgroup over an LDS tile or
+    one POI per workgroup: the device-pointer entry chooses by the radius
+    alone, the host-pointer entry by whether four consecutive POIs fit one tile
+    (on a 3 px grid they do, on a 16 px grid they do not).  Both shapes give
+    the same bits."""
+    wl = synth.plain_512()
+    xy = synth.poi_grid(512, 512, step)
+    xy = np.ascontiguousarray(xy[synth.solvable_mask(xy, 512, 512, 15)])
+    u, v, _ = wl.field.displacement(xy[:, 0], xy[:, 1])
+    ini = pkg.initial_guesses(np.round(u), np.round(v))
+    s = _solver(pkg, ctx, 15, 1, "znssd", 3, precision="mixed")
+    host = s.solve_batch(wl.reference, wl.deformed, xy, ini)
+    pair = s.prepare(wl.reference, wl.deformed)
+    dxy, dini = ctx.to_device(xy), ctx.to_device(ini)
+    dout = ctx.alloc(xy.shape[0] * pkg.RESULT_DTYPE.itemsize)
+    params = pkg.make_params(s.config, pkg.PRECISION_MIXED)
+    ctx.solve_device(params, pair.ref, pair.grad, pair.coeffs, dxy.ptr, dini.ptr, xy.shape[0], dout.ptr)
+    ctx.synchronize()
+    dev = dout.download(pkg.RESULT_DTYPE)
+    _report(f"step {step}: {xy.shape[0]} POIs, {int((host['status'] == 0).sum())} Success")
+    assert (host["status"] == 0).mean() > 0.95
+    assert dev.tobytes() == host.tobytes()
+
+
 def test_large_field_2048_fp64_and_mixed(pkg, ctx, orc, synth):
     """2048^2 float32 speckle, sinusoidal field, noise sigma 2, radius 20, second
     order, quintic, step 5 (161604 POIs): FP64 vs oracle on a bounded sample,

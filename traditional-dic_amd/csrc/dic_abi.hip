@@ -586,9 +586,18 @@ int validate_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image 
     return DIC_OK;
 }
 
+int host_tile_hint(const dic_icgn_params *params, const dic_coeffs *def, const double *xy, const dic_initial *init,
+                   int n)
+{
+    const bool candidate = params->shape_order == 1 && params->precision != DIC_PRECISION_FP64 && !params->masked &&
+                           params->optimizer == DIC_OPTIMIZER_ICGN && grouped_radius(params->subset_radius);
+    if (!candidate) return 0;
+    return tile_fit_share(xy, init, n, params->subset_radius, def->degree) >= 0.5 ? 1 : 0;
+}
+
 int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
                      const dic_coeffs *def, const uint8_t *roi_dev, const double *xy, const dic_initial *init, int n,
-                     dic_result *out, hipStream_t s)
+                     dic_result *out, hipStream_t s, int tile_hint)
 {
     const int *table = nullptr;
     int count = 0;
@@ -622,6 +631,7 @@ int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref,
     L.init = init;
     L.out = out;
     L.n = n;
+    L.tile_hint = tile_hint;
     return check_hip(ctx, launch_icgn(L, s), "icgn launch");
 }
 
@@ -674,7 +684,8 @@ int dic_icgn_solve_batch(dic_ctx *ctx, const dic_icgn_params *params, const dic_
     if (rc == DIC_OK) rc = check_hip(ctx, hipMemcpyAsync(din, init, bin, hipMemcpyHostToDevice, s), "init upload");
     if (rc == DIC_OK && droi)
         rc = check_hip(ctx, hipMemcpyAsync(droi, roi_host, broi, hipMemcpyHostToDevice, s), "roi upload");
-    if (rc == DIC_OK) rc = run_solve(ctx, params, ref, grad, def, droi, dxy, din, n, dout, s);
+    if (rc == DIC_OK) rc = run_solve(ctx, params, ref, grad, def, droi, dxy, din, n, dout, s,
+                                     host_tile_hint(params, def, poi_xy, init, n));
     if (rc == DIC_OK) rc = check_hip(ctx, hipMemcpyAsync(out, dout, bout, hipMemcpyDeviceToHost, s), "result download");
     if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(s), "solve sync");
     return rc;

@@ -66,7 +66,7 @@ int validate_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image 
                    const dic_coeffs *def, int n);
 int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
               const dic_coeffs *def, const uint8_t *roi_dev, const double *xy, const dic_initial *init, int n,
-              dic_result *out, hipStream_t s);
+              dic_result *out, hipStream_t s, int tile_hint = -1);
 
 // LDS of one solve workgroup: the static per-POI block (icgn.hip) plus the
 // per-sample buffer of FP64 two-pass ZNSSD IC-GN (one value) or of FGN (value
@@ -172,7 +172,19 @@ struct IcgnLaunch {
     const dic_initial *init;
     dic_result *out;
     int n;
+    // first-order FP32 launch shape: 1 four POIs per workgroup over an LDS
+    // tile, 0 one POI per workgroup, -1 decide by the radius (grouped_radius)
+    int tile_hint = -1;
 };
 hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s);
+
+// Share of the quads of consecutive POIs (POI positions and guesses on the
+// host) whose starting subsets fit one LDS tile; the host-pointer entry points
+// pass tile_hint = share >= 1/2.
+double tile_fit_share(const double *xy, const dic_initial *init, int n, int radius, int degree);
+// tile_hint of a solve whose POIs and guesses are on the host (0 where the
+// tile kernel cannot run: second order, FP64, masked, FGN, radius > 24)
+int host_tile_hint(const dic_icgn_params *params, const dic_coeffs *def, const double *xy, const dic_initial *init,
+                   int n);
 
 } // namespace dic

@@ -1928,11 +1928,11 @@ hipError_t launch_fgn_variant(const IcgnArgs &args, hipStream_t s)
 }
 
 template <int P, int OBJ, int D, typename R, bool MASKED>
-hipError_t launch_variant(const IcgnArgs &args, hipStream_t s)
+hipError_t launch_variant(const IcgnArgs &args, bool group, hipStream_t s)
 {
     if (args.fgn) return launch_fgn_variant<P, OBJ, D, R, MASKED>(args, s);
     constexpr bool TWOPASS = OBJ == DIC_OBJECTIVE_ZNSSD && sizeof(R) == 8;
-    if (P == 6 && sizeof(R) == 4 && !MASKED && grouped_radius(args.radius)) {
+    if (P == 6 && sizeof(R) == 4 && !MASKED && group) {
         const int groups = (args.n + kGroupPois - 1) / kGroupPois;
         hipLaunchKernelGGL((icgn_group_kernel<OBJ, D>), dim3(8 * ((groups + 7) / 8)), dim3(64 * kGroupPois), 0, s,
                            args);
@@ -1949,25 +1949,26 @@ hipError_t launch_variant(const IcgnArgs &args, hipStream_t s)
 }
 
 template <int P, int OBJ, int D, typename R>
-hipError_t dispatch_masked(const IcgnArgs &args, bool masked, hipStream_t s)
+hipError_t dispatch_masked(const IcgnArgs &args, bool masked, bool group, hipStream_t s)
 {
-    return masked ? launch_variant<P, OBJ, D, R, true>(args, s) : launch_variant<P, OBJ, D, R, false>(args, s);
+    return masked ? launch_variant<P, OBJ, D, R, true>(args, group, s)
+                  : launch_variant<P, OBJ, D, R, false>(args, group, s);
 }
 
 template <int P, int OBJ, int D>
-hipError_t dispatch_precision(const IcgnArgs &args, int precision, bool masked, hipStream_t s)
+hipError_t dispatch_precision(const IcgnArgs &args, int precision, bool masked, bool group, hipStream_t s)
 {
-    return precision == DIC_PRECISION_FP64 ? dispatch_masked<P, OBJ, D, double>(args, masked, s)
-                                           : dispatch_masked<P, OBJ, D, float>(args, masked, s);
+    return precision == DIC_PRECISION_FP64 ? dispatch_masked<P, OBJ, D, double>(args, masked, group, s)
+                                           : dispatch_masked<P, OBJ, D, float>(args, masked, group, s);
 }
 
 template <int P, int OBJ>
-hipError_t dispatch_degree(const IcgnArgs &args, int degree, int precision, bool masked, hipStream_t s)
+hipError_t dispatch_degree(const IcgnArgs &args, int degree, int precision, bool masked, bool group, hipStream_t s)
 {
     switch (degree) {
-    case 1: return dispatch_precision<P, OBJ, 1>(args, precision, masked, s);
-    case 3: return dispatch_precision<P, OBJ, 3>(args, precision, masked, s);
-    case 5: return dispatch_precision<P, OBJ, 5>(args, precision, masked, s);
+    case 1: return dispatch_precision<P, OBJ, 1>(args, precision, masked, group, s);
+    case 3: return dispatch_precision<P, OBJ, 3>(args, precision, masked, group, s);
+    case 5: return dispatch_precision<P, OBJ, 5>(args, precision, masked, group, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1975,6 +1976,30 @@ hipError_t dispatch_degree(const IcgnArgs &args, int degree, int precision, bool
 } // namespace
 
 size_t solve_static_lds() { return sizeof(IcgnShared); }
+
+double tile_fit_share(const double *xy, const dic_initial *init, int n, int radius, int degree)
+{
+    // the box of grow_box over each quad, and stage_tile's size test (before
+    // its clamp to the image, which can only shrink the box)
+    const int quads = (n + kGroupPois - 1) / kGroupPois, e = radius + radius / 10 + 3, ns = degree + 1;
+    if (quads == 0) return 0.0;
+    int fit = 0;
+    for (int q = 0; q < quads; ++q) {
+        int xlo = kBigBox, ylo = kBigBox, xhi = -kBigBox, yhi = -kBigBox;
+        for (int k = q * kGroupPois; k < std::min(n, (q + 1) * kGroupPois); ++k) {
+            if (!(std::fabs(init[k].u) < 1e6 && std::fabs(init[k].v) < 1e6)) continue;
+            if (!(std::fabs(xy[2 * k]) < 1e9 && std::fabs(xy[2 * k + 1]) < 1e9)) continue;
+            const int x = (int)std::lround(xy[2 * k]) + (int)std::floor(init[k].u);
+            const int y = (int)std::lround(xy[2 * k + 1]) + (int)std::floor(init[k].v);
+            xlo = std::min(xlo, x - e);
+            xhi = std::max(xhi, x + e);
+            ylo = std::min(ylo, y - e);
+            yhi = std::max(yhi, y + e);
+        }
+        if (xhi >= xlo && yhi >= ylo && xhi - xlo + ns <= kGroupPitch && yhi - ylo + ns <= kGroupRows) ++fit;
+    }
+    return (double)fit / (double)quads;
+}
 
 hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s)
 {
@@ -2011,11 +2036,15 @@ hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s)
     a.n = L.n;
     const bool masked = pr.masked != 0;
     const bool ssd = pr.objective == DIC_OBJECTIVE_SSD;
+    // first-order FP32: four POIs per workgroup over an LDS tile where the
+    // caller's POI layout lets four neighbours share one (tile_hint), else,
+    // with no layout known, wherever the radius allows it
+    const bool group = L.tile_hint < 0 ? grouped_radius(pr.subset_radius) : L.tile_hint != 0;
     if (P == 6)
-        return ssd ? dispatch_degree<6, DIC_OBJECTIVE_SSD>(a, L.degree, L.precision, masked, s)
-                   : dispatch_degree<6, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, s);
-    return ssd ? dispatch_degree<12, DIC_OBJECTIVE_SSD>(a, L.degree, L.precision, masked, s)
-               : dispatch_degree<12, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, s);
+        return ssd ? dispatch_degree<6, DIC_OBJECTIVE_SSD>(a, L.degree, L.precision, masked, group, s)
+                   : dispatch_degree<6, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, group, s);
+    return ssd ? dispatch_degree<12, DIC_OBJECTIVE_SSD>(a, L.degree, L.precision, masked, group, s)
+               : dispatch_degree<12, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, group, s);
 }
 
 

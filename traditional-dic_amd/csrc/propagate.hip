@@ -60,7 +60,7 @@ int solve_host(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref
         rc = check_hip(ctx, hipMemcpyAsync(d.init, ini.data(), sizeof(dic_initial) * n, hipMemcpyHostToDevice, s),
                        "guess upload");
     if (rc == DIC_OK) rc = run_solve(ctx, params, ref, grad, def, params->masked ? d.roi : nullptr, d.xy, d.init, n,
-                                     d.out, s);
+                                     d.out, s, host_tile_hint(params, def, xy.data(), ini.data(), n));
     if (rc == DIC_OK)
         rc = check_hip(ctx, hipMemcpyAsync(res.data(), d.out, sizeof(dic_result) * n, hipMemcpyDeviceToHost, s),
                        "result download");
