@@ -91,6 +91,38 @@ def test_gpu_propagation_matches_oracle_window(pkg, ctx, orc, stereo):
     assert (d > 1e-5).sum() <= 0.03 * d.size and d.max() <= 1e-4
 
 
+@pytest.mark.parametrize("method,order,criterion", [("icgn", 2, "ssd"), ("forward_gauss_newton", 1, "znssd")])
+def test_gpu_propagation_other_solvers_match_oracle_window(pkg, ctx, orc, stereo, method, order, criterion):
+    """The walk over the other solver variants the reference's factory can
+    build (reliability_propagation.cpp:191-196): second-order SSD IC-GN and
+    first-order ZNSSD FGN, radius 21, on the same window and seed as the
+    oracle's best-first walk.  The acceptance limit (max ZNSSD 2.0) accepts
+    every solved point, so the valid sets must agree; values may come from
+    other parents than in the oracle's walk."""
+    ref, dfm, roi, seed_row, _ = crop_case(stereo, "left_temporal")
+    pre = orc.Precomputed(dfm, degree=5)
+    gx, gy = orc.reference_gradient_cache(ref)
+    cpu, ncpu = orc.propagate(ref, pre, gx, gy, roi, seed_row, radius=21, max_iterations=50, threshold=1e-6,
+                              shape_order=order, objective=1 if criterion == "ssd" else 0,
+                              optimizer=1 if method != "icgn" else 0, spacing=3, max_znssd=2.0)
+    s = seed_row[0]
+    seed = _seed(pkg, s[0], s[1], s[2], s[3], s[4:8], s[8])
+    cfg = pkg.config_from_dict({
+        "subset": {"radius": 21}, "shape_function": {"order": order},
+        "optimization": {"method": method, "max_iterations": 50, "convergence_threshold": 1e-6},
+        "correlation": {"criterion": criterion}, "interpolation": {"degree": 5},
+        "reliability_propagation": {"spacing": 3, "max_znssd": 2.0}})
+    solver = pkg.make_solver(cfg, "fp64", context=ctx)
+    _, gpu, ngpu = pkg.propagate(ref, dfm, roi, seed, solver=solver)
+    v = cpu["valid"] == 1
+    d = np.abs(gpu["p"][v][:, :2] - cpu["p"][v][:, :2]).max(axis=1) if v.any() else np.zeros(1)
+    _report(f"{method} order {order} {criterion}: gpu {ngpu} / oracle {ncpu} points, valid {int(v.sum())}, "
+            f"max |du| {d.max():.2e}, outside 1e-5 {(d > 1e-5).sum()}")
+    assert ngpu == ncpu and v.sum() > 1000
+    assert np.array_equal(gpu["valid"], cpu["valid"])
+    assert (d > 1e-5).sum() <= 0.03 * d.size and d.max() <= 1e-4
+
+
 def test_gpu_propagation_strict_threshold_and_bad_seed(pkg, ctx, stereo):
     rows = stereo["field_left_temporal"]
     ref, dfm = read_gray(stereo["img_00_L"]), read_gray(stereo["img_04_L"])
