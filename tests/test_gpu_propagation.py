@@ -123,6 +123,23 @@ def test_gpu_propagation_other_solvers_match_oracle_window(pkg, ctx, orc, stereo
     assert (d > 1e-5).sum() <= 0.03 * d.size and d.max() <= 1e-4
 
 
+def test_gpu_propagation_mixed_precision_follows_fp64(pkg, ctx, stereo):
+    """The golden configuration walked in mixed precision: the same valid set as
+    the FP64 walk from the same seed and u, v within 1e-3 px of it."""
+    ref, dfm, roi, seed_row, _ = crop_case(stereo, "left_temporal")
+    s = seed_row[0]
+    seed = _seed(pkg, s[0], s[1], s[2], s[3], s[4:8], s[8])
+    out = {}
+    for prec in ("fp64", "mixed"):
+        solver = pkg.ICGNSolver(_cfg(pkg), precision=prec, context=ctx)
+        _, out[prec], n = pkg.propagate(ref, dfm, roi, seed, solver=solver)
+    v = out["fp64"]["valid"] == 1
+    d = np.abs(out["mixed"]["p"][v][:, :2] - out["fp64"]["p"][v][:, :2]).max(axis=1)
+    _report(f"mixed vs fp64 walk: {int(v.sum())} valid, max |du| {d.max():.2e}")
+    assert v.sum() > 1000 and np.array_equal(out["mixed"]["valid"], out["fp64"]["valid"])
+    assert d.max() <= 1e-3
+
+
 def test_gpu_propagation_strict_threshold_and_bad_seed(pkg, ctx, stereo):
     rows = stereo["field_left_temporal"]
     ref, dfm = read_gray(stereo["img_00_L"]), read_gray(stereo["img_04_L"])
