@@ -43,6 +43,10 @@ __device__ __forceinline__ int wrap(int k, int L)
     k %= L;
     return k < 0 ? k + L : k;
 }
+// the next and the previous index of a period-L sequence (one compare, no
+// division per step of the recursive filters)
+__device__ __forceinline__ int wrap_next(int kk, int L) { return kk + 1 == L ? 0 : kk + 1; }
+__device__ __forceinline__ int wrap_prev(int kk, int L) { return kk == 0 ? L - 1 : kk - 1; }
 
 struct PoleArgs {
     int count;
@@ -106,8 +110,7 @@ __global__ void prefilter_rows_kernel(const float *__restrict__ img, int W, int 
     const int T = poles.warmup;
     if (c0 < c1) {
         double y1 = 0.0, y2 = 0.0;
-        for (int k = c0 - T; k < c1; ++k) {
-            const int kk = wrap(k, PW);
+        for (int k = c0 - T, kk = wrap(c0 - T, PW); k < c1; ++k, kk = wrap_next(kk, PW)) {
             y1 = line[kk] + poles.z0 * y1;
             y2 = y1 + poles.z1 * y2;
             if (k >= c0) buf[kk] = (poles.count == 2) ? y2 : y1;
@@ -116,8 +119,7 @@ __global__ void prefilter_rows_kernel(const float *__restrict__ img, int W, int 
     __syncthreads();
     if (c0 < c1) {
         double w1 = 0.0, w2 = 0.0;
-        for (int k = c1 - 1 + T; k >= c0; --k) {
-            const int kk = wrap(k, PW);
+        for (int k = c1 - 1 + T, kk = wrap(c1 - 1 + T, PW); k >= c0; --k, kk = wrap_prev(kk, PW)) {
             w1 = buf[kk] + poles.z0 * w1;
             w2 = w1 + poles.z1 * w2;
             if (k < c1) line[kk] = poles.gain * ((poles.count == 2) ? w2 : w1);
@@ -137,8 +139,7 @@ __global__ void prefilter_cols_causal_kernel(const double *__restrict__ in, int 
     const int r0 = blockIdx.y * kColChunk;
     const int r1 = min(r0 + kColChunk, PH);
     double y1 = 0.0, y2 = 0.0;
-    for (int k = r0 - poles.warmup; k < r1; ++k) {
-        const int kk = wrap(k, PH);
+    for (int k = r0 - poles.warmup, kk = wrap(r0 - poles.warmup, PH); k < r1; ++k, kk = wrap_next(kk, PH)) {
         y1 = in[(size_t)kk * PW + x] + poles.z0 * y1;
         y2 = y1 + poles.z1 * y2;
         if (k >= r0) out[(size_t)kk * PW + x] = (poles.count == 2) ? y2 : y1;
@@ -154,8 +155,8 @@ __global__ void prefilter_cols_anticausal_kernel(const double *__restrict__ in, 
     const int r0 = blockIdx.y * kColChunk;
     const int r1 = min(r0 + kColChunk, PH);
     double w1 = 0.0, w2 = 0.0;
-    for (int k = r1 - 1 + poles.warmup; k >= r0; --k) {
-        const int kk = wrap(k, PH);
+    for (int k = r1 - 1 + poles.warmup, kk = wrap(r1 - 1 + poles.warmup, PH); k >= r0;
+         --k, kk = wrap_prev(kk, PH)) {
         w1 = in[(size_t)kk * PW + x] + poles.z0 * w1;
         w2 = w1 + poles.z1 * w2;
         if (k < r1) out[(size_t)kk * PW + x] = (OUT)(poles.gain * ((poles.count == 2) ? w2 : w1));
