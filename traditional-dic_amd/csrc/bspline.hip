@@ -276,7 +276,7 @@ hipError_t launch_prefilter(const float *img, int W, int H, int degree, int bord
                                 (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(prefilter_rows_kernel, dim3(PH), dim3(128), lds, s, img, W, H, b, poles, tmp_a);
+    hipLaunchKernelGGL(prefilter_rows_kernel, dim3(PH), dim3(256), lds, s, img, W, H, b, poles, tmp_a);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     dim3 grid((PW + 63) / 64, (PH + kColChunk - 1) / kColChunk);
