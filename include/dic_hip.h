@@ -182,7 +182,8 @@ int dic_icgn_solve_batch_device(dic_ctx *ctx, const dic_icgn_params *params, con
  * reference, which solves pair after pair and POI after POI (the callers
  * ReliabilityPropagation::propagate, reliability_propagation.cpp:233-354, and
  * the per-pair workflow loop); each POI's arithmetic is the same as
- * dic_icgn_solve_batch_device's, bit for bit.  All coefficient images of a
+ * dic_icgn_solve_batch_device's (with the item's optimizer), bit for bit;
+ * FGN items run one POI per workgroup whatever the mapping.  All coefficient images of a
  * batch share one B-spline degree and one precision; every array of an item
  * is a device pointer.  The item arrays must stay valid while the batch is
  * used; a batch is replayed with dic_batch_run as often as wanted (the inputs
@@ -199,6 +200,8 @@ typedef struct dic_batch_item {
     int32_t shape_order;          /* 1 or 2 */
     int32_t max_iterations;
     double convergence_threshold;
+    int32_t optimizer;            /* DIC_OPTIMIZER_*: IC-GN (gradient required) or FGN (gradient unused) */
+    int32_t reserved;             /* 0 */
 } dic_batch_item;
 
 enum { DIC_BATCH_GRID = 0, DIC_BATCH_PERSISTENT = 1, DIC_BATCH_TILE = 2 };

@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from conftest import ROOT
 
 
@@ -31,6 +33,45 @@ def test_struct_layouts(pkg):
     assert ctypes.sizeof(pkg._lib.SeedParams) == 144
     assert pkg._lib.ESTIMATE_DTYPE.itemsize == 80
     assert pkg._lib.SEED_EVAL_DTYPE.itemsize == 176
+
+
+def test_ctypes_mirrors_match_the_header(pkg, tmp_path):
+    """Every struct of include/dic_hip.h, compiled as plain C by gcc: its size
+    (and the offset of each field the Python layer fills) equals the ctypes or
+    numpy mirror's."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not found")
+    lib = pkg._lib
+    mirrors = {
+        "dic_icgn_params": lib.IcgnParams, "dic_batch_item": lib.BatchItem, "dic_batch_tuning": lib.BatchTuning,
+        "dic_server_config": lib.ServerConfig, "dic_request": lib.Request, "dic_server_stats": lib.ServerStats,
+        "dic_propagation_params": lib.PropagationParams, "dic_integer_search_params": lib.IntegerSearchParams,
+        "dic_seed_params": lib.SeedParams,
+    }
+    dtypes = {"dic_initial": lib.INITIAL_DTYPE, "dic_result": lib.RESULT_DTYPE, "dic_seed": lib.SEED_DTYPE,
+              "dic_estimate": lib.ESTIMATE_DTYPE, "dic_seed_evaluation": lib.SEED_EVAL_DTYPE}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dic_hip.h"', "int main(void) {"]
+    for name in list(mirrors) + list(dtypes):
+        lines.append(f'    printf("{name} %zu\\n", sizeof({name}));')
+    for name, st in mirrors.items():
+        for field, _ in st._fields_:
+            lines.append(f'    printf("{name}.{field} %zu\\n", offsetof({name}, {field}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-std=c99", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                              text=True).stdout.splitlines())
+    for name, st in mirrors.items():
+        assert int(got[name]) == ctypes.sizeof(st), name
+        for field, _ in st._fields_:
+            assert int(got[f"{name}.{field}"]) == getattr(st, field).offset, (name, field)
+    for name, dt in dtypes.items():
+        assert int(got[name]) == dt.itemsize, name
 
 
 def test_version_string(pkg):

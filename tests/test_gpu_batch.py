@@ -49,7 +49,8 @@ def _per_pair(pkg, ctx, prec, items):
     res = []
     for it in items:
         cfg = pkg.SubsetConfig(subset_radius=it["subset_radius"],
-                               shape_function="second_order" if it["shape_order"] == 2 else "first_order")
+                               shape_function="second_order" if it["shape_order"] == 2 else "first_order",
+                               optimizer="forward_gauss_newton" if it.get("optimizer") == 1 else "icgn")
         params = pkg.make_params(cfg, prec)
         out = ctx.alloc(it["n"] * pkg.RESULT_DTYPE.itemsize)
         ctx.solve_device(params, it["reference"], it["gradient"], it["deformed"], it["poi_xy"].ptr,
@@ -107,6 +108,34 @@ def test_batch_equals_per_pair_launches(pkg, ctx, cases, precision):
                 ("bit-identical to per-pair launches" if same else str([_diff(g, r) for g, r in zip(got, ref)])))
         for k, (g, r) in enumerate(zip(got, ref)):
             if t[0] in (0, 2):  # grid and tile run the per-pair code: same bits
+                assert _same_bits(g, r), (t, k, _diff(g, r))
+            else:
+                _assert_equivalent(g, r, 1e-9 if precision == "fp64" else 1e-5)
+    b.destroy()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_fgn_items_equal_per_pair_launches(pkg, ctx, cases, precision):
+    """Items that name the forward Gauss-Newton optimizer (ForwardGaussNewtonSolver)
+    run in their own launch groups, one POI per workgroup with the per-sample
+    LDS buffer; beside IC-GN items, under every mapping, each item gives its
+    per-pair launch's results bit for bit."""
+    prec, items, keep = _setup(pkg, ctx, cases, precision)
+    for k, it in enumerate(items):
+        if k % 2 == 0:
+            it["optimizer"] = pkg._lib.OPTIMIZER_FGN
+    ref = _per_pair(pkg, ctx, prec, items)
+    b = ctx.batch(items)
+    for t in [(0, 0, 1, 3), (1, 8, 4, 1), (2, 0, 1, 0)]:
+        for it in items:
+            it["out"].upload(np.zeros(it["n"], pkg.RESULT_DTYPE))
+        b.set_tuning(*t)
+        b.run()
+        got = _batch_results(pkg, items)
+        _report(f"fgn items {precision}, tuning {b.tuning().as_dict()}: "
+                f"{sum(int((g['status'] == 0).sum()) for g in got)} Success")
+        for k, (g, r) in enumerate(zip(got, ref)):
+            if items[k].get("optimizer") == 1 or t[0] in (0, 2):
                 assert _same_bits(g, r), (t, k, _diff(g, r))
             else:
                 _assert_equivalent(g, r, 1e-9 if precision == "fp64" else 1e-5)

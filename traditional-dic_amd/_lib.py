@@ -92,6 +92,8 @@ class BatchItem(ctypes.Structure):
         ("shape_order", ctypes.c_int32),
         ("max_iterations", ctypes.c_int32),
         ("convergence_threshold", ctypes.c_double),
+        ("optimizer", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -503,12 +505,12 @@ class Context:
     def batch(self, items, objective: int = OBJECTIVE_ZNSSD) -> "Batch":
         """items: sequence of dicts with keys reference, gradient, deformed (handles),
         poi_xy, init, out (DeviceBuffer), n, subset_radius, shape_order,
-        max_iterations, convergence_threshold."""
+        max_iterations, convergence_threshold, optimizer (OPTIMIZER_ICGN by default)."""
         arr = (BatchItem * max(1, len(items)))()
         keep = []
         for k, it in enumerate(items):
             arr[k].reference = it["reference"].handle
-            arr[k].gradient = it["gradient"].handle
+            arr[k].gradient = it["gradient"].handle if it.get("gradient") is not None else None
             arr[k].deformed = it["deformed"].handle
             arr[k].poi_xy = it["poi_xy"].ptr
             arr[k].init = it["init"].ptr
@@ -518,6 +520,7 @@ class Context:
             arr[k].shape_order = int(it["shape_order"])
             arr[k].max_iterations = int(it.get("max_iterations", 30))
             arr[k].convergence_threshold = float(it.get("convergence_threshold", 1e-3))
+            arr[k].optimizer = int(it.get("optimizer", OPTIMIZER_ICGN))
             keep.append(it)
         h = _vp()
         self.check(self.lib.dic_batch_create(self.handle, arr, len(items), int(objective), ctypes.byref(h)))

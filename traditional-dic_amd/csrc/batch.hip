@@ -6,6 +6,7 @@
 #include <cmath>
 #include <new>
 #include <numeric>
+#include <tuple>
 #include <vector>
 
 #include "dic_internal.hpp"
@@ -57,18 +58,21 @@ int valid_tuning(dic_ctx *ctx, const dic_batch_tuning &t)
     return DIC_OK;
 }
 
-// Launch groups: descriptors of one shape order (second order first); for
-// FP64 ZNSSD also of one LDS size class (power of two of the subset's bytes),
-// so a small subset is not limited by the LDS the largest one needs.
+// Launch groups: descriptors of one optimizer and shape order (FGN, then
+// second order first); where the solve keeps per-sample values in LDS (FP64
+// ZNSSD IC-GN, FGN) also of one LDS size class (power of two of the
+// subset's bytes), so a small subset is not limited by the LDS the largest
+// one needs.
 std::vector<std::vector<int>> make_groups(const dic_batch *b)
 {
-    const bool twopass = b->objective == DIC_OBJECTIVE_ZNSSD && b->precision == DIC_PRECISION_FP64;
-    std::map<std::pair<int, int>, std::vector<int>> m; // (-order, -class) -> descriptors
+    std::map<std::tuple<int, int, int>, std::vector<int>> m; // (-fgn, -order, -class) -> descriptors
     for (int p = 0; p < (int)b->pairs.size(); ++p) {
+        const IcgnArgs &a = b->pairs[p].a;
+        const size_t bytes = solve_dynamic_lds(a.fgn != 0, b->precision, b->objective, a.nfull);
         int cls = 0;
-        if (twopass)
-            while ((size_t(1) << cls) < sizeof(double) * (size_t)b->pairs[p].a.nfull) ++cls;
-        m[{-b->pairs[p].shape_order, -cls}].push_back(p);
+        if (bytes > 0)
+            while ((size_t(1) << cls) < bytes) ++cls;
+        m[{-a.fgn, -b->pairs[p].shape_order, -cls}].push_back(p);
     }
     std::vector<std::vector<int>> g;
     for (auto &kv : m) g.push_back(kv.second);
@@ -93,9 +97,9 @@ std::vector<int2> make_items(const dic_batch *b, int order, int mapping, std::ve
     groups.clear();
     for (std::vector<int> ids : make_groups(b)) {
         const size_t start = items.size();
-        BatchGroup G{b->pairs[ids[0]].shape_order, 0, (int)start, 0};
+        BatchGroup G{b->pairs[ids[0]].shape_order, 0, (int)start, 0, b->pairs[ids[0]].a.fgn};
         for (int p : ids) G.max_nfull = std::max(G.max_nfull, b->pairs[p].a.nfull);
-        if (tile_group(b, G.shape_order, mapping)) {
+        if (!G.fgn && tile_group(b, G.shape_order, mapping)) {
             if (order == DIC_BATCH_ORDER_COST)
                 std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) { return b->cost[x] > b->cost[y]; });
             for (int p : ids)
@@ -212,7 +216,12 @@ int dic_batch_create(dic_ctx *ctx, const dic_batch_item *items, int n_items, int
                        "device attribute");
     for (int k = 0; k < n_items && rc == DIC_OK; ++k) {
         const dic_batch_item &it = items[k];
-        if (!it.reference || !it.gradient || !it.deformed || it.n < 0) {
+        if (it.optimizer != DIC_OPTIMIZER_ICGN && it.optimizer != DIC_OPTIMIZER_FGN) {
+            rc = set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "batch item " + std::to_string(k) + ": unknown optimizer");
+            break;
+        }
+        const bool fgn = it.optimizer == DIC_OPTIMIZER_FGN;
+        if (!it.reference || (!fgn && !it.gradient) || !it.deformed || it.n < 0) {
             rc = set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "batch item " + std::to_string(k) + ": null handle");
             break;
         }
@@ -233,12 +242,13 @@ int dic_batch_create(dic_ctx *ctx, const dic_batch_item *items, int n_items, int
             b->precision = c->precision;
             b->degree = c->degree;
         }
-        if (c->precision != b->precision || it.gradient->precision != b->precision || c->degree != b->degree) {
+        if (c->precision != b->precision || (it.gradient && it.gradient->precision != b->precision) ||
+            c->degree != b->degree) {
             rc = set_error(ctx, DIC_ERR_INVALID_ARGUMENT,
                            "batch item " + std::to_string(k) + ": degree or precision differs from item 0");
             break;
         }
-        if (it.gradient->width != it.reference->width || it.gradient->height != it.reference->height) {
+        if (it.gradient && (it.gradient->width != it.reference->width || it.gradient->height != it.reference->height)) {
             rc = set_error(ctx, DIC_ERR_INVALID_ARGUMENT,
                            "batch item " + std::to_string(k) + ": gradient planes do not match the reference");
             break;
@@ -246,7 +256,7 @@ int dic_batch_create(dic_ctx *ctx, const dic_batch_item *items, int n_items, int
         const int *table = nullptr;
         int count = 0;
         rc = get_subset_table(ctx, it.subset_radius, &table, &count);
-        if (rc == DIC_OK) rc = check_solve_lds(ctx, false, c->precision, objective, it.subset_radius, count);
+        if (rc == DIC_OK) rc = check_solve_lds(ctx, fgn, c->precision, objective, it.subset_radius, count);
         if (rc != DIC_OK) break;
         BatchPair bp{};
         IcgnArgs &a = bp.a;
@@ -257,8 +267,8 @@ int dic_batch_create(dic_ctx *ctx, const dic_batch_item *items, int n_items, int
         a.pitch = c->pitch;
         a.b = c->border;
         a.ref = it.reference->data;
-        a.gx = it.gradient->gx;
-        a.gy = it.gradient->gy;
+        a.gx = it.gradient ? it.gradient->gx : nullptr;
+        a.gy = it.gradient ? it.gradient->gy : nullptr;
         a.coeffs = c->data;
         a.roi = nullptr;
         a.table = table;
@@ -268,6 +278,7 @@ int dic_batch_create(dic_ctx *ctx, const dic_batch_item *items, int n_items, int
         a.max_it = it.max_iterations;
         a.thr = it.convergence_threshold;
         a.min_needed = it.shape_order == 2 ? 12 : 6;
+        a.fgn = fgn ? 1 : 0;
         a.xy = it.poi_xy;
         a.init = it.init;
         a.out = it.out;
@@ -280,9 +291,6 @@ int dic_batch_create(dic_ctx *ctx, const dic_batch_item *items, int n_items, int
     }
     if (rc == DIC_OK && b->total > 0x7fffffff)
         rc = set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "a batch holds at most 2^31 - 1 POIs");
-    const bool twopass = objective == DIC_OBJECTIVE_ZNSSD && b->precision == DIC_PRECISION_FP64;
-    if (rc == DIC_OK && twopass && (size_t)b->max_nfull * sizeof(double) > 64 * 1024 * 2)
-        rc = set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "FP64 ZNSSD batch: subset too large for LDS");
     if (rc == DIC_OK) {
         rc = check_hip(ctx, ctx_malloc(ctx, (void **)&b->d_pairs, sizeof(BatchPair) * b->pairs.size()), "batch alloc");
         if (rc == DIC_OK)
@@ -359,7 +367,8 @@ int dic_batch_autotune(dic_batch *b, const dic_batch_tuning *candidates, int n_c
                 for (int order : {DIC_BATCH_ORDER_NATURAL, DIC_BATCH_ORDER_COST})
                     cand.push_back({DIC_BATCH_PERSISTENT, wpc, chunk, order});
         bool first_fp32 = false;
-        for (const BatchPair &p : b->pairs) first_fp32 = first_fp32 || tile_group(b, p.shape_order, DIC_BATCH_TILE);
+        for (const BatchPair &p : b->pairs)
+            first_fp32 = first_fp32 || (!p.a.fgn && tile_group(b, p.shape_order, DIC_BATCH_TILE));
         if (first_fp32)
             for (int order : {DIC_BATCH_ORDER_NATURAL, DIC_BATCH_ORDER_COST, DIC_BATCH_ORDER_XCD})
                 cand.push_back({DIC_BATCH_TILE, 0, 1, order});

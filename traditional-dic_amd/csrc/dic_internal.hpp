@@ -120,12 +120,14 @@ struct BatchPair {
     int reserved;
 };
 
-// Descriptors launched together: same shape order (and, for FP64 ZNSSD, the
-// same LDS size class); their items are items[first, first + count).
+// Descriptors launched together: same optimizer and shape order (and, where
+// the solve buffers per-sample values in LDS, the same LDS size class); their
+// items are items[first, first + count).
 struct BatchGroup {
     int shape_order;
     int max_nfull;
     int first, count;
+    int fgn; // forward Gauss-Newton descriptors (one POI per workgroup, per-sample LDS buffer)
 };
 constexpr int kMaxBatchGroups = 64; // queue counters in the batch's 256-byte counter buffer
 

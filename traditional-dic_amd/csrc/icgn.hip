@@ -1914,6 +1914,23 @@ __global__ __launch_bounds__(64, (kMinWaves<P, R>)) void fgn_kernel(IcgnArgs a)
     fgn_solve_poi<P, OBJ, D, R, MASKED>(a, poi, sh, reinterpret_cast<R *>(fbuf));
 }
 
+// Batch FGN descriptors: one POI per workgroup, item after item (the
+// dispatch order the item table holds); the per-sample buffer is sized for
+// the group's largest subset.
+template <int P, int OBJ, int D, typename R>
+__global__ __launch_bounds__(64, (kMinWaves<P, R>)) void fgn_batch_kernel(const BatchPair *__restrict__ pairs,
+                                                                          const int2 *__restrict__ items, int nitems,
+                                                                          unsigned *, int)
+{
+    __shared__ IcgnShared sh;
+    extern __shared__ double fbuf[];
+    if ((int)blockIdx.x >= nitems) return;
+    const int2 it = items[blockIdx.x];
+    const int pid = __builtin_amdgcn_readfirstlane(it.x);
+    const int poi = __builtin_amdgcn_readfirstlane(it.y);
+    fgn_solve_poi<P, OBJ, D, R, false>(pairs[pid].a, poi, sh, reinterpret_cast<R *>(fbuf));
+}
+
 template <int P, int OBJ, int D, typename R, bool MASKED>
 hipError_t launch_fgn_variant(const IcgnArgs &args, hipStream_t s)
 {
@@ -2080,6 +2097,32 @@ const void *batch_func(int order, bool persistent, int objective, int degree, in
                       : batch_func_obj<6, false>(objective, degree, precision);
 }
 
+template <int P, int OBJ, int D>
+const void *fgn_batch_func_prec(int precision)
+{
+    return precision == DIC_PRECISION_FP64 ? (const void *)fgn_batch_kernel<P, OBJ, D, double>
+                                           : (const void *)fgn_batch_kernel<P, OBJ, D, float>;
+}
+template <int P, int OBJ>
+const void *fgn_batch_func_deg(int degree, int precision)
+{
+    switch (degree) {
+    case 1: return fgn_batch_func_prec<P, OBJ, 1>(precision);
+    case 3: return fgn_batch_func_prec<P, OBJ, 3>(precision);
+    case 5: return fgn_batch_func_prec<P, OBJ, 5>(precision);
+    default: return nullptr;
+    }
+}
+const void *fgn_batch_func(int order, int objective, int degree, int precision)
+{
+    const bool ssd = objective == DIC_OBJECTIVE_SSD;
+    if (order == 2)
+        return ssd ? fgn_batch_func_deg<12, DIC_OBJECTIVE_SSD>(degree, precision)
+                   : fgn_batch_func_deg<12, DIC_OBJECTIVE_ZNSSD>(degree, precision);
+    return ssd ? fgn_batch_func_deg<6, DIC_OBJECTIVE_SSD>(degree, precision)
+               : fgn_batch_func_deg<6, DIC_OBJECTIVE_ZNSSD>(degree, precision);
+}
+
 template <int OBJ>
 const void *batch_group_func_deg(int degree)
 {
@@ -2107,6 +2150,16 @@ hipError_t batch_launch_of(const BatchLaunch &L, int g, Launch1 &o)
     o.items = L.items + G.first;
     o.nitems = G.count;
     o.counter = L.counter + g;
+    if (G.fgn) { // one POI per workgroup whatever the mapping
+        o.func = fgn_batch_func(G.shape_order, L.objective, L.degree, L.precision);
+        if (!o.func) return hipErrorInvalidValue;
+        o.lds = solve_dynamic_lds(true, L.precision, L.objective, G.max_nfull);
+        o.grid = (unsigned)G.count;
+        o.block = 64u;
+        if (o.lds > 48 * 1024)
+            return hipFuncSetAttribute(o.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)o.lds);
+        return hipSuccess;
+    }
     if (L.tile && G.shape_order == 1 && L.precision != DIC_PRECISION_FP64) {
         o.func = L.objective == DIC_OBJECTIVE_SSD ? batch_group_func_deg<DIC_OBJECTIVE_SSD>(L.degree)
                                                   : batch_group_func_deg<DIC_OBJECTIVE_ZNSSD>(L.degree);
