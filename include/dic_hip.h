@@ -284,7 +284,7 @@ typedef struct dic_server_config {
     int32_t degree;               /* B-spline degree 1, 3 or 5 */
     int32_t precision;            /* DIC_PRECISION_* */
     int32_t objective;            /* DIC_OBJECTIVE_* */
-    int32_t reserved;
+    int32_t optimizer;            /* DIC_OPTIMIZER_*: IC-GN, or FGN (ForwardGaussNewtonSolver) */
 } dic_server_config;
 
 typedef struct dic_request {

@@ -29,9 +29,9 @@ def seqs(synth, pkg):
     return out
 
 
-def _direct_chain(pkg, ctx, seq, precision):
+def _direct_chain(pkg, ctx, seq, precision, optimizer="icgn"):
     ref, frames, xy, ini = seq
-    cfg = pkg.SubsetConfig(subset_radius=15)
+    cfg = pkg.SubsetConfig(subset_radius=15, optimizer=optimizer)
     params = pkg.make_params(cfg, precision)
     dref = ctx.upload_image(ref)
     grad = ctx.ref_gradient(dref, 0, 5, 3, precision)
@@ -71,6 +71,25 @@ def test_served_fp64_matches_direct_chain_and_oracle(pkg, ctx, orc, seqs):
     for f in range(len(frames)):
         u = (f + 1) * 2e-4 * (xy[:, 0] - xc)
         assert np.abs(got["p"][f, ok[f], 0] - u[ok[f]]).max() < 0.05
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_served_fgn_matches_direct_chain(pkg, ctx, seqs, precision):
+    """A server configured with the forward Gauss-Newton optimizer: each frame
+    of a request equals the chain of direct FGN solves (identical statuses,
+    |du|, |dv| <= 1e-9 px)."""
+    prec = pkg.PRECISION_FP64 if precision == "fp64" else pkg.PRECISION_MIXED
+    srv = ctx.server(radius=15, shape_order=1, degree=5, precision=prec, max_active=8, max_width=512,
+                     max_height=512, optimizer=pkg._lib.OPTIMIZER_FGN)
+    req = srv.submit(*seqs[1])
+    got = req.wait(60.0)
+    srv.close()
+    want = _direct_chain(pkg, ctx, seqs[1], prec, optimizer="forward_gauss_newton")
+    assert np.array_equal(got["status"], want["status"])
+    ok = want["status"] == 0
+    d = np.abs(got["p"][..., :2] - want["p"][..., :2])[ok].max(initial=0)
+    _report(f"served FGN {precision} vs direct chain: {ok.mean():.4f} ok, max |du| {d:.2e}")
+    assert d <= 1e-9 and ok.mean() > 0.98
 
 
 def test_alone_and_in_mix_are_identical(pkg, ctx, seqs):

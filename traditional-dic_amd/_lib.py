@@ -123,7 +123,7 @@ class ServerConfig(ctypes.Structure):
         ("degree", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("objective", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("optimizer", ctypes.c_int32),
     ]
 
 
@@ -542,10 +542,10 @@ class Context:
 
     def server(self, radius=15, shape_order=1, degree=5, precision=PRECISION_MIXED, objective=OBJECTIVE_ZNSSD,
                max_active=64, max_width=4096, max_height=4096, hbm_budget=0, max_iterations=30,
-               convergence_threshold=1e-3) -> "Server":
+               convergence_threshold=1e-3, optimizer=OPTIMIZER_ICGN) -> "Server":
         cfg = ServerConfig(int(hbm_budget), int(max_active), int(max_width), int(max_height), int(radius),
                            int(shape_order), int(max_iterations), float(convergence_threshold), int(degree),
-                           int(precision), int(objective), 0)
+                           int(precision), int(objective), int(optimizer))
         h = _vp()
         self.check(self.lib.dic_server_create(self.handle, ctypes.byref(cfg), ctypes.byref(h)))
         return Server(self, h)

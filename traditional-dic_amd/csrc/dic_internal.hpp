@@ -154,6 +154,7 @@ struct RangeLaunch {
     int npairs;
     int total;              // offsets[npairs]
     int max_nfull;
+    int fgn = 0;            // forward Gauss-Newton descriptors
 };
 hipError_t launch_range(const RangeLaunch &L, hipStream_t s);
 hipError_t launch_init_from_results(const dic_result *prev, dic_initial *init, int n, hipStream_t s);

@@ -1931,6 +1931,26 @@ __global__ __launch_bounds__(64, (kMinWaves<P, R>)) void fgn_batch_kernel(const 
     fgn_solve_poi<P, OBJ, D, R, false>(pairs[pid].a, poi, sh, reinterpret_cast<R *>(fbuf));
 }
 
+// Serving ticks with FGN: as icgn_range_kernel, one POI per workgroup, the
+// per-sample buffer sized for the server's subset.
+template <int P, int OBJ, int D, typename R>
+__global__ __launch_bounds__(64, (kMinWaves<P, R>)) void fgn_range_kernel(const BatchPair *__restrict__ pairs,
+                                                                          const int *__restrict__ offsets, int npairs)
+{
+    __shared__ IcgnShared sh;
+    extern __shared__ double fbuf[];
+    const int g = (int)blockIdx.x;
+    if (g >= offsets[npairs]) return;
+    int lo = 0, hi = npairs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (offsets[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const int pid = __builtin_amdgcn_readfirstlane(lo);
+    fgn_solve_poi<P, OBJ, D, R, false>(pairs[pid].a, g - offsets[pid], sh, reinterpret_cast<R *>(fbuf));
+}
+
 template <int P, int OBJ, int D, typename R, bool MASKED>
 hipError_t launch_fgn_variant(const IcgnArgs &args, hipStream_t s)
 {
@@ -2123,6 +2143,32 @@ const void *fgn_batch_func(int order, int objective, int degree, int precision)
                : fgn_batch_func_deg<6, DIC_OBJECTIVE_ZNSSD>(degree, precision);
 }
 
+template <int P, int OBJ, int D>
+const void *fgn_range_func_prec(int precision)
+{
+    return precision == DIC_PRECISION_FP64 ? (const void *)fgn_range_kernel<P, OBJ, D, double>
+                                           : (const void *)fgn_range_kernel<P, OBJ, D, float>;
+}
+template <int P, int OBJ>
+const void *fgn_range_func_deg(int degree, int precision)
+{
+    switch (degree) {
+    case 1: return fgn_range_func_prec<P, OBJ, 1>(precision);
+    case 3: return fgn_range_func_prec<P, OBJ, 3>(precision);
+    case 5: return fgn_range_func_prec<P, OBJ, 5>(precision);
+    default: return nullptr;
+    }
+}
+const void *fgn_range_func(int order, int objective, int degree, int precision)
+{
+    const bool ssd = objective == DIC_OBJECTIVE_SSD;
+    if (order == 2)
+        return ssd ? fgn_range_func_deg<12, DIC_OBJECTIVE_SSD>(degree, precision)
+                   : fgn_range_func_deg<12, DIC_OBJECTIVE_ZNSSD>(degree, precision);
+    return ssd ? fgn_range_func_deg<6, DIC_OBJECTIVE_SSD>(degree, precision)
+               : fgn_range_func_deg<6, DIC_OBJECTIVE_ZNSSD>(degree, precision);
+}
+
 template <int OBJ>
 const void *batch_group_func_deg(int degree)
 {
@@ -2279,6 +2325,20 @@ const void *range_func_obj(int objective, int degree, int precision)
 hipError_t launch_range(const RangeLaunch &L, hipStream_t s)
 {
     if (L.total <= 0) return hipSuccess;
+    if (L.fgn) {
+        const void *ff = fgn_range_func(L.shape_order, L.objective, L.degree, L.precision);
+        if (!ff) return hipErrorInvalidValue;
+        const size_t fl = solve_dynamic_lds(true, L.precision, L.objective, L.max_nfull);
+        if (fl > 48 * 1024) {
+            hipError_t e = hipFuncSetAttribute(ff, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fl);
+            if (e != hipSuccess) return e;
+        }
+        const BatchPair *pairs = L.pairs;
+        const int *offsets = L.offsets;
+        int npairs = L.npairs;
+        void *args[] = {(void *)&pairs, (void *)&offsets, (void *)&npairs};
+        return hipLaunchKernel(ff, dim3((unsigned)L.total), dim3(64), args, fl, s);
+    }
     const void *func = L.shape_order == 2 ? range_func_obj<12>(L.objective, L.degree, L.precision)
                                           : range_func_obj<6>(L.objective, L.degree, L.precision);
     if (!func) return hipErrorInvalidValue;

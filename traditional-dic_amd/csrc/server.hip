@@ -307,6 +307,7 @@ int build_tick(dic_server *s, Tick &t, int slot)
         a.max_it = s->cfg.max_iterations;
         a.thr = s->cfg.convergence_threshold;
         a.min_needed = s->cfg.shape_order == 2 ? 12 : 6;
+        a.fgn = s->cfg.optimizer == DIC_OPTIMIZER_FGN ? 1 : 0;
         a.xy = sl->xy;
         a.init = sl->init;
         a.out = sl->out[f & 1];
@@ -333,6 +334,7 @@ int build_tick(dic_server *s, Tick &t, int slot)
     L.npairs = np;
     L.total = total;
     L.max_nfull = s->nfull;
+    L.fgn = s->cfg.optimizer == DIC_OPTIMIZER_FGN ? 1 : 0;
     if (rc == DIC_OK) rc = hipcheck(s, launch_range(L, s->compute), "tick launch");
     if (rc == DIC_OK) rc = hipcheck(s, hipEventRecord(s->solved[slot], s->compute), "event");
     if (rc == DIC_OK) rc = hipcheck(s, hipStreamWaitEvent(s->copy, s->solved[slot], 0), "event");
@@ -483,6 +485,8 @@ int dic_server_create(dic_ctx *ctx, const dic_server_config *cfg, dic_server **o
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "server: unknown precision");
     if (c.objective != DIC_OBJECTIVE_ZNSSD && c.objective != DIC_OBJECTIVE_SSD)
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "server: unknown objective");
+    if (c.optimizer != DIC_OPTIMIZER_ICGN && c.optimizer != DIC_OPTIMIZER_FGN)
+        return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "server: unknown optimizer");
     const int b = 3;
     if (c.max_width + 2 * b > 16384 ||
         (size_t)(c.max_width + 2 * b) * (c.max_height + 2 * b) * sizeof(double) >= ((size_t)1 << 31))
@@ -494,7 +498,9 @@ int dic_server_create(dic_ctx *ctx, const dic_server_config *cfg, dic_server **o
     s->border = b;
     s->t0 = std::chrono::steady_clock::now();
     int rc = get_subset_table(ctx, c.subset_radius, &s->table, &s->nfull);
-    if (rc == DIC_OK) rc = check_solve_lds(ctx, false, c.precision, c.objective, c.subset_radius, s->nfull);
+    if (rc == DIC_OK)
+        rc = check_solve_lds(ctx, c.optimizer == DIC_OPTIMIZER_FGN, c.precision, c.objective, c.subset_radius,
+                             s->nfull);
     auto hc = [&](hipError_t e, const char *w) {
         if (rc == DIC_OK) rc = check_hip(ctx, e, w);
     };
