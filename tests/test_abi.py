@@ -74,6 +74,37 @@ def test_ctypes_mirrors_match_the_header(pkg, tmp_path):
         assert int(got[name]) == dt.itemsize, name
 
 
+def test_no_fallback_without_library_or_device():
+    """The product path fails loudly: without libdic_hip.so the loader raises,
+    and on a host without a GPU creating a context raises (no CPU path)."""
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import __graft_entry__ as g\n"
+        "pkg = g.load_package()\n"
+        "try:\n"
+        "    pkg._lib.load_library('/nonexistent/libdic_hip.so')\n"
+        "    print('loaded')\n"
+        "except ImportError as e:\n"
+        "    print('missing:', 'no CPU fallback' in str(e))\n"
+        "pkg._lib._lib = None\n"
+        "import torch\n"
+        "if not torch.cuda.is_available():\n"
+        "    try:\n"
+        "        pkg.Context(0)\n"
+        "        print('context')\n"
+        "    except pkg.DicError as e:\n"
+        "        print('nodevice:', e.code)\n"
+    ) % ROOT
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300).stdout
+    assert "missing: True" in out, out
+    assert "loaded" not in out and "context" not in out, out
+    import torch
+    if not torch.cuda.is_available():
+        assert "nodevice: 5" in out, out  # DIC_ERR_NO_DEVICE
+
+
 def test_version_string(pkg):
     assert b"gfx950" in pkg.load_library().dic_version()
 
