@@ -29,9 +29,10 @@ def seqs(synth, pkg):
     return out
 
 
-def _direct_chain(pkg, ctx, seq, precision, optimizer="icgn"):
+def _direct_chain(pkg, ctx, seq, precision, optimizer="icgn", objective="znssd", order=1):
     ref, frames, xy, ini = seq
-    cfg = pkg.SubsetConfig(subset_radius=15, optimizer=optimizer)
+    cfg = pkg.SubsetConfig(subset_radius=15, optimizer=optimizer, objective=objective,
+                           shape_function="second_order" if order == 2 else "first_order")
     params = pkg.make_params(cfg, precision)
     dref = ctx.upload_image(ref)
     grad = ctx.ref_gradient(dref, 0, 5, 3, precision)
@@ -90,6 +91,23 @@ def test_served_fgn_matches_direct_chain(pkg, ctx, seqs, precision):
     d = np.abs(got["p"][..., :2] - want["p"][..., :2])[ok].max(initial=0)
     _report(f"served FGN {precision} vs direct chain: {ok.mean():.4f} ok, max |du| {d:.2e}")
     assert d <= 1e-9 and ok.mean() > 0.98
+
+
+@pytest.mark.parametrize("objective,order", [("ssd", 1), ("znssd", 2), ("ssd", 2)])
+def test_served_other_configs_match_direct_chain(pkg, ctx, seqs, objective, order):
+    """Servers configured for the SSD criterion and the second-order shape
+    function (mixed precision): each frame equals the chain of direct solves
+    of the same configuration bit for bit."""
+    obj = pkg._lib.OBJECTIVE_SSD if objective == "ssd" else pkg._lib.OBJECTIVE_ZNSSD
+    srv = ctx.server(radius=15, shape_order=order, degree=5, precision=pkg.PRECISION_MIXED, objective=obj,
+                     max_active=8, max_width=512, max_height=512)
+    got = srv.submit(*seqs[2]).wait(60.0)
+    srv.close()
+    want = _direct_chain(pkg, ctx, seqs[2], pkg.PRECISION_MIXED, objective=objective, order=order)
+    ok = want["status"] == 0
+    _report(f"served {objective} order {order}: {ok.mean():.4f} ok, same bits {got.tobytes() == want.tobytes()}")
+    assert ok.mean() > 0.95
+    assert got.tobytes() == want.tobytes()
 
 
 def test_alone_and_in_mix_are_identical(pkg, ctx, seqs):
