@@ -142,6 +142,29 @@ def test_batch_fgn_items_equal_per_pair_launches(pkg, ctx, cases, precision):
     b.destroy()
 
 
+@pytest.mark.parametrize("degree", [1, 3])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_lower_degrees_equal_per_pair_launches(pkg, ctx, cases, precision, degree):
+    """Linear and cubic B-splines: grid and tile mappings give the per-pair
+    results bit for bit, the persistent mapping within its tolerance."""
+    prec, items, keep = _setup(pkg, ctx, cases, precision, degree=degree)
+    ref = _per_pair(pkg, ctx, prec, items)
+    b = ctx.batch(items)
+    for t in [(0, 0, 1, 3), (1, 8, 4, 1), (2, 0, 1, 0)]:
+        for it in items:
+            it["out"].upload(np.zeros(it["n"], pkg.RESULT_DTYPE))
+        b.set_tuning(*t)
+        b.run()
+        got = _batch_results(pkg, items)
+        for k, (g, r) in enumerate(zip(got, ref)):
+            if t[0] in (0, 2):
+                assert _same_bits(g, r), (degree, t, k, _diff(g, r))
+            else:
+                _assert_equivalent(g, r, 1e-9 if precision == "fp64" else 1e-5)
+    _report(f"batch degree {degree} {precision}: {sum(int((r['status'] == 0).sum()) for r in ref)} Success")
+    b.destroy()
+
+
 def test_batch_graph_replays_with_eager_work_between(pkg, ctx, cases):
     prec, items, keep = _setup(pkg, ctx, cases, "mixed")
     b = ctx.batch(items)
