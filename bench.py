@@ -645,6 +645,8 @@ def main():
         "fp64": {"value": round(float(value64), 1), "ms_per_step": round(timing["fp64"]["ms_per_step"], 4),
                  "converged": int(ok64.sum()), "roofline": roofline("fp64", fp),
                  "rms_error_vs_truth_px": round(float(err_truth), 5)},
+        "request_latency_ms": (dict(serving["request_latency_ms"], source="serving leg")
+                               if serving else None),
         "mixed_vs_fp64_speedup": round(float(value / value64), 3),
         "mixed_vs_fp64_max_px": float(np.max(dmix[ok64], initial=0.0)),
         "cpu_baseline": cpu,
