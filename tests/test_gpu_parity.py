@@ -171,6 +171,29 @@ def test_icgn_masked_parity(pkg, ctx, orc, small_pair, order):
     assert (cpu["status"] == 0).sum() > 100 and (cpu["status"] == 2).sum() > 100
 
 
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("objective", ["znssd", "ssd"])
+def test_icgn_masked_mixed_within_1e3_of_oracle(pkg, ctx, orc, small_pair, order, objective):
+    """The masked FP32 kernels (ROI-limited samples, the minimum-sample rule):
+    the oracle's statuses and valid flags, u, v within 1e-3 px of it."""
+    ref, dfm, xy, u0, v0 = small_pair
+    H, W = ref.shape
+    yy, xx = np.mgrid[0:H, 0:W]
+    roi = (((xx - 100) ** 2 + (yy - 90) ** 2) < 70 ** 2).astype(np.uint8)
+    roi[80:100, 60:75] = 0
+    ini = _ini(pkg, u0, v0, conf=0.5)
+    s = _solver(pkg, ctx, 15, order, objective, 3, precision="mixed")
+    gpu = s.solve_batch(ref, dfm, xy, ini, roi=roi)
+    cpu = _oracle(orc, ref, dfm, xy, ini, 15, order, objective, 3, roi=roi)
+    ok = cpu["status"] == 0
+    d = np.abs(gpu["p"][ok, :2] - cpu["p"][ok, :2]).max(initial=0)
+    _report(f"masked mixed order={order} {objective}: {ok.sum()} ok, max |du| {d:.2e}, "
+            f"status diff {int((gpu['status'] != cpu['status']).sum())}")
+    assert np.array_equal(gpu["status"], cpu["status"])
+    assert np.array_equal(gpu["valid"], cpu["valid"])
+    assert ok.sum() > 100 and d <= 1e-3
+
+
 def test_icgn_failure_statuses(pkg, ctx, orc, synth):
     ref = synth.quantize_unit(synth.render(128, 128, 260, 5))
     flat = np.full_like(ref, 0.5)
