@@ -145,6 +145,26 @@ def test_icgn_fp64_parity(pkg, ctx, orc, small_pair, order, objective, degree):
     assert it_same > 0.99
 
 
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("objective", ["znssd", "ssd"])
+@pytest.mark.parametrize("degree", [1, 3, 5])
+def test_icgn_mixed_within_1e3_of_oracle(pkg, ctx, orc, small_pair, order, objective, degree):
+    """Mixed precision over the same sweep as the FP64 parity test: the
+    oracle's statuses and valid flags, u, v within 1e-3 px of it."""
+    ref, dfm, xy, u0, v0 = small_pair
+    ini = _ini(pkg, u0, v0, conf=0.25)
+    thr = 1e-3 if objective == "znssd" else 1e-4
+    gpu = _solver(pkg, ctx, 15, order, objective, degree, thr=thr, precision="mixed").solve_batch(ref, dfm, xy, ini)
+    cpu = _oracle(orc, ref, dfm, xy, ini, 15, order, objective, degree, thr=thr)
+    ok = cpu["status"] == 0
+    d = np.abs(gpu["p"][ok, :2] - cpu["p"][ok, :2]).max(initial=0)
+    _report(f"mixed order={order} {objective} d={degree}: ok={ok.sum()} max |du| {d:.2e} "
+            f"status diff {int((gpu['status'] != cpu['status']).sum())}")
+    assert np.array_equal(gpu["status"], cpu["status"])
+    assert np.array_equal(gpu["valid"], cpu["valid"])
+    assert ok.sum() > 100 and d <= 1e-3
+
+
 def test_icgn_bspline_gradient_mode(pkg, ctx, orc, small_pair):
     """Standalone ICGNSolver::solve differentiates the reference B-spline at +0.5 px."""
     ref, dfm, xy, u0, v0 = small_pair
