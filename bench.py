@@ -321,7 +321,9 @@ def propagation_leg(pkg, ctx, reps=3):
     """Reliability-guided propagation over the reference's stereo-plate case
     (1440x1080, ROI, radius 41, quintic, ||dp|| < 1e-6, spacing 3, one seed),
     left_temporal pair, plain FP64: the GPU frontier walk (all 7654 ROI grid
-    points) timed end to end from resident images; the oracle's sequential
+    points) timed end to end from host images (upload, gradient cache,
+    prefilter, walk, download, as the reference's propagate builds its cache
+    and coefficients inside the call); the oracle's sequential
     best-first walk timed on a 340x320 window of the same case (2688 POIs)
     on one host thread, as the reference runs it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -342,8 +344,8 @@ def propagation_leg(pkg, ctx, reps=3):
                                                   "convergence_threshold": 1e-6},
         "interpolation": {"degree": 5}, "reliability_propagation": {"spacing": 3, "max_znssd": 2.0}})
     solver = pkg.ICGNSolver(cfg, precision="fp64", context=ctx)
-    pkg.propagate(ref, dfm, st["roi"], seed, solver=solver)  # warm (prepares the pair)
-    ts = []
+    pkg.propagate(ref, dfm, st["roi"], seed, solver=solver)  # warm (module load, first launches)
+    ts = []  # each call uploads the pair and builds its gradient cache and coefficients, as :182-183 do
     for _ in range(reps):
         t0 = time.perf_counter()
         _, grid, n = pkg.propagate(ref, dfm, st["roi"], seed, solver=solver)

@@ -198,35 +198,153 @@ private:
     const BSplineImage *img_;
 };
 
-// ICGNSolver: solve / solve_with_interpolators / solve_with_mask, plus
-// solve_batch over many POIs (the form the GPU wants).
+// dic::Mask: W x H bytes, nonzero = valid (src/core/mask.cpp:95-98).
+struct Mask {
+    int width = 0, height = 0;
+    std::vector<uint8_t> valid; // row-major, width * height
+};
+
+namespace detail {
+// What a prepared pair was built from and how: the solver checks it against
+// the images and configuration of every call that hands the pair over.
+struct PairKey {
+    const void *reference = nullptr, *deformed = nullptr;
+    int degree = 0, border = 0, exact = 0, precision = 0, gradient_mode = 0;
+};
+inline void check_pair(const PairKey &have, const PairKey &want)
+{
+    if (have.reference != want.reference || have.deformed != want.deformed)
+        throw std::invalid_argument("dic_hip: the prepared pair was built from other images than the ones passed");
+    if (have.degree != want.degree || have.border != want.border || have.exact != want.exact)
+        throw std::invalid_argument("dic_hip: the prepared pair's B-spline settings differ from the solver's");
+    if (have.precision != want.precision || have.gradient_mode != want.gradient_mode)
+        throw std::invalid_argument("dic_hip: the prepared pair's precision or gradient mode differs from the solver's");
+}
+} // namespace detail
+
+// The interpolators of one (reference, deformed) pair: the reference-gradient
+// cache and the deformed image's coefficients, resident on the device
+// (reliability_propagation.cpp:182-185 builds the same two objects).  It
+// remembers which Image objects it was built from; both must outlive it.
+class PreparedPair {
+public:
+    PreparedPair(Context &ctx, const Image &reference, const Image &deformed, const BSplinePrecomputeConfig &ip,
+                 Precision p, int gradient_mode = DIC_GRADIENT_CENTRAL)
+        : ctx_(ctx.get())
+    {
+        key_ = {&reference, &deformed, ip.degree, ip.border, ip.use_exact_prefilter ? 1 : 0, (int)p, gradient_mode};
+        check(ctx_, dic_ref_gradient(ctx_, reference.get(), gradient_mode, ip.degree, ip.border, (int)p, &grad_));
+        const int rc = dic_bspline_prefilter(ctx_, deformed.get(), ip.degree, ip.border, key_.exact, (int)p, &coeffs_);
+        if (rc != DIC_OK) {
+            dic_grad_free(grad_);
+            check(ctx_, rc);
+        }
+    }
+    ~PreparedPair()
+    {
+        dic_grad_free(grad_);
+        dic_coeffs_free(coeffs_);
+    }
+    PreparedPair(PreparedPair &&o) noexcept : ctx_(o.ctx_), key_(o.key_), grad_(o.grad_), coeffs_(o.coeffs_)
+    {
+        o.grad_ = nullptr;
+        o.coeffs_ = nullptr;
+    }
+    PreparedPair(const PreparedPair &) = delete;
+    PreparedPair &operator=(const PreparedPair &) = delete;
+    PreparedPair &operator=(PreparedPair &&) = delete;
+    const detail::PairKey &key() const { return key_; }
+    const dic_grad *gradient() const { return grad_; }
+    const dic_coeffs *coefficients() const { return coeffs_; }
+
+private:
+    dic_ctx *ctx_;
+    detail::PairKey key_;
+    dic_grad *grad_ = nullptr;
+    dic_coeffs *coeffs_ = nullptr;
+};
+
+// ICGNSolver: solve / solve_with_interpolators / solve_with_mask with the
+// reference's arguments, plus solve_batch over many POIs (the form the GPU
+// wants).  solve() prepares the pair for the call, as the reference's
+// standalone solve() builds its interpolators (icgn.cpp:363-382); the other
+// forms take a PreparedPair and throw std::invalid_argument when it was built
+// from other Image objects or settings.  Images of different sizes, or an
+// ROI of another size, give InvalidInput records with the initial u, v
+// (icgn.cpp:464-470, :648-652), as in the reference.
 class ICGNSolver {
 public:
-    explicit ICGNSolver(Context &ctx, SubsetConfig cfg = {}, Precision p = Precision::FP64)
-        : ctx_(ctx.get()), cfg_(cfg), precision_(p)
+    explicit ICGNSolver(Context &ctx, SubsetConfig cfg = {}, Precision p = Precision::FP64,
+                        int gradient_mode = DIC_GRADIENT_CENTRAL)
+        : ctx_(&ctx), cfg_(cfg), precision_(p), gradient_mode_(gradient_mode)
     {
     }
 
-    // Prepares the pair: pipeline reference gradients (central difference,
-    // reliability_propagation.cpp:84-112) and the deformed coefficient image.
-    void prepare(const Image &reference, const Image &deformed)
+    PreparedPair prepare(const Image &reference, const Image &deformed) const
     {
-        release();
-        const auto &ip = cfg_.image_precompute;
-        check(ctx_, dic_ref_gradient(ctx_, reference.get(), DIC_GRADIENT_CENTRAL, ip.degree, ip.border,
-                                     (int)precision_, &grad_));
-        check(ctx_, dic_bspline_prefilter(ctx_, deformed.get(), ip.degree, ip.border,
-                                          ip.use_exact_prefilter ? 1 : 0, (int)precision_, &coeffs_));
-        ref_ = &reference;
+        return PreparedPair(*ctx_, reference, deformed, cfg_.image_precompute, precision_, gradient_mode_);
     }
 
-    std::vector<Displacement2D> solve_batch(const std::vector<double> &poi_xy,
+    std::vector<Displacement2D> solve_batch(const Image &reference, const Image &deformed,
+                                            const std::vector<double> &poi_xy,
                                             const std::vector<InitialDisplacement> &initial,
-                                            const uint8_t *roi = nullptr) const
+                                            const PreparedPair *pair = nullptr, const Mask *roi = nullptr) const
     {
-        if (!ref_) throw std::runtime_error("dic_hip: prepare() the image pair first");
         const int n = (int)initial.size();
         if ((int)poi_xy.size() != 2 * n) throw std::invalid_argument("dic_hip: one (x, y) per initial guess");
+        if (reference.width() != deformed.width() || reference.height() != deformed.height() ||
+            (roi && (roi->width != reference.width() || roi->height != reference.height() ||
+                     roi->valid.size() != (size_t)roi->width * (size_t)roi->height)))
+            return invalid(poi_xy, initial);
+        if (!pair) {
+            const PreparedPair own = prepare(reference, deformed);
+            return run(reference, own, poi_xy, initial, roi);
+        }
+        const auto &ip = cfg_.image_precompute;
+        detail::check_pair(pair->key(), {&reference, &deformed, ip.degree, ip.border, ip.use_exact_prefilter ? 1 : 0,
+                                         (int)precision_, gradient_mode_});
+        return run(reference, *pair, poi_xy, initial, roi);
+    }
+
+    Displacement2D solve(const Image &reference, const Image &deformed, double x, double y,
+                         const InitialDisplacement &initial) const
+    {
+        return solve_batch(reference, deformed, {x, y}, {initial})[0];
+    }
+
+    Displacement2D solve_with_interpolators(const Image &reference, const Image &deformed, double x, double y,
+                                            const InitialDisplacement &initial, const PreparedPair &pair) const
+    {
+        return solve_batch(reference, deformed, {x, y}, {initial}, &pair)[0];
+    }
+
+    Displacement2D solve_with_mask(const Image &reference, const Image &deformed, const Mask &roi, double x, double y,
+                                   const InitialDisplacement &initial, const PreparedPair &pair) const
+    {
+        return solve_batch(reference, deformed, {x, y}, {initial}, &pair, &roi)[0];
+    }
+
+private:
+    static std::vector<Displacement2D> invalid(const std::vector<double> &poi_xy,
+                                               const std::vector<InitialDisplacement> &initial)
+    {
+        std::vector<Displacement2D> out(initial.size());
+        for (size_t i = 0; i < initial.size(); ++i) {
+            out[i].x = poi_xy[2 * i];
+            out[i].y = poi_xy[2 * i + 1];
+            out[i].u = initial[i].u;
+            out[i].v = initial[i].v;
+            out[i].correlation = initial[i].confidence;
+            out[i].status = SolverStatus::InvalidInput;
+            out[i].valid = false;
+        }
+        return out;
+    }
+
+    std::vector<Displacement2D> run(const Image &reference, const PreparedPair &pair, const std::vector<double> &poi_xy,
+                                    const std::vector<InitialDisplacement> &initial, const Mask *roi) const
+    {
+        const int n = (int)initial.size();
         dic_icgn_params p{};
         p.subset_radius = cfg_.subset_radius;
         p.max_iterations = cfg_.max_iterations;
@@ -243,8 +361,9 @@ public:
             ini[i] = {initial[i].u, initial[i].v, initial[i].du_dx, initial[i].du_dy,
                       initial[i].dv_dx, initial[i].dv_dy, initial[i].confidence};
         std::vector<dic_result> raw(n);
-        check(ctx_, dic_icgn_solve_batch(ctx_, &p, ref_->get(), grad_, coeffs_, roi, poi_xy.data(), ini.data(), n,
-                                         raw.data()));
+        check(ctx_->get(), dic_icgn_solve_batch(ctx_->get(), &p, reference.get(), pair.gradient(), pair.coefficients(),
+                                                roi ? roi->valid.data() : nullptr, poi_xy.data(), ini.data(), n,
+                                                raw.data()));
         std::vector<Displacement2D> out(n);
         for (int i = 0; i < n; ++i) {
             const dic_result &r = raw[i];
@@ -271,35 +390,10 @@ public:
         return out;
     }
 
-    Displacement2D solve_with_interpolators(double x, double y, const InitialDisplacement &initial) const
-    {
-        return solve_batch({x, y}, {initial})[0];
-    }
-
-    Displacement2D solve_with_mask(const uint8_t *roi, double x, double y, const InitialDisplacement &initial) const
-    {
-        return solve_batch({x, y}, {initial}, roi)[0];
-    }
-
-    ~ICGNSolver() { release(); }
-    ICGNSolver(const ICGNSolver &) = delete;
-    ICGNSolver &operator=(const ICGNSolver &) = delete;
-
-private:
-    void release()
-    {
-        dic_grad_free(grad_);
-        dic_coeffs_free(coeffs_);
-        grad_ = nullptr;
-        coeffs_ = nullptr;
-        ref_ = nullptr;
-    }
-    dic_ctx *ctx_;
+    Context *ctx_;
     SubsetConfig cfg_;
     Precision precision_;
-    const Image *ref_ = nullptr;
-    dic_grad *grad_ = nullptr;
-    dic_coeffs *coeffs_ = nullptr;
+    int gradient_mode_;
 };
 
 // ForwardGaussNewtonSolver: the same surface; the deformed image's B-spline

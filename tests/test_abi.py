@@ -3,6 +3,7 @@ include/dic_hip.h declares; struct layouts agree with the header."""
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -167,3 +168,37 @@ def test_seed_config_from_dict_mirrors_binding(pkg):
     # padding.cpp:28-36: search + max radius + border
     cfg.truncate_roi_subsets = True
     assert pkg.recommended_subset_padding(cfg) == 30 + 41 + 3
+
+
+def test_cpp_pair_guard(tmp_path):
+    """dic_hip.hpp refuses a PreparedPair handed over with other images or
+    settings (std::invalid_argument) and accepts the matching one.  Compiled
+    with plain g++; the guard runs before any device call."""
+    src = tmp_path / "guard.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include "dic_hip.hpp"
+using dic_hip::detail::PairKey;
+static int refused(const PairKey &have, const PairKey &want)
+{
+    try { dic_hip::detail::check_pair(have, want); } catch (const std::invalid_argument &) { return 1; }
+    return 0;
+}
+int main()
+{
+    int a, b, c;
+    const PairKey k{&a, &b, 5, 3, 1, 0, 0};
+    PairKey other_def = k, other_ref = k, swapped = k, degree = k, border = k, exact = k, prec = k, grad = k;
+    other_def.deformed = &c; other_ref.reference = &c; swapped.reference = &b; swapped.deformed = &a;
+    degree.degree = 3; border.border = 4; exact.exact = 0; prec.precision = 1; grad.gradient_mode = 1;
+    std::printf("%d %d %d %d %d %d %d %d %d\n", refused(k, k), refused(k, other_def), refused(k, other_ref),
+                refused(k, swapped), refused(k, degree), refused(k, border), refused(k, exact), refused(k, prec),
+                refused(k, grad));
+    return 0;
+}
+''')
+    exe = tmp_path / "guard"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    "-o", str(exe), str(src)], check=True, capture_output=True, timeout=120)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60).stdout.split()
+    assert out == ["0"] + ["1"] * 8, out

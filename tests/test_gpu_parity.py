@@ -85,6 +85,18 @@ def test_prefilter_wide_image(pkg, ctx, orc):
     assert np.abs(c.download() - orc.form_coefficients(img, 5, 3, True)).max() <= 1e-12
 
 
+def test_prefilter_refuses_rows_beyond_lds(pkg, ctx):
+    """A padded row of 10306 samples needs 164896 bytes of LDS in the row pass,
+    more than one workgroup gets on MI355X (160 KiB): the call, and a server
+    configured for such frames, are refused with the bytes needed."""
+    img = ctx.upload_image(np.zeros((4, 10300), np.float32))
+    with pytest.raises(pkg.DicError, match="needs 164896 bytes of LDS"):
+        ctx.prefilter(img, 5, 3, True, pkg.PRECISION_FP64)
+    with pytest.raises(pkg.DicError, match="needs 164896 bytes of LDS"):
+        ctx.server(max_width=10300, max_height=64)
+    ctx.prefilter(ctx.upload_image(np.zeros((4, 10200), np.float32)), 5, 3, True, pkg.PRECISION_MIXED)
+
+
 @pytest.mark.parametrize("degree", [1, 3, 5])
 def test_reference_gradients(pkg, ctx, orc, degree):
     rng = np.random.default_rng(30 + degree)
@@ -175,19 +187,20 @@ def test_icgn_bspline_gradient_mode(pkg, ctx, orc, small_pair):
 
 
 @pytest.mark.parametrize("order", [1, 2])
-def test_icgn_masked_parity(pkg, ctx, orc, small_pair, order):
+@pytest.mark.parametrize("objective", ["znssd", "ssd"])
+def test_icgn_masked_parity(pkg, ctx, orc, small_pair, order, objective):
+    """FP64 masked variants at the drop-in bar: ZNSSD (icgn.cpp:629-818,
+    :1364-1562) and SSD (:990-1154, :1741-1912), both orders."""
     ref, dfm, xy, u0, v0 = small_pair
     H, W = ref.shape
     yy, xx = np.mgrid[0:H, 0:W]
     roi = (((xx - 100) ** 2 + (yy - 90) ** 2) < 70 ** 2).astype(np.uint8)
     roi[80:100, 60:75] = 0
     ini = _ini(pkg, u0, v0, conf=0.5)
-    cfg = pkg.SubsetConfig(subset_radius=15, max_iterations=30, convergence_threshold=1e-3,
-                           shape_function="second_order" if order == 2 else "first_order")
-    cfg.image_precompute.degree = 3
-    gpu = pkg.ICGNSolver(cfg, context=ctx).solve_batch(ref, dfm, xy, ini, roi=roi)
-    cpu = _oracle(orc, ref, dfm, xy, ini, 15, order, "znssd", 3, roi=roi)
-    _assert_parity(gpu, cpu)
+    gpu = _solver(pkg, ctx, 15, order, objective, 3).solve_batch(ref, dfm, xy, ini, roi=roi)
+    cpu = _oracle(orc, ref, dfm, xy, ini, 15, order, objective, 3, roi=roi)
+    du, dv = _assert_parity(gpu, cpu)
+    _report(f"masked fp64 order={order} {objective}: {(cpu['status'] == 0).sum()} ok, max |du| {du:.2e}")
     assert (cpu["status"] == 0).sum() > 100 and (cpu["status"] == 2).sum() > 100
 
 
@@ -324,6 +337,46 @@ def test_device_entry_matches_host_entry(pkg, ctx, synth):
         ctx.synchronize()
         dev = dout.download(pkg.RESULT_DTYPE)
         assert dev.tobytes() == host.tobytes()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_one_solver_follows_every_pair(pkg, ctx, synth, precision):
+    """One solver over several deformed frames: frames built as temporaries in
+    a loop (CPython reuses the ids of freed arrays), one host buffer refilled
+    in place, and an explicitly prepared pair; each result equals a fresh
+    solver's, bit for bit, and the frames really differ."""
+    W = H = 160
+    ref = synth.quantize_unit(synth.render(W, H, 400, 23))
+    xy = synth.poi_grid(W, H, 8)
+    xy = np.ascontiguousarray(xy[synth.solvable_mask(xy, W, H, 15)])
+    shifts = [(0.4, -0.3), (1.7, 0.6), (-1.2, 2.1)]
+
+    def frame(k):
+        f = synth.affine(shifts[k][0], shifts[k][1], 1e-3, 0.0, 0.0, -1e-3, 80.0, 80.0)
+        return synth.quantize_unit(synth.render(W, H, 400, 23, f))
+
+    ini = _ini(pkg, np.zeros(xy.shape[0]), np.zeros(xy.shape[0]))
+    fresh = [_solver(pkg, ctx, 15, 1, "znssd", 5, precision=precision).solve_batch(ref, frame(k), xy, ini)
+             for k in range(3)]
+    for k in range(3):
+        u = np.median(fresh[k]["p"][fresh[k]["status"] == 0, 0])
+        assert abs(u - shifts[k][0]) < 0.05, (k, u)
+    s = _solver(pkg, ctx, 15, 1, "znssd", 5, precision=precision)
+    for k in range(3):  # temporaries
+        assert s.solve_batch(ref, frame(k), xy, ini).tobytes() == fresh[k].tobytes(), k
+    buf = np.empty((H, W), np.float32)
+    for k in (2, 0, 1):  # one buffer refilled in place
+        buf[...] = frame(k)
+        assert s.solve_batch(ref, buf, xy, ini).tobytes() == fresh[k].tobytes(), k
+    for k in (1, 2):  # an explicit prepared pair (the interpolators of solve_with_interpolators)
+        dfm = frame(k)
+        pair = s.prepare(ref, dfm)
+        try:
+            assert s.solve_batch(ref, dfm, xy, ini, pair=pair).tobytes() == fresh[k].tobytes()
+            one = s.solve_with_interpolators(ref, dfm, xy[3], ini[3], pair)
+            assert one.tobytes() == fresh[k][3].tobytes()
+        finally:
+            pair.free()
 
 
 @pytest.mark.parametrize("step", [3, 16])

@@ -303,6 +303,15 @@ def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
 
 
+def _checked_roi(roi: np.ndarray, img: "DeviceImage") -> np.ndarray:
+    """The ROI as contiguous bytes of the image's shape (H, W): the C-ABI reads
+    W * H bytes from the pointer, so any other shape is refused here."""
+    r = np.asarray(roi)
+    if r.shape != (img.height, img.width):
+        raise ValueError(f"roi has shape {r.shape}, the image is {img.height} x {img.width} (H x W)")
+    return np.ascontiguousarray(r, np.uint8)
+
+
 class Context:
     """Owns a dic_ctx (device, stream, HBM budget)."""
 
@@ -431,7 +440,7 @@ class Context:
         out = np.zeros(n, RESULT_DTYPE)
         roi_p = None
         if roi is not None:
-            r = np.ascontiguousarray(roi, dtype=np.uint8)
+            r = _checked_roi(roi, ref)
             roi_p = _ptr(r)
         self.check(self.lib.dic_icgn_solve_batch(self.handle, ctypes.byref(params), ref.handle, grad.handle,
                                                  coeffs.handle, roi_p, _ptr(xy), _ptr(ini), n, _ptr(out)))
@@ -440,7 +449,7 @@ class Context:
     def propagate(self, params: IcgnParams, spacing: int, max_znssd: float, ref: "DeviceImage", grad,
                   coeffs: "Coefficients", roi: np.ndarray, seeds: np.ndarray):
         """dic_propagate: returns (grid results (gh, gw), points_computed)."""
-        r = np.ascontiguousarray(roi, np.uint8)
+        r = _checked_roi(roi, ref)
         sd = np.ascontiguousarray(seeds, SEED_DTYPE)
         step = max(int(spacing), 0) + 1
         W, H = ref.width, ref.height
@@ -460,7 +469,7 @@ class Context:
         """dic_integer_search over host points; ESTIMATE_DTYPE records."""
         xy = np.ascontiguousarray(points_xy, np.float64).reshape(-1, 2)
         pr = np.ascontiguousarray(prior_uv, np.float64).reshape(-1, 2) if prior_uv is not None else None
-        r = np.ascontiguousarray(roi, np.uint8) if roi is not None else None
+        r = _checked_roi(roi, ref) if roi is not None else None
         out = np.zeros(xy.shape[0], ESTIMATE_DTYPE)
         self.check(self.lib.dic_integer_search(self.handle, ctypes.byref(params), ref.handle, deformed.handle,
                                                _ptr(r) if r is not None else None, _ptr(xy),
@@ -468,7 +477,7 @@ class Context:
         return out
 
     def seed_candidates(self, params: SeedParams, ref: "DeviceImage", roi: np.ndarray) -> np.ndarray:
-        r = np.ascontiguousarray(roi, np.uint8)
+        r = _checked_roi(roi, ref)
         out = np.zeros((max(1, params.seed_count), 2))
         n = ctypes.c_int(0)
         self.check(self.lib.dic_seed_candidates(self.handle, ctypes.byref(params), ref.handle, _ptr(r), _ptr(out),
@@ -477,7 +486,7 @@ class Context:
 
     def select_seed(self, params: SeedParams, ref: "DeviceImage", deformed: "DeviceImage", roi: np.ndarray):
         """dic_select_seed: (SEED_EVAL_DTYPE evaluations, index of the best seed or -1)."""
-        r = np.ascontiguousarray(roi, np.uint8)
+        r = _checked_roi(roi, ref)
         out = np.zeros(max(1, params.seed_count), SEED_EVAL_DTYPE)
         n, best = ctypes.c_int(0), ctypes.c_int(-1)
         self.check(self.lib.dic_select_seed(self.handle, ctypes.byref(params), ref.handle, deformed.handle, _ptr(r),

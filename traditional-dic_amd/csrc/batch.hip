@@ -30,6 +30,10 @@ struct dic_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int64_t runs = 0;
+    // recorded after every graph launch on the stream it ran on (possibly a
+    // caller's): the item table and the graph are not touched before it
+    hipEvent_t last_run = nullptr;
+    bool run_recorded = false;
 };
 
 namespace {
@@ -37,8 +41,17 @@ namespace {
 constexpr int kInterleaveRun = 256; // POIs per turn of DIC_BATCH_ORDER_INTERLEAVE
 constexpr int kXcds = 8;            // accelerator complex dies of an MI355X
 
+// Waits for the last replay, whatever stream it was launched on, so that its
+// kernels no longer read the item table or belong to the graph.
+void wait_last_run(dic_batch *b)
+{
+    if (b->run_recorded) (void)hipEventSynchronize(b->last_run);
+    b->run_recorded = false;
+}
+
 void drop_graph(dic_batch *b)
 {
+    wait_last_run(b);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -160,6 +173,7 @@ int prepare(dic_batch *b)
         if (b->groups.size() > (size_t)kMaxBatchGroups)
             return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "too many launch groups in one batch");
         if (!items.empty()) {
+            wait_last_run(b);
             int rc = check_hip(ctx, hipMemcpyAsync(b->d_items, items.data(), sizeof(int2) * items.size(),
                                                    hipMemcpyHostToDevice, ctx->stream),
                                "batch item upload");
@@ -196,6 +210,8 @@ void release(dic_batch *b)
     ctx_free(b->ctx, b->d_pairs, sizeof(BatchPair) * b->pairs.size());
     ctx_free(b->ctx, b->d_items, sizeof(int2) * (size_t)b->total);
     ctx_free(b->ctx, b->d_counter, sizeof(unsigned) * kMaxBatchGroups);
+    if (b->last_run) (void)hipEventDestroy(b->last_run);
+    b->last_run = nullptr;
 }
 
 } // namespace
@@ -214,6 +230,8 @@ int dic_batch_create(dic_ctx *ctx, const dic_batch_item *items, int n_items, int
     b->objective = objective;
     int rc = check_hip(ctx, hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
                        "device attribute");
+    if (rc == DIC_OK)
+        rc = check_hip(ctx, hipEventCreateWithFlags(&b->last_run, hipEventDisableTiming), "batch event");
     for (int k = 0; k < n_items && rc == DIC_OK; ++k) {
         const dic_batch_item &it = items[k];
         if (it.optimizer != DIC_OPTIMIZER_ICGN && it.optimizer != DIC_OPTIMIZER_FGN) {
@@ -346,7 +364,11 @@ int dic_batch_run(dic_batch *b, void *stream)
     if (rc != DIC_OK) return rc;
     hipStream_t s = stream ? (hipStream_t)stream : b->ctx->stream;
     rc = check_hip(b->ctx, hipGraphLaunch(b->exec, s), "batch graph launch");
-    if (rc == DIC_OK) ++b->runs;
+    if (rc == DIC_OK) rc = check_hip(b->ctx, hipEventRecord(b->last_run, s), "batch event");
+    if (rc == DIC_OK) {
+        b->run_recorded = true;
+        ++b->runs;
+    }
     return rc;
 }
 
@@ -425,7 +447,7 @@ void dic_batch_destroy(dic_batch *b)
 {
     if (!b) return;
     if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
-    release(b);
+    release(b); // waits for the last replay on whichever stream it ran
     delete b;
 }
 

@@ -254,6 +254,8 @@ __global__ void sample_kernel(const CS *__restrict__ coeffs, int W, int H, int p
 
 } // namespace
 
+size_t prefilter_row_lds(int PW) { return sizeof(double) * 2 * (size_t)PW; }
+
 hipError_t launch_prefilter(const float *img, int W, int H, int degree, int border, int exact, int precision,
                             void *out, double *tmp_a, double *tmp_b, hipStream_t s)
 {
@@ -269,7 +271,7 @@ hipError_t launch_prefilter(const float *img, int W, int H, int degree, int bord
             hipLaunchKernelGGL(pad_kernel<float>, grid, dim3(256), 0, s, img, W, H, b, exact ? 1 : 0, (float *)out);
         return hipGetLastError();
     }
-    const size_t lds = sizeof(double) * 2 * (size_t)PW;
+    const size_t lds = prefilter_row_lds(PW);
     hipError_t e = hipSuccess;
     if (lds > 64 * 1024) {
         e = hipFuncSetAttribute((const void *)prefilter_rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,

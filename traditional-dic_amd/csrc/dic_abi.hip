@@ -34,6 +34,23 @@ size_t solve_dynamic_lds(bool fgn, int precision, int objective, int nfull)
     return twopass ? sizeof(double) * (size_t)nfull : 0;
 }
 
+int check_prefilter_size(dic_ctx *ctx, int W, int H, int border, const char *who)
+{
+    const int b = border < 3 ? 3 : border;
+    const size_t PW = (size_t)W + 2 * (size_t)b, PH = (size_t)H + 2 * (size_t)b;
+    if (PW * PH * sizeof(double) >= ((size_t)1 << 31))
+        return set_error(ctx, DIC_ERR_UNSUPPORTED,
+                         std::string(who) + ": padded coefficient image of 2 GiB or more (" + std::to_string(PW) +
+                             " x " + std::to_string(PH) + ")");
+    const size_t need = prefilter_row_lds((int)PW);
+    if (ctx->max_lds != 0 && need > ctx->max_lds)
+        return set_error(ctx, DIC_ERR_UNSUPPORTED,
+                         std::string(who) + ": a padded row of " + std::to_string(PW) + " samples needs " +
+                             std::to_string(need) + " bytes of LDS per workgroup; the device has " +
+                             std::to_string(ctx->max_lds));
+    return DIC_OK;
+}
+
 int check_solve_lds(dic_ctx *ctx, bool fgn, int precision, int objective, int radius, int nfull)
 {
     const size_t need = solve_static_lds() + solve_dynamic_lds(fgn, precision, objective, nfull);
@@ -378,9 +395,8 @@ int dic_bspline_prefilter(dic_ctx *ctx, const dic_image *img, int degree, int bo
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "unknown precision");
     const int b = border < 3 ? 3 : border;
     const int PW = img->width + 2 * b, PH = img->height + 2 * b;
-    if (PW > 16384) return set_error(ctx, DIC_ERR_UNSUPPORTED, "padded width above 16384");
-    if ((size_t)PW * PH * sizeof(double) >= ((size_t)1 << 31))
-        return set_error(ctx, DIC_ERR_UNSUPPORTED, "padded coefficient image of 2 GiB or more");
+    const int rc0 = check_prefilter_size(ctx, img->width, img->height, b, "B-spline prefilter");
+    if (rc0 != DIC_OK) return rc0;
     dic_coeffs *c = new (std::nothrow) dic_coeffs();
     if (!c) return set_error(ctx, DIC_ERR_HIP, "host allocation failed");
     c->ctx = ctx;

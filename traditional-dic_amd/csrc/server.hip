@@ -488,9 +488,8 @@ int dic_server_create(dic_ctx *ctx, const dic_server_config *cfg, dic_server **o
     if (c.optimizer != DIC_OPTIMIZER_ICGN && c.optimizer != DIC_OPTIMIZER_FGN)
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "server: unknown optimizer");
     const int b = 3;
-    if (c.max_width + 2 * b > 16384 ||
-        (size_t)(c.max_width + 2 * b) * (c.max_height + 2 * b) * sizeof(double) >= ((size_t)1 << 31))
-        return set_error(ctx, DIC_ERR_UNSUPPORTED, "server: frames too large for the prefilter");
+    int rc0 = check_prefilter_size(ctx, c.max_width, c.max_height, b, "server: largest frame");
+    if (rc0 != DIC_OK) return rc0;
     dic_server *s = new (std::nothrow) dic_server;
     if (!s) return set_error(ctx, DIC_ERR_HIP, "host allocation failed");
     s->ctx = ctx;

@@ -254,7 +254,16 @@ class ICGNSolver:
     precision: "fp64" (plain drop-in, the reference's arithmetic) or "mixed"
     (FP32 sampling and compensated sums, FP64 solve/update).
     gradient: "central" (pipeline convention, reliability_propagation.cpp:84-112)
-    or "bspline" (standalone solve(), bspline.cpp:289-294)."""
+    or "bspline" (standalone solve(), bspline.cpp:289-294).
+
+    Nothing is cached between calls: solve() / solve_batch() on host images
+    prepare the pair (gradient cache + B-spline coefficients) for that call
+    and free it afterwards, as the reference's solve() builds its
+    interpolators per call (icgn.cpp:363-382).  A caller that solves one pair
+    many times prepares it once with prepare() and hands the PreparedPair to
+    solve_with_interpolators / solve_with_mask / solve_batch(pair=...), as the
+    reference's callers hand over their interpolators
+    (reliability_propagation.cpp:182-185, :341-354)."""
 
     def __init__(self, config: SubsetConfig | dict | None = None, precision: str = "fp64",
                  gradient: str = "central", device: int = 0, context: _lib.Context | None = None):
@@ -264,46 +273,75 @@ class ICGNSolver:
         self.precision = _lib.PRECISION_FP64 if precision == "fp64" else _lib.PRECISION_MIXED
         self.gradient_mode = _lib.GRADIENT_CENTRAL if gradient == "central" else _lib.GRADIENT_BSPLINE_HALF
         self.ctx = context if context is not None else _lib.Context(device)
-        self._pair = None
-        self._pair_key = None
 
-    def prepare(self, reference: np.ndarray, deformed: np.ndarray) -> PreparedPair:
-        key = (id(reference), id(deformed))
-        if self._pair is None or self._pair_key != key:
-            if self._pair is not None:
-                self._pair.free()
-            self._pair = PreparedPair(self.ctx, reference, deformed, self.config, self.precision, self.gradient_mode)
-            self._pair_key = key
-        return self._pair
+    def prepare(self, reference, deformed) -> PreparedPair:
+        """A new device-resident pair (uploaded host arrays or borrowed
+        DeviceImage handles) with its gradient cache and deformed coefficients;
+        the caller owns it and frees it with free()."""
+        return PreparedPair(self.ctx, reference, deformed, self.config, self.precision, self.gradient_mode)
 
-    def solve_batch(self, reference, deformed, points, initial, roi=None) -> np.ndarray:
+    def solve_batch(self, reference, deformed, points, initial, roi=None, pair: PreparedPair | None = None) -> np.ndarray:
         """Many POIs at once; returns RESULT_DTYPE records (Displacement2D + iterations).
-        With roi (and config.truncate_roi_subsets) the masked variants run (solve_with_mask)."""
-        if reference.shape != deformed.shape:
-            # icgn.cpp:464-470: every POI reports InvalidInput with the initial u, v
-            xy = np.asarray(points, np.float64).reshape(-1, 2)
-            ini = np.ascontiguousarray(initial, INITIAL_DTYPE)
-            out = np.zeros(xy.shape[0], RESULT_DTYPE)
-            out["x"], out["y"] = xy[:, 0], xy[:, 1]
-            out["p"][:, 0], out["p"][:, 1] = ini["u"], ini["v"]
-            out["correlation"] = ini["confidence"]
-            out["status"] = _lib.INVALID_INPUT
-            return out
-        pair = self.prepare(reference, deformed)
-        masked = roi is not None
-        params = make_params(self.config, self.precision, masked)
-        return self.ctx.solve(params, pair.ref, pair.grad, pair.coeffs, points, initial,
-                              None if roi is None else np.asarray(roi, np.uint8))
+        With roi the masked variants run (solve_with_mask).  pair: a PreparedPair
+        of (reference, deformed) to use instead of preparing one for this call;
+        it must have their size."""
+        xy = np.asarray(points, np.float64).reshape(-1, 2)
+        ini = np.ascontiguousarray(initial, INITIAL_DTYPE)
+        rw, rh = _image_size(reference)
+        dw, dh = _image_size(deformed)
+        if (rw, rh) != (dw, dh) or rw <= 0 or rh <= 0:
+            return invalid_results(xy, ini)  # icgn.cpp:464-470
+        if roi is not None:
+            roi = np.asarray(roi)
+            if roi.shape != (rh, rw):
+                return invalid_results(xy, ini)  # icgn.cpp:648-652
+            roi = np.ascontiguousarray(roi, np.uint8)
+        owned = pair is None
+        if owned:
+            pair = self.prepare(reference, deformed)
+        elif (pair.width, pair.height) != (rw, rh):
+            raise ValueError(f"prepared pair is {pair.width}x{pair.height}, the images are {rw}x{rh}")
+        try:
+            params = make_params(self.config, self.precision, roi is not None)
+            return self.ctx.solve(params, pair.ref, pair.grad, pair.coeffs, xy, ini, roi)
+        finally:
+            if owned:
+                pair.free()
 
     def solve(self, reference, deformed, point, initial) -> np.void:
         return self.solve_batch(reference, deformed, np.asarray(point, np.float64).reshape(1, 2),
                                 np.atleast_1d(np.asarray(initial, INITIAL_DTYPE)))[0]
 
-    solve_with_interpolators = solve
-
-    def solve_with_mask(self, reference, deformed, roi, point, initial) -> np.void:
+    def solve_with_interpolators(self, reference, deformed, point, initial, pair: PreparedPair) -> np.void:
+        """subset_solver.hpp:41-51: the pair's gradient cache and coefficients
+        play the reference and deformed interpolators."""
         return self.solve_batch(reference, deformed, np.asarray(point, np.float64).reshape(1, 2),
-                                np.atleast_1d(np.asarray(initial, INITIAL_DTYPE)), roi=roi)[0]
+                                np.atleast_1d(np.asarray(initial, INITIAL_DTYPE)), pair=pair)[0]
+
+    def solve_with_mask(self, reference, deformed, roi, point, initial, pair: PreparedPair | None = None) -> np.void:
+        return self.solve_batch(reference, deformed, np.asarray(point, np.float64).reshape(1, 2),
+                                np.atleast_1d(np.asarray(initial, INITIAL_DTYPE)), roi=roi, pair=pair)[0]
+
+
+def _image_size(img) -> tuple[int, int]:
+    """(width, height) of a host array or a DeviceImage."""
+    if isinstance(img, _lib.DeviceImage):
+        return img.width, img.height
+    a = np.asarray(img)
+    if a.ndim != 2:
+        raise ValueError("Image must be a 2D numpy array.")
+    return a.shape[1], a.shape[0]
+
+
+def invalid_results(xy: np.ndarray, ini: np.ndarray) -> np.ndarray:
+    """Every POI InvalidInput with the initial u, v and correlation = confidence
+    (the reference's early exits, icgn.cpp:455-470, :639-652)."""
+    out = np.zeros(xy.shape[0], RESULT_DTYPE)
+    out["x"], out["y"] = xy[:, 0], xy[:, 1]
+    out["p"][:, 0], out["p"][:, 1] = ini["u"], ini["v"]
+    out["correlation"] = ini["confidence"]
+    out["status"] = _lib.INVALID_INPUT
+    return out
 
 
 class ForwardGaussNewtonSolver(ICGNSolver):
@@ -328,16 +366,35 @@ def make_solver(config: SubsetConfig | dict | None = None, precision: str = "fp6
 
 def propagate(reference: np.ndarray, deformed: np.ndarray, roi: np.ndarray, seeds: np.ndarray,
               config: SubsetConfig | dict | None = None, precision: str = "fp64",
-              solver: ICGNSolver | None = None):
+              solver: ICGNSolver | None = None, pair: PreparedPair | None = None):
     """ReliabilityPropagation::propagate (reliability_propagation.cpp:130-394) on
     the GPU from caller-supplied seeds (SEED_DTYPE: x, y, displacement).
-    Returns (results_to_dict of the grid, grid results (gh, gw), points_computed)."""
+    Returns (results_to_dict of the grid, grid results (gh, gw), points_computed).
+    pair: a PreparedPair of (reference, deformed) to reuse (otherwise one is
+    prepared for this call and freed).  Images of different sizes, an ROI of
+    another size, a subset radius below 1, a negative spacing or no seeds give
+    the reference's empty result (:140-146)."""
     solver = solver or make_solver(config, precision)
     cfg = solver.config
-    pair = solver.prepare(reference, deformed)
-    params = make_params(cfg, solver.precision, masked=bool(cfg.truncate_roi_subsets))
-    grid, n = solver.ctx.propagate(params, cfg.propagation_spacing, cfg.propagation_max_znssd, pair.ref, pair.grad,
-                                   pair.coeffs, np.asarray(roi, np.uint8), seeds)
+    rw, rh = _image_size(reference)
+    dw, dh = _image_size(deformed)
+    roi = np.asarray(roi)
+    if ((rw, rh) != (dw, dh) or rw <= 0 or rh <= 0 or roi.shape != (rh, rw) or cfg.subset_radius < 1
+            or cfg.propagation_spacing < 0 or len(seeds) == 0):
+        empty = np.zeros((0, 0), RESULT_DTYPE)
+        return results_to_dict(empty.ravel()), empty, 0
+    owned = pair is None
+    if owned:
+        pair = solver.prepare(reference, deformed)
+    elif (pair.width, pair.height) != (rw, rh):
+        raise ValueError(f"prepared pair is {pair.width}x{pair.height}, the images are {rw}x{rh}")
+    try:
+        params = make_params(cfg, solver.precision, masked=bool(cfg.truncate_roi_subsets))
+        grid, n = solver.ctx.propagate(params, cfg.propagation_spacing, cfg.propagation_max_znssd, pair.ref,
+                                       pair.grad, pair.coeffs, np.ascontiguousarray(roi, np.uint8), seeds)
+    finally:
+        if owned:
+            pair.free()
     return results_to_dict(grid.ravel()), grid, n
 
 
