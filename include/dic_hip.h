@@ -177,6 +177,24 @@ int dic_icgn_solve_batch_device(dic_ctx *ctx, const dic_icgn_params *params, con
                                 const double *poi_xy_device, const dic_initial *init_device, int n,
                                 dic_result *out_device, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  dic_icgn_solve_batch_device with the same arguments,
+ * kernels and results, but no allocation of any kind: the circular-subset
+ * offset table, which the other entry allocates on the first use of each
+ * radius, is rebuilt at every call by a small kernel on `stream` into a
+ * static device array of DIC_NOALLOC_MAX_SAMPLES entries (128 KiB, held by
+ * every GPU the library runs on until the process ends), so subset radii up
+ * to DIC_NOALLOC_MAX_RADIUS are accepted and larger ones are refused with
+ * DIC_ERR_UNSUPPORTED.  Each GPU has one such array: calls that overlap on
+ * one GPU (two streams or two host threads) race on it. */
+#define DIC_NOALLOC_MAX_SAMPLES 32768
+#define DIC_NOALLOC_MAX_RADIUS 102
+int dic_icgn_solve_batch_device_noalloc(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref,
+                                        const dic_grad *grad, const dic_coeffs *def, const uint8_t *roi_device,
+                                        const double *poi_xy_device, const dic_initial *init_device, int n,
+                                        dic_result *out_device, void *stream);
+
 /* ---- Batched solving: POIs of many image pairs, with uneven subset radii and
  * shape orders, in one launch captured in a hipGraph.  No counterpart in the
  * reference, which solves pair after pair and POI after POI (the callers

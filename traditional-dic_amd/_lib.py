@@ -44,8 +44,9 @@ EXPORTED_SYMBOLS = (
     "dic_host_alloc", "dic_host_free", "dic_server_create", "dic_server_submit", "dic_server_wait",
     "dic_server_frames_done", "dic_server_stats_get", "dic_server_now", "dic_server_error", "dic_server_destroy",
     "dic_propagate", "dic_integer_search", "dic_seed_candidates", "dic_select_seed", "dic_image_mirror_pad",
-    "dic_image_download",
+    "dic_image_download", "dic_icgn_solve_batch_device_noalloc",
 )
+NOALLOC_MAX_RADIUS = 102  # DIC_NOALLOC_MAX_RADIUS
 
 # Displacement2D + iterations (dic_result); InitialDisplacement (dic_initial)
 RESULT_DTYPE = np.dtype([
@@ -232,6 +233,8 @@ _SIGNATURES = {
     "dic_bspline_sample": ([_vp, _vp, _vp, _i32, _vp, _vp], _i32),
     "dic_icgn_solve_batch": ([_vp, ctypes.POINTER(IcgnParams), _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp], _i32),
     "dic_icgn_solve_batch_device": (
+        [_vp, ctypes.POINTER(IcgnParams), _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp], _i32),
+    "dic_icgn_solve_batch_device_noalloc": (
         [_vp, ctypes.POINTER(IcgnParams), _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp], _i32),
     "dic_buffer_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], _i32),
     "dic_buffer_free": ([_vp, _vp], None),
@@ -510,6 +513,15 @@ class Context:
                                                         coeffs.handle, roi_ptr, xy_ptr, init_ptr, int(n), out_ptr,
                                                         stream))
 
+
+    def solve_device_noalloc(self, params: IcgnParams, ref: "DeviceImage", grad: "Gradient", coeffs: "Coefficients",
+                             xy_ptr: int, init_ptr: int, n: int, out_ptr: int, roi_ptr: int | None = None,
+                             stream: int | None = None):
+        """dic_icgn_solve_batch_device_noalloc: solve_device without any device
+        allocation (radius <= NOALLOC_MAX_RADIUS)."""
+        self.check(self.lib.dic_icgn_solve_batch_device_noalloc(
+            self.handle, ctypes.byref(params), ref.handle, grad.handle if grad is not None else None, coeffs.handle,
+            roi_ptr, xy_ptr, init_ptr, int(n), out_ptr, stream))
 
     def batch(self, items, objective: int = OBJECTIVE_ZNSSD) -> "Batch":
         """items: sequence of dicts with keys reference, gradient, deformed (handles),

@@ -179,6 +179,11 @@ int dic_ctx_create(int device, uint64_t hbm_budget_bytes, dic_ctx **out)
         return DIC_ERR_HIP;
     }
     ctx->stream = ctx->own_stream;
+    if (noalloc_setup(ctx) != DIC_OK) { // the device address of the no-allocation entry's table
+        (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return DIC_ERR_HIP;
+    }
     *out = ctx;
     return DIC_OK;
 }
@@ -624,6 +629,13 @@ int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref,
                              params->subset_radius, count);
         if (rc != DIC_OK) return rc;
     }
+    return launch_solve(ctx, params, ref, grad, def, roi_dev, xy, init, n, out, s, tile_hint, table, count);
+}
+
+int launch_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
+                 const dic_coeffs *def, const uint8_t *roi_dev, const double *xy, const dic_initial *init, int n,
+                 dic_result *out, hipStream_t s, int tile_hint, const int *table, int count)
+{
     IcgnLaunch L;
     L.params = params;
     // size mismatch between reference and deformed: every POI reports

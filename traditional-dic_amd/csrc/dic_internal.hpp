@@ -28,6 +28,8 @@ struct dic_ctx {
     size_t staging_bytes = 0;
     // LDS one workgroup may use on this device (hipDeviceAttributeMaxSharedMemoryPerBlock)
     size_t max_lds = 0;
+    // device address of the no-allocation entry's table array on this device
+    int *noalloc_table = nullptr;
 };
 
 struct dic_image {
@@ -67,6 +69,14 @@ int validate_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image 
 int run_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
               const dic_coeffs *def, const uint8_t *roi_dev, const double *xy, const dic_initial *init, int n,
               dic_result *out, hipStream_t s, int tile_hint = -1);
+// run_solve's launch with the circular-subset table given (count entries,
+// dy-outer / dx-inner order, packed (dy << 16) | (dx & 0xffff)).
+int launch_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
+                 const dic_coeffs *def, const uint8_t *roi_dev, const double *xy, const dic_initial *init, int n,
+                 dic_result *out, hipStream_t s, int tile_hint, const int *table, int count);
+// Takes the device address of the no-allocation entry's table array on the
+// context's device (current at the call); dic_ctx_create calls it.
+int noalloc_setup(dic_ctx *ctx);
 
 // LDS of one solve workgroup: the static per-POI block (icgn.hip) plus the
 // per-sample buffer of FP64 two-pass ZNSSD IC-GN (one value) or of FGN (value
