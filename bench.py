@@ -137,12 +137,20 @@ def batched_leg(pkg, ctx, synth, reps=3):
     mx = [it["out"].download(pkg.RESULT_DTYPE) for it in st["mixed"][0]]
     conv = conv64 = 0
     dmax = 0.0
+    hit = {"fp64": 0, "mixed": 0}
+    split = split_ran_out = 0
     for f, m in zip(fp, mx):
         ok64 = (f["status"] == pkg.SUCCESS) & (f["valid"] == 1)
+        okm = (m["status"] == pkg.SUCCESS) & (m["valid"] == 1)
         d = np.maximum(np.abs(m["p"][:, 0] - f["p"][:, 0]), np.abs(m["p"][:, 1] - f["p"][:, 1]))
-        conv += int(((m["status"] == pkg.SUCCESS) & (m["valid"] == 1) & ok64 & (d <= 1e-3)).sum())
+        conv += int((okm & ok64 & (d <= 1e-3)).sum())
         conv64 += int(ok64.sum())
         dmax = max(dmax, float(np.max(d[ok64], initial=0.0)))
+        hit["fp64"] += int((ok64 & (f["iterations"] == 30)).sum())
+        hit["mixed"] += int((okm & (m["iterations"] == 30)).sum())
+        sp = ok64 & ~(okm & (d <= 1e-3))
+        split += int(sp.sum())
+        split_ran_out += int((sp & ((f["iterations"] == 30) | (m["iterations"] == 30))).sum())
     b.destroy()
     sizes = sorted({c.reference.shape[0] for c in cases})
     out = {"workload": (f"{len(cases)} pairs, sizes {sizes} (square), radius 10..30, order 1/2, step 4..16, "
@@ -154,7 +162,9 @@ def batched_leg(pkg, ctx, synth, reps=3):
            "converged": conv, "converged_fp64": conv64,
            "value": round(conv / (t_batch * 1e-3), 1), "fp64_value": round(conv64 / (t_fp64 * 1e-3), 1),
            "vs_fp64_drop_in": round((conv / t_batch) / (conv64 / t_fp64), 3),
-           "mixed_vs_fp64_max_px": dmax}
+           "mixed_vs_fp64_max_px": dmax,
+           "hit_max_iterations": hit,
+           "fp64_converged_not_mixed": {"pois": split, "of_which_at_iteration_limit": split_ran_out}}
     st.clear()
     log(f"batched: {out}")
     return out

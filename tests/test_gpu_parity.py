@@ -440,3 +440,28 @@ def test_cpp_host_layer_solves(pkg):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     _report(r.stdout.strip())
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_closing_a_context_releases_what_it_owns(pkg, synth):
+    """A context closed while its images, coefficients, buffers, a batch and a
+    server are still referenced releases them first; dropping them afterwards
+    calls nothing on the destroyed context (a failed test keeps its frame's
+    objects alive past the session's context teardown)."""
+    import gc
+    c = pkg.Context(0)
+    img = synth.quantize_unit(synth.render(96, 96, 120, 3))
+    ref, dfm = c.upload_image(img), c.upload_image(img)
+    grad = c.ref_gradient(ref, pkg.GRADIENT_CENTRAL, 5, 3, pkg.PRECISION_MIXED)
+    co = c.prefilter(dfm, 5, 3, True, pkg.PRECISION_MIXED)
+    xy = np.array([[48.0, 48.0]])
+    dxy, dini = c.to_device(xy), c.to_device(pkg.initial_guesses([0.0], [0.0]))
+    out = c.alloc(pkg.RESULT_DTYPE.itemsize)
+    b = c.batch([dict(reference=ref, gradient=grad, deformed=co, poi_xy=dxy, init=dini, out=out, n=1,
+                      subset_radius=10, shape_order=1)])
+    b.run()
+    srv = c.server(radius=10, max_width=96, max_height=96)
+    c.close()
+    assert c.handle is None
+    assert all(h.handle is None for h in (ref, dfm, grad, co, b, srv)) and out.ptr is None and dxy.ptr is None
+    del ref, dfm, grad, co, dxy, dini, out, b, srv
+    gc.collect()

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -320,11 +321,19 @@ class Context:
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
         self.lib = load_library()
+        # everything created on this context (handles, buffers, batches,
+        # servers): released before the context itself, whatever order the
+        # garbage collector finds them in
+        self._dependents = weakref.WeakSet()
         h = _vp()
         rc = self.lib.dic_ctx_create(int(device), int(hbm_budget_bytes), ctypes.byref(h))
         if rc != DIC_OK:
             raise DicError(rc, "dic_ctx_create failed (no HIP device?)")
         self.handle = h
+
+    def _own(self, obj):
+        self._dependents.add(obj)
+        return obj
 
     def check(self, rc: int):
         if rc != DIC_OK:
@@ -333,6 +342,13 @@ class Context:
 
     def close(self):
         if getattr(self, "handle", None):
+            # servers and batches first (their work runs on this context), then
+            # the device handles and buffers, then the context
+            deps = list(getattr(self, "_dependents", ()))
+            for kind in (Server, Batch, _Handle, DeviceBuffer):
+                for d in deps:
+                    if isinstance(d, kind):
+                        d._release()
             self.lib.dic_ctx_destroy(self.handle)
             self.handle = None
 
@@ -605,6 +621,7 @@ class Server:
 
     def __init__(self, ctx, handle):
         self.ctx, self.handle = ctx, handle
+        ctx._own(self)
 
     def submit(self, reference: np.ndarray, frames, poi_xy: np.ndarray, init: np.ndarray) -> ServedRequest:
         ref = np.ascontiguousarray(reference, np.float32)
@@ -641,6 +658,9 @@ class Server:
         return e.decode() if e else ""
 
     def close(self):
+        self._release()
+
+    def _release(self):
         if self.handle:
             self.ctx.lib.dic_server_destroy(self.handle)
             self.handle = None
@@ -657,6 +677,7 @@ class Batch:
 
     def __init__(self, ctx: Context, handle, items):
         self.ctx, self.handle, self.items = ctx, handle, items
+        ctx._own(self)
 
     def set_tuning(self, mapping: int, waves_per_cu: int = 8, chunk: int = 1, order: int = BATCH_ORDER_COST):
         t = BatchTuning(mapping, waves_per_cu, chunk, order)
@@ -687,6 +708,9 @@ class Batch:
         return best, ms.value
 
     def destroy(self):
+        self._release()
+
+    def _release(self):
         if self.handle:
             self.ctx.lib.dic_batch_destroy(self.handle)
             self.handle = None
@@ -703,6 +727,7 @@ class DeviceBuffer:
 
     def __init__(self, ctx: Context, ptr: int, nbytes: int):
         self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+        ctx._own(self)
 
     def download(self, dtype, count: int | None = None) -> np.ndarray:
         dt = np.dtype(dtype)
@@ -717,6 +742,9 @@ class DeviceBuffer:
         self.ctx.check(self.ctx.lib.dic_buffer_upload(self.ctx.handle, self.ptr, _ptr(a), a.nbytes))
 
     def free(self):
+        self._release()
+
+    def _release(self):
         if self.ptr and self.ctx.handle:
             self.ctx.lib.dic_buffer_free(self.ctx.handle, self.ptr)
         self.ptr = None
@@ -734,8 +762,12 @@ class _Handle:
     def __init__(self, ctx: Context, handle):
         self.ctx = ctx
         self.handle = handle
+        ctx._own(self)
 
     def free(self):
+        self._release()
+
+    def _release(self):
         if self.handle and self.ctx.handle:
             getattr(self.ctx.lib, self._free)(self.handle)
         self.handle = None

@@ -197,6 +197,7 @@ struct IcgnLaunch {
 };
 hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s);
 
+
 // Share of the quads of consecutive POIs (POI positions and guesses on the
 // host) whose starting subsets fit one LDS tile; the host-pointer entry points
 // pass tile_hint = share >= 1/2.

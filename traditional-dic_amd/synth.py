@@ -241,32 +241,35 @@ class PairCase:
     seed: int
 
 
+def batched_pair(j: int, sizes=(256, 512, 1024, 2048, 4096), seed0: int = 3000,
+                 density: float = 4000.0 / (512 * 512)) -> PairCase:
+    """Pair j of the uneven batch (seed seed0 + j): its size drawn from
+    `sizes`, a random affine warp about the centre (|u0|, |v0| < 3 px, strains
+    < 5e-3), subset radius U{10..30}, shape order U{1, 2} and POI step
+    U{4..16}; speckle at the 512^2 workload's blob density, 8-bit quantised;
+    init = integer truth."""
+    s = seed0 + j
+    rng = np.random.default_rng(s)
+    n = int(sizes[rng.integers(len(sizes))])
+    u0, v0 = rng.uniform(-3.0, 3.0, 2)
+    ux, uy, vx, vy = rng.uniform(-5e-3, 5e-3, 4)
+    radius = int(rng.integers(10, 31))
+    order = int(rng.integers(1, 3))
+    step = int(rng.integers(4, 17))
+    c = 0.5 * (n - 1)
+    f = affine(u0, v0, ux, uy, vx, vy, c, c)
+    nb = max(1, int(round(density * n * n)))
+    ref = quantize_unit(render(n, n, nb, s))
+    dfm = quantize_unit(render(n, n, nb, s, f))
+    xy = poi_grid(n, n, step)
+    xy = xy[solvable_mask(xy, n, n, radius)]
+    return PairCase(ref, dfm, f, xy, _integer_init(f, xy), radius, order, step, s)
+
+
 def batched_pairs(n_pairs: int = 64, sizes=(256, 512, 1024, 2048, 4096), seed0: int = 3000,
                   density: float = 4000.0 / (512 * 512)) -> list:
-    """Uneven batch: pair j (seed seed0 + j) draws its size from `sizes`, a
-    random affine warp about the centre (|u0|, |v0| < 3 px, strains < 5e-3),
-    subset radius U{10..30}, shape order U{1, 2} and POI step U{4..16};
-    speckle at the 512^2 workload's blob density, 8-bit quantised; init =
-    integer truth."""
-    cases = []
-    for j in range(n_pairs):
-        s = seed0 + j
-        rng = np.random.default_rng(s)
-        n = int(sizes[rng.integers(len(sizes))])
-        u0, v0 = rng.uniform(-3.0, 3.0, 2)
-        ux, uy, vx, vy = rng.uniform(-5e-3, 5e-3, 4)
-        radius = int(rng.integers(10, 31))
-        order = int(rng.integers(1, 3))
-        step = int(rng.integers(4, 17))
-        c = 0.5 * (n - 1)
-        f = affine(u0, v0, ux, uy, vx, vy, c, c)
-        nb = max(1, int(round(density * n * n)))
-        ref = quantize_unit(render(n, n, nb, s))
-        dfm = quantize_unit(render(n, n, nb, s, f))
-        xy = poi_grid(n, n, step)
-        xy = xy[solvable_mask(xy, n, n, radius)]
-        cases.append(PairCase(ref, dfm, f, xy, _integer_init(f, xy), radius, order, step, s))
-    return cases
+    """The uneven batch: pairs 0 .. n_pairs - 1 of batched_pair."""
+    return [batched_pair(j, sizes, seed0, density) for j in range(n_pairs)]
 
 
 def strain_sequence(width: int, height: int, n_frames: int, seed: int, step_strain: float = 2e-4,
