@@ -513,13 +513,13 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                     failed = true;
                     break;
                 }
-                const double gnorm = sqrt(var), sc = 2.0 / fnorm;
+                const double gnorm = sqrt(var), sc = 2.0 / fnorm, ig = 1.0 / gnorm;
 #pragma unroll
                 for (int j = 0; j < P; ++j) { // FP32 only: sg holds the monomial sums
                     const double v = (double)wave_total_f32(sg[j].s) * sd_half(j);
-                    if (lane == 0) sh.red[j] = sc * (sh.sfsd[j] - (v - md * sh.ssd[j]) / gnorm);
+                    if (lane == 0) sh.red[j] = sc * (sh.sfsd[j] - (v - md * sh.ssd[j]) * ig);
                 }
-                if (lane == 0) sh.red[P] = sh.sfh2 - 2.0 * (SFG - md * sh.sfh) / gnorm + var / (gnorm * gnorm);
+                if (lane == 0) sh.red[P] = sh.sfh2 - 2.0 * (SFG - md * sh.sfh) * ig + var * (ig * ig);
             } else {
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
@@ -698,8 +698,8 @@ __device__ Tile stage_tile(const IcgnArgs &a, int xlo, int ylo, int xhi, int yhi
 
 constexpr int kBigBox = 1 << 30;
 
-template <int OBJ, int D>
-__global__ __launch_bounds__(64 * kGroupPois, (kMinWaves<6, float>)) void icgn_group_kernel(IcgnArgs a)
+template <int P, int OBJ, int D>
+__global__ __launch_bounds__(64 * kGroupPois, (kMinWaves<P, float>)) void icgn_group_kernel(IcgnArgs a)
 {
     __shared__ IcgnShared sh[kGroupPois];
     __shared__ float tile[kGroupPitch * kGroupRows];
@@ -714,7 +714,7 @@ __global__ __launch_bounds__(64 * kGroupPois, (kMinWaves<6, float>)) void icgn_g
     __syncthreads();
     const int poi = g * kGroupPois + (int)(threadIdx.x >> 6);
     if (poi >= a.n) return;
-    solve_poi<6, OBJ, D, float, false>(a, poi, sh[threadIdx.x >> 6], nullptr, t);
+    solve_poi<P, OBJ, D, float, false>(a, poi, sh[threadIdx.x >> 6], nullptr, t);
 }
 
 // Serving ticks, first order FP32: POIs g .. g + 3 of the tick's contiguous
@@ -1417,7 +1417,7 @@ hipError_t launch_variant(const IcgnArgs &args, bool group, hipStream_t s)
     constexpr bool TWOPASS = OBJ == DIC_OBJECTIVE_ZNSSD && sizeof(R) == 8;
     if (P == 6 && sizeof(R) == 4 && !MASKED && group) {
         const int groups = (args.n + kGroupPois - 1) / kGroupPois;
-        hipLaunchKernelGGL((icgn_group_kernel<OBJ, D>), dim3(8 * ((groups + 7) / 8)), dim3(64 * kGroupPois), 0, s,
+        hipLaunchKernelGGL((icgn_group_kernel<P, OBJ, D>), dim3(8 * ((groups + 7) / 8)), dim3(64 * kGroupPois), 0, s,
                            args);
         return hipGetLastError();
     }
