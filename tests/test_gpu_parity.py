@@ -65,7 +65,8 @@ def _assert_parity(gpu, cpu, tol_uv=TOL_UV, tol_zncc=TOL_ZNCC, zncc=True):
 @pytest.mark.parametrize("exact", [True, False])
 def test_prefilter_parity(pkg, ctx, orc, degree, exact):
     rng = np.random.default_rng(degree)
-    for shape in ((37, 53), (130, 257)):
+    # padded heights and widths below the recursions' warm-up length included
+    for shape in ((37, 53), (130, 257), (4, 10), (1, 300), (20, 7), (3, 2100), (70, 1)):
         img = rng.random(shape).astype(np.float32)
         dimg = ctx.upload_image(img)
         for precision, tol in ((pkg.PRECISION_FP64, 1e-12), (pkg.PRECISION_MIXED, 2e-6)):
@@ -86,15 +87,25 @@ def test_prefilter_wide_image(pkg, ctx, orc):
 
 
 def test_prefilter_refuses_rows_beyond_lds(pkg, ctx):
-    """A padded row of 10306 samples needs 164896 bytes of LDS in the row pass,
-    more than one workgroup gets on MI355X (160 KiB): the call, and a server
-    configured for such frames, are refused with the bytes needed."""
+    """The row pass holds a whole padded row in LDS: two lines of doubles in
+    FP64 (16 bytes a sample); for FP32 storage two lines of floats, each
+    extended by the recursions' warm-up and one thread's chunk, with one spare
+    float per chunk (about 8.25 bytes a sample).  A padded row of 10306
+    samples needs 164896 bytes in FP64 and one of 20006 samples 165664 bytes
+    in FP32 storage, more than one workgroup gets on MI355X (160 KiB): the
+    call, and a server configured for such frames, are refused with the bytes
+    needed; narrower rows go through."""
     img = ctx.upload_image(np.zeros((4, 10300), np.float32))
     with pytest.raises(pkg.DicError, match="needs 164896 bytes of LDS"):
         ctx.prefilter(img, 5, 3, True, pkg.PRECISION_FP64)
-    with pytest.raises(pkg.DicError, match="needs 164896 bytes of LDS"):
-        ctx.server(max_width=10300, max_height=64)
-    ctx.prefilter(ctx.upload_image(np.zeros((4, 10200), np.float32)), 5, 3, True, pkg.PRECISION_MIXED)
+    ctx.prefilter(img, 5, 3, True, pkg.PRECISION_MIXED)
+    wide = ctx.upload_image(np.zeros((4, 20000), np.float32))
+    with pytest.raises(pkg.DicError, match="needs 165664 bytes of LDS"):
+        ctx.prefilter(wide, 5, 3, True, pkg.PRECISION_MIXED)
+    with pytest.raises(pkg.DicError, match="needs 165664 bytes of LDS"):
+        ctx.server(max_width=20000, max_height=64)
+    ctx.prefilter(ctx.upload_image(np.zeros((4, 10200), np.float32)), 5, 3, True, pkg.PRECISION_FP64)
+    ctx.prefilter(ctx.upload_image(np.zeros((4, 19700), np.float32)), 5, 3, True, pkg.PRECISION_MIXED)
 
 
 @pytest.mark.parametrize("degree", [1, 3, 5])

@@ -146,9 +146,8 @@ __global__ void prefilter_cols_causal_kernel(const double *__restrict__ in, int 
     }
 }
 
-template <typename OUT>
 __global__ void prefilter_cols_anticausal_kernel(const double *__restrict__ in, int PW, int PH, PoleArgs poles,
-                                                 OUT *__restrict__ out)
+                                                 double *__restrict__ out)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= PW) return;
@@ -159,8 +158,190 @@ __global__ void prefilter_cols_anticausal_kernel(const double *__restrict__ in, 
          --k, kk = wrap_prev(kk, PH)) {
         w1 = in[(size_t)kk * PW + x] + poles.z0 * w1;
         w2 = w1 + poles.z1 * w2;
-        if (k < r1) out[(size_t)kk * PW + x] = (OUT)(poles.gain * ((poles.count == 2) ? w2 : w1));
+        if (k < r1) out[(size_t)kk * PW + x] = poles.gain * ((poles.count == 2) ? w2 : w1);
     }
+}
+
+// ---- FP32 storage (the mixed path): two passes with an FP32 intermediate ----
+// The recursions run in FP64; every chunk starts kF32Warm steps early, which is
+// exact to |z|^28 (6e-11 for the quintic's larger pole, 1e-16 for the cubic
+// pole), below the FP32 rounding of the stored coefficients.  Both passes load
+// a thread's whole window (warm-up included) before the recursion starts, so
+// the serial chain waits on no memory.  A single pole has z1 = 0, and then the
+// second stage (y2 = y1 + 0 * y2) passes y1 through unchanged.  Each padded
+// pixel moves 16 bytes of HBM (image read, intermediate write and read,
+// coefficient write) against 8 compulsory.
+constexpr int kF32Warm = 28;
+constexpr int kF32ColChunk = 32;
+
+// Rows: one workgroup per padded row, one thread per CH samples.  The row
+// lives in LDS as floats in an extended index e = j + kF32Warm, j the
+// period-PW sample index (a line with the kF32Warm samples before sample 0 and
+// CH after the last, the causal intermediate with CH + kF32Warm after it), so
+// every window is a run of consecutive e with no wrap test.  Slot e + e / CH
+// keeps one spare float per CH entries: with CH even the lanes' windows start
+// an odd stride apart, and lanes reading the same step hit distinct banks.
+template <int CH>
+__device__ __forceinline__ int ext_slot(int e)
+{
+    return e + e / CH;
+}
+
+template <int CH>
+__host__ __device__ constexpr int f32_line_len(int PW)
+{
+    return PW + CH + kF32Warm;
+}
+template <int CH>
+__host__ __device__ constexpr int f32_buf_len(int PW)
+{
+    return PW + CH + 2 * kF32Warm;
+}
+template <int CH>
+__host__ __device__ constexpr int f32_slots(int len)
+{
+    return len + len / CH + 1;
+}
+
+template <int CH>
+__global__ __launch_bounds__(1024) void prefilter_rows_f32_kernel(const float *__restrict__ img, int W, int H,
+                                                                  int b, PoleArgs poles, float *__restrict__ out)
+{
+    extern __shared__ float lf[];
+    const int PW = W + 2 * b;
+    const int LL = f32_line_len<CH>(PW), BL = f32_buf_len<CH>(PW);
+    float *line = lf;
+    float *buf = lf + f32_slots<CH>(LL);
+    const int y = blockIdx.x;
+    const int sy = sym_index(y - b, H);
+    const float *row = img + (size_t)sy * W;
+    for (int e = threadIdx.x; e < LL; e += blockDim.x) {
+        int j = e - kF32Warm;
+        if ((unsigned)j >= (unsigned)PW) j = wrap(j, PW);
+        int x = j - b;
+        if ((unsigned)x >= (unsigned)W) x = sym_index(x, W);
+        line[ext_slot<CH>(e)] = row[x];
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int c0 = t * CH;
+    const int n = min(CH, PW - c0); // samples this thread owns (none past the row)
+    const int base = t * (CH + 1);  // ext_slot<CH>(c0) for a window starting at e = c0
+    const double z0 = poles.z0, z1 = poles.z1;
+    if (n > 0) {
+        // causal: line e in [c0, c0 + CH + kF32Warm) is sample c0 - kF32Warm + i
+        float v[CH + kF32Warm];
+#pragma unroll
+        for (int i = 0; i < CH + kF32Warm; ++i) v[i] = line[base + i + i / CH];
+        double y1 = 0.0, y2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < CH + kF32Warm; ++i) {
+            y1 = (double)v[i] + z0 * y1;
+            y2 = y1 + z1 * y2;
+            if (i >= kF32Warm && i - kF32Warm < n) buf[base + i + i / CH] = (float)y2; // e = c0 + i
+        }
+    }
+    __syncthreads();
+    // the causal values past the row's end, for the anticausal warm-up
+    for (int e = PW + kF32Warm + t; e < BL; e += blockDim.x)
+        buf[ext_slot<CH>(e)] = buf[ext_slot<CH>(wrap(e - kF32Warm, PW) + kF32Warm)];
+    __syncthreads();
+    if (n > 0) {
+        // anticausal: buf e = c0 + kF32Warm + i is sample c0 + i
+        float v[CH + kF32Warm];
+#pragma unroll
+        for (int i = 0; i < CH + kF32Warm; ++i) v[i] = buf[base + (i + kF32Warm) + (i + kF32Warm) / CH];
+        double w1 = 0.0, w2 = 0.0;
+#pragma unroll
+        for (int i = CH + kF32Warm - 1; i >= 0; --i) {
+            w1 = (double)v[i] + z0 * w1;
+            w2 = w1 + z1 * w2;
+            if (i < n) line[base + (i + kF32Warm) + (i + kF32Warm) / CH] = (float)(poles.gain * w2);
+        }
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < PW; x += blockDim.x) out[(size_t)y * PW + x] = line[ext_slot<CH>(x + kF32Warm)];
+}
+
+// Columns: one thread per (column, kF32ColChunk-row chunk) loads the rows
+// from kF32Warm above the chunk to 2 kF32Warm below its start + chunk, runs the
+// causal recursion down and the anticausal one back up; every load and store
+// is a coalesced row segment across the wave.  The last chunk of a column
+// runs past the final row into the period's start, as the periodic recursion
+// does, and stores only its own rows.
+__global__ __launch_bounds__(64) void prefilter_cols_f32_kernel(const float *__restrict__ in, int PW, int PH,
+                                                                 PoleArgs poles, float *__restrict__ out)
+{
+    constexpr int CC = kF32ColChunk, L = CC + 2 * kF32Warm;
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    if (x >= PW) return;
+    const int r0 = blockIdx.y * CC;
+    const int n = min(CC, PH - r0);
+    // kk: the period-PH row index of r0 - kF32Warm + i (the image may have
+    // fewer rows than the warm-up: the start takes a true modulo, every step wraps)
+    int kk = wrap(r0 - kF32Warm, PH);
+    float v[L];
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        v[i] = in[(size_t)kk * PW + x];
+        kk = wrap_next(kk, PH);
+    }
+    const double z0 = poles.z0, z1 = poles.z1;
+    double y1 = 0.0, y2 = 0.0;
+    float c[CC + kF32Warm]; // causal values of rows r0 .. r0 + CC + kF32Warm - 1
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        y1 = (double)v[i] + z0 * y1;
+        y2 = y1 + z1 * y2;
+        if (i >= kF32Warm) c[i - kF32Warm] = (float)y2;
+    }
+    double w1 = 0.0, w2 = 0.0;
+#pragma unroll
+    for (int i = CC + kF32Warm - 1; i >= 0; --i) {
+        w1 = (double)c[i] + z0 * w1;
+        w2 = w1 + z1 * w2;
+        if (i < n) out[(size_t)(r0 + i) * PW + x] = (float)(poles.gain * w2);
+    }
+}
+
+// Samples per row thread: the candidate that makes the fewest wave-steps per
+// row (waves per row x window length), at most 1024 threads a row.
+int f32_row_chunk(int PW)
+{
+    static const int cand[] = {8, 16, 24, 32};
+    int best = 32;
+    long best_cost = -1;
+    for (int ch : cand) {
+        const int threads = (PW + ch - 1) / ch;
+        if (threads > 1024) continue;
+        const long cost = (long)((threads + 63) / 64) * (ch + kF32Warm);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = ch;
+        }
+    }
+    return best;
+}
+
+template <int CH>
+size_t f32_row_lds_bytes(int PW)
+{
+    return sizeof(float) * (size_t)(f32_slots<CH>(f32_line_len<CH>(PW)) + f32_slots<CH>(f32_buf_len<CH>(PW)));
+}
+
+template <int CH>
+hipError_t launch_rows_f32(const float *img, int W, int H, int b, const PoleArgs &poles, float *mid, hipStream_t s)
+{
+    const int PW = W + 2 * b, PH = H + 2 * b;
+    const size_t lds = f32_row_lds_bytes<CH>(PW);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)prefilter_rows_f32_kernel<CH>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const int threads = ((PW + CH - 1) / CH + 63) / 64 * 64;
+    hipLaunchKernelGGL(prefilter_rows_f32_kernel<CH>, dim3(PH), dim3(threads), lds, s, img, W, H, b, poles, mid);
+    return hipGetLastError();
 }
 
 // Pipeline reference gradient (reliability_propagation.cpp:84-112): central
@@ -254,7 +435,16 @@ __global__ void sample_kernel(const CS *__restrict__ coeffs, int W, int H, int p
 
 } // namespace
 
-size_t prefilter_row_lds(int PW) { return sizeof(double) * 2 * (size_t)PW; }
+size_t prefilter_row_lds(int PW, int precision)
+{
+    if (precision == DIC_PRECISION_FP64) return sizeof(double) * 2 * (size_t)PW; // two lines of doubles
+    switch (f32_row_chunk(PW)) {                                                   // two extended lines of floats
+    case 8: return f32_row_lds_bytes<8>(PW);
+    case 16: return f32_row_lds_bytes<16>(PW);
+    case 24: return f32_row_lds_bytes<24>(PW);
+    default: return f32_row_lds_bytes<32>(PW);
+    }
+}
 
 hipError_t launch_prefilter(const float *img, int W, int H, int degree, int border, int exact, int precision,
                             void *out, double *tmp_a, double *tmp_b, hipStream_t s)
@@ -271,8 +461,21 @@ hipError_t launch_prefilter(const float *img, int W, int H, int degree, int bord
             hipLaunchKernelGGL(pad_kernel<float>, grid, dim3(256), 0, s, img, W, H, b, exact ? 1 : 0, (float *)out);
         return hipGetLastError();
     }
-    const size_t lds = prefilter_row_lds(PW);
     hipError_t e = hipSuccess;
+    if (!fp64) {
+        float *mid = reinterpret_cast<float *>(tmp_a);
+        switch (f32_row_chunk(PW)) {
+        case 8: e = launch_rows_f32<8>(img, W, H, b, poles, mid, s); break;
+        case 16: e = launch_rows_f32<16>(img, W, H, b, poles, mid, s); break;
+        case 24: e = launch_rows_f32<24>(img, W, H, b, poles, mid, s); break;
+        default: e = launch_rows_f32<32>(img, W, H, b, poles, mid, s); break;
+        }
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(prefilter_cols_f32_kernel, dim3((PW + 63) / 64, (PH + kF32ColChunk - 1) / kF32ColChunk),
+                           dim3(64), 0, s, mid, PW, PH, poles, (float *)out);
+        return hipGetLastError();
+    }
+    const size_t lds = sizeof(double) * 2 * (size_t)PW;
     if (lds > 64 * 1024) {
         e = hipFuncSetAttribute((const void *)prefilter_rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
@@ -284,12 +487,7 @@ hipError_t launch_prefilter(const float *img, int W, int H, int degree, int bord
     dim3 grid((PW + 63) / 64, (PH + kColChunk - 1) / kColChunk);
     hipLaunchKernelGGL(prefilter_cols_causal_kernel, grid, dim3(64), 0, s, tmp_a, PW, PH, poles, tmp_b);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (fp64)
-        hipLaunchKernelGGL(prefilter_cols_anticausal_kernel<double>, grid, dim3(64), 0, s, tmp_b, PW, PH, poles,
-                           (double *)out);
-    else
-        hipLaunchKernelGGL(prefilter_cols_anticausal_kernel<float>, grid, dim3(64), 0, s, tmp_b, PW, PH, poles,
-                           (float *)out);
+    hipLaunchKernelGGL(prefilter_cols_anticausal_kernel, grid, dim3(64), 0, s, tmp_b, PW, PH, poles, (double *)out);
     return hipGetLastError();
 }
 

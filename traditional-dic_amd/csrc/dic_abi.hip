@@ -34,7 +34,7 @@ size_t solve_dynamic_lds(bool fgn, int precision, int objective, int nfull)
     return twopass ? sizeof(double) * (size_t)nfull : 0;
 }
 
-int check_prefilter_size(dic_ctx *ctx, int W, int H, int border, const char *who)
+int check_prefilter_size(dic_ctx *ctx, int W, int H, int border, int precision, const char *who)
 {
     const int b = border < 3 ? 3 : border;
     const size_t PW = (size_t)W + 2 * (size_t)b, PH = (size_t)H + 2 * (size_t)b;
@@ -42,7 +42,7 @@ int check_prefilter_size(dic_ctx *ctx, int W, int H, int border, const char *who
         return set_error(ctx, DIC_ERR_UNSUPPORTED,
                          std::string(who) + ": padded coefficient image of 2 GiB or more (" + std::to_string(PW) +
                              " x " + std::to_string(PH) + ")");
-    const size_t need = prefilter_row_lds((int)PW);
+    const size_t need = prefilter_row_lds((int)PW, precision);
     if (ctx->max_lds != 0 && need > ctx->max_lds)
         return set_error(ctx, DIC_ERR_UNSUPPORTED,
                          std::string(who) + ": a padded row of " + std::to_string(PW) + " samples needs " +
@@ -400,7 +400,7 @@ int dic_bspline_prefilter(dic_ctx *ctx, const dic_image *img, int degree, int bo
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "unknown precision");
     const int b = border < 3 ? 3 : border;
     const int PW = img->width + 2 * b, PH = img->height + 2 * b;
-    const int rc0 = check_prefilter_size(ctx, img->width, img->height, b, "B-spline prefilter");
+    const int rc0 = check_prefilter_size(ctx, img->width, img->height, b, precision, "B-spline prefilter");
     if (rc0 != DIC_OK) return rc0;
     dic_coeffs *c = new (std::nothrow) dic_coeffs();
     if (!c) return set_error(ctx, DIC_ERR_HIP, "host allocation failed");

@@ -91,11 +91,11 @@ int get_subset_table(dic_ctx *ctx, int radius, const int **table, int *count);
 
 // bspline.hip
 // Bytes of LDS one workgroup of the exact prefilter's row pass needs for a
-// padded row of PW samples; check_prefilter_size refuses (DIC_ERR_UNSUPPORTED,
+// padded row of PW samples in the given storage precision; check_prefilter_size refuses (DIC_ERR_UNSUPPORTED,
 // with the bytes needed) a frame whose padded rows exceed the device's LDS or
 // whose padded FP64 image reaches 2 GiB.
-size_t prefilter_row_lds(int PW);
-int check_prefilter_size(dic_ctx *ctx, int W, int H, int border, const char *who);
+size_t prefilter_row_lds(int PW, int precision);
+int check_prefilter_size(dic_ctx *ctx, int W, int H, int border, int precision, const char *who);
 hipError_t launch_prefilter(const float *img, int W, int H, int degree, int border, int exact, int precision,
                             void *out, double *tmp_a, double *tmp_b, hipStream_t s);
 hipError_t launch_gradient_central(const float *img, int W, int H, int precision, void *gx, void *gy, hipStream_t s);

@@ -488,7 +488,7 @@ int dic_server_create(dic_ctx *ctx, const dic_server_config *cfg, dic_server **o
     if (c.optimizer != DIC_OPTIMIZER_ICGN && c.optimizer != DIC_OPTIMIZER_FGN)
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "server: unknown optimizer");
     const int b = 3;
-    int rc0 = check_prefilter_size(ctx, c.max_width, c.max_height, b, "server: largest frame");
+    int rc0 = check_prefilter_size(ctx, c.max_width, c.max_height, b, c.precision, "server: largest frame");
     if (rc0 != DIC_OK) return rc0;
     dic_server *s = new (std::nothrow) dic_server;
     if (!s) return set_error(ctx, DIC_ERR_HIP, "host allocation failed");
