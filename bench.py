@@ -170,63 +170,126 @@ def batched_leg(pkg, ctx, synth, reps=3):
     return out
 
 
-def serving_leg(pkg, ctx, synth, duration_s=6.0, rate=20.0, seed=4000):
-    """Poisson request stream (rate req/s for duration_s): each request is an
-    incremental uniaxial-strain sequence of U{8..64} frames at 1024^2 or
-    2448x2048 (50/50) against its resident reference frame; radius 15, first
-    order, POI step 7, quintic, mixed precision; 200 GB admission budget.
-    Requests of one size share one rendered, pinned frame pool (64 frames,
+def _serving_pools(ctx, synth, seed):
+    """Requests of one size share one rendered, pinned frame pool (64 frames,
     strain 2e-4 per frame, noise sigma 1.5)."""
-    t0 = time.time()
     pools = {}
     for (w, h) in ((1024, 1024), (2448, 2048)):
         bufs = [ctx.host_array((h, w), np.float32) for _ in range(65)]
         ref, frames = synth.strain_sequence(w, h, 64, seed=seed + w, out=bufs)
         xy = synth.strain_pois(w, h)
-        pools[(w, h)] = (ref, frames, xy, np.zeros(xy.shape[0], pkg.INITIAL_DTYPE))
-    log(f"serving frame pools rendered in {time.time() - t0:.1f}s")
-    rng = np.random.default_rng(seed)
-    arrivals, t = [], 0.0
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t > duration_s:
-            break
-        size = (1024, 1024) if rng.random() < 0.5 else (2448, 2048)
-        arrivals.append((t, size, int(rng.integers(8, 65))))
-    srv = ctx.server(radius=15, shape_order=1, degree=5, precision=pkg.PRECISION_MIXED, max_active=256,
-                     max_width=2448, max_height=2048, hbm_budget=200 * 10**9)
-    # warm the kernels and allocations with one short request
-    ref, frames, xy, ini = pools[(1024, 1024)]
-    srv.submit(ref, frames[:2], xy, ini).wait(120.0)
+        pools[(w, h)] = (ref, frames, xy)
+    return pools
+
+
+def _serve(pkg, ctx, pools, arrivals, precision, lag, paced, against=None, keep=False):
+    """One pass of the request list through a fresh server.  paced: submit at
+    the arrival times (a live stream); else all at t = 0 (saturation).
+    against: per-request (u, v, ok) of another pass, to count POIs within
+    1e-3 px of it; keep: return this pass's (u, v, ok) per request."""
+    srv = ctx.server(radius=15, shape_order=1, degree=5, precision=precision, max_active=256, max_width=2448,
+                     max_height=2048, hbm_budget=200 * 10**9, guess_lag=lag)
+    ref, frames, xy = pools[(1024, 1024)]
+    srv.submit(ref, frames[:2], xy, np.zeros(xy.shape[0], pkg.INITIAL_DTYPE)).wait(120.0)  # warm
     reqs = []
     start = time.perf_counter()
     for (ta, size, nf) in arrivals:
-        delay = ta - (time.perf_counter() - start)
-        if delay > 0:
-            time.sleep(delay)
-        ref, frames, xy, ini = pools[size]
-        reqs.append(srv.submit(ref, frames[:nf], xy, ini))
-    for r in reqs:
-        r.wait(600.0)
+        if paced:
+            delay = ta - (time.perf_counter() - start)
+            if delay > 0:
+                time.sleep(delay)
+        ref, frames, xy = pools[size]
+        reqs.append(srv.submit(ref, frames[:nf], xy, np.zeros(xy.shape[0], pkg.INITIAL_DTYPE)))
+    frame_lat, req_lat, kept = [], [], []
+    pois = ok_n = conv = 0
+    for i, r in enumerate(reqs):
+        res = r.wait(900.0)
+        frame_lat.append(r.frame_done_s - r.arrival)
+        req_lat.append(r.frame_done_s[-1] - r.arrival)
+        ok = (res["status"] == pkg.SUCCESS) & (res["valid"] == 1)
+        pois += res.size
+        ok_n += int(ok.sum())
+        if against is not None:
+            u, v, ok2 = against[i]
+            d = np.maximum(np.abs(res["p"][..., 0] - u), np.abs(res["p"][..., 1] - v))
+            conv += int((ok & ok2 & (d <= 1e-3)).sum())
+        if keep:
+            kept.append((res["p"][..., 0].copy(), res["p"][..., 1].copy(), ok))
+        r.results = None  # the pass keeps only what it reports
     st = srv.stats()
-    frame_lat = np.concatenate([r.frame_done_s - r.arrival for r in reqs])
-    req_lat = np.array([r.frame_done_s[-1] - r.arrival for r in reqs])
     first = min(r.arrival for r in reqs)
     last = max(r.frame_done_s[-1] for r in reqs)
-    pois = int(sum(r.results.size for r in reqs))
-    ok = int(sum(int(((r.results["status"] == pkg.SUCCESS) & (r.results["valid"] == 1)).sum()) for r in reqs))
     srv.close()
-    out = {"workload": (f"Poisson {rate:g} req/s for {duration_s:g} s, {len(reqs)} requests of 8..64 frames at "
-                        f"1024^2 / 2448x2048, radius 15, first order, step 7, quintic, mixed, 200 GB budget"),
-           "requests": len(reqs), "frames": int(frame_lat.size), "pois": pois, "converged": ok,
-           "span_s": round(last - first, 3), "converged_pois_per_s": round(ok / (last - first), 1),
-           "offered_pois_per_s": round(pois / duration_s, 1),
+    frame_lat = np.concatenate(frame_lat)
+    req_lat = np.array(req_lat)
+    span = last - first
+    out = {"span_s": round(span, 3), "pois": pois, "success": ok_n,
+           "success_pois_per_s": round(ok_n / span, 1),
            "frame_latency_ms": {"p50": round(float(np.percentile(frame_lat, 50)) * 1e3, 2),
                                 "p99": round(float(np.percentile(frame_lat, 99)) * 1e3, 2)},
            "request_latency_ms": {"p50": round(float(np.percentile(req_lat, 50)) * 1e3, 2),
                                   "p99": round(float(np.percentile(req_lat, 99)) * 1e3, 2)},
            "ticks": st["ticks"], "max_active_seen": st["max_active_seen"],
            "peak_hbm_gb": round(st["peak_bytes_held"] / 1e9, 2)}
+    if against is not None:
+        out["converged_vs_other"] = conv
+    return out, kept
+
+
+def serving_leg(pkg, ctx, synth, duration_s=6.0, rate=20.0, seed=4000):
+    """Poisson request stream (rate req/s for duration_s): each request is an
+    incremental uniaxial-strain sequence of U{8..64} frames at 1024^2 or
+    2448x2048 (50/50) against its resident reference frame; radius 15, first
+    order, POI step 7, quintic; 200 GB admission budget.
+
+    Live stream (requests submitted at their arrival times): mixed precision
+    with guess lag K = 1 (frame f from frame f - 1) and K = 4 (frame f from
+    f - 4, up to 4 frames of a request per tick), and the plain FP64 drop-in
+    at K = 4: frame and request latency p50/p99; mixed K = 1 and 4 again at a
+    quarter of the rate.  Saturation (the same
+    requests all submitted at t = 0, K = 4): the FP64 pass, then the mixed
+    pass, whose converged POIs are those within 1e-3 px of FP64; their
+    converged POIs/s ratio is the served mixed-vs-FP64 figure."""
+    t0 = time.time()
+    pools = _serving_pools(ctx, synth, seed)
+    log(f"serving frame pools rendered in {time.time() - t0:.1f}s")
+
+    def poisson(r):
+        rng = np.random.default_rng(seed)
+        out, t = [], 0.0
+        while True:
+            t += rng.exponential(1.0 / r)
+            if t > duration_s:
+                return out
+            size = (1024, 1024) if rng.random() < 0.5 else (2448, 2048)
+            out.append((t, size, int(rng.integers(8, 65))))
+
+    arrivals, light = poisson(rate), poisson(rate / 4)
+    mixed, fp64 = pkg.PRECISION_MIXED, pkg.PRECISION_FP64
+    stream, stream_light = {}, {}
+    for name, prec, lag in (("mixed_lag1", mixed, 1), ("mixed_lag4", mixed, 4), ("fp64_lag4", fp64, 4)):
+        stream[name], _ = _serve(pkg, ctx, pools, arrivals, prec, lag, paced=True)
+        log(f"serving stream {name}: {stream[name]}")
+    for name, lag in (("mixed_lag1", 1), ("mixed_lag4", 4)):
+        stream_light[name], _ = _serve(pkg, ctx, pools, light, mixed, lag, paced=True)
+        log(f"serving light stream {name}: {stream_light[name]}")
+    sat64, ref64 = _serve(pkg, ctx, pools, arrivals, fp64, 4, paced=False, keep=True)
+    log(f"serving saturated fp64: {sat64}")
+    satmx, _ = _serve(pkg, ctx, pools, arrivals, mixed, 4, paced=False, against=ref64)
+    log(f"serving saturated mixed: {satmx}")
+    del ref64
+    mx_rate = satmx["converged_vs_other"] / satmx["span_s"]
+    fp_rate = sat64["success"] / sat64["span_s"]
+    out = {"workload": (f"Poisson {rate:g} req/s for {duration_s:g} s, {len(arrivals)} requests of 8..64 frames at "
+                        f"1024^2 / 2448x2048, radius 15, first order, step 7, quintic, 200 GB budget"),
+           "requests": len(arrivals), "frames": int(sum(a[2] for a in arrivals)), "pois": satmx["pois"],
+           "offered_pois_per_s": round(satmx["pois"] / duration_s, 1),
+           "stream": stream,
+           "stream_quarter_rate": dict(stream_light, requests=len(light), rate_req_per_s=rate / 4),
+           "saturated": {"fp64_lag4": sat64, "mixed_lag4": satmx,
+                         "mixed_converged_pois_per_s": round(mx_rate, 1),
+                         "fp64_converged_pois_per_s": round(fp_rate, 1),
+                         "mixed_vs_fp64": round(mx_rate / fp_rate, 3)}}
     log(f"serving: {out}")
     return out
 
@@ -484,6 +547,77 @@ def seeding_leg(pkg, ctx, wl, reps=3):
     return out
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, fp, mx, radius, degree, fraction=1, reps=5, one_thread_stride=16):
+    """The oracle's FP64 restatement of the reference path on the host cores:
+    precompute (B-spline coefficients + per-pixel local blocks of the deformed
+    frame, reference gradient cache) and IC-GN over the POIs, each timed
+    `reps` times and reported as the median, at the allowed thread count
+    (every `fraction`-th POI; all by default) and at 1 thread (every
+    `one_thread_stride`-th POI, scaled to the full job)."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as orc
+    n = wl.poi_xy.shape[0]
+    nproc = len(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
+
+    def run(nthreads, sel):
+        pre_s, solve_s, res = [], [], None
+        for _ in range(reps):
+            orc.set_threads(nthreads)
+            c0 = time.perf_counter()
+            pre = orc.Precomputed(wl.deformed, degree=degree)
+            gx, gy = orc.reference_gradient_cache(wl.reference)
+            c1 = time.perf_counter()
+            res = orc.icgn_solve(wl.reference, pre, gx, gy, wl.poi_xy[sel], wl.init[sel], radius=radius,
+                                 max_iterations=30, threshold=1e-3, shape_order=2, threads=nthreads)
+            c2 = time.perf_counter()
+            del pre
+            pre_s.append(c1 - c0)
+            solve_s.append(c2 - c1)
+        orc.set_threads(0)
+        return float(np.median(pre_s)), float(np.median(solve_s)), res
+
+    sel = np.arange(0, n, max(1, fraction))
+    pre_m, solve_m, cres = run(threads, sel)
+    sel1 = np.arange(0, n, one_thread_stride)
+    pre_1, solve_1, _ = run(1, sel1)
+    okc = (cres["status"] == 0) & (cres["valid"] == 1)
+    conv = okc.sum() * n / sel.size
+    t_job = pre_m + solve_m * n / sel.size
+    t_job1 = pre_1 + solve_1 * n / sel1.size
+    dgpu = np.maximum(np.abs(fp["p"][sel, 0] - cres["p"][:, 0]), np.abs(fp["p"][sel, 1] - cres["p"][:, 1]))
+    dmx = np.maximum(np.abs(mx["p"][sel, 0] - cres["p"][:, 0]), np.abs(mx["p"][sel, 1] - cres["p"][:, 1]))
+    out = {"value": round(conv / t_job, 1), "unit": "converged POIs/s", "cores": threads, "kind": "port",
+           "sample": (f"oracle FP64 restatement: precompute (coefficients + per-pixel local blocks) of the 2048^2 "
+                      f"frame and IC-GN over " + (f"all {n} POIs" if sel.size == n else
+                                                   f"every {fraction}th POI ({sel.size} of {n}), scaled")
+                      + f" on {threads} threads; median of {reps}"),
+           "cpu_model": _cpu_model(), "nproc": nproc,
+           "median_of": reps,
+           "threads": {"n": threads, "precompute_s": round(pre_m, 4), "solve_s": round(solve_m, 4),
+                       "job_s": round(t_job, 4), "converged_pois_per_s": round(conv / t_job, 1)},
+           "one_thread": {"precompute_s": round(pre_1, 4), "solve_s": round(solve_1, 4), "pois": int(sel1.size),
+                          "job_s_scaled": round(t_job1, 3), "converged_pois_per_s": round(conv / t_job1, 1)},
+           "note": ("the port allocates all per-pixel blocks in one array and prefilters by recursion; the "
+                    "reference allocates one heap matrix per pixel and prefilters by FFT, so this baseline "
+                    "flatters the CPU"),
+           "fp64_gpu_vs_oracle_max_px": float(np.max(dgpu[okc], initial=0.0)),
+           "mixed_gpu_vs_oracle_max_px": float(np.max(dmx[okc], initial=0.0)),
+           "status_agreement": float(np.mean(cres["status"] == fp["status"][sel]))}
+    log(f"cpu baseline: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -537,13 +671,16 @@ def main():
     ctx.synchronize()
 
     def step(name):
+        """Events before the gradient, the prefilter, the solve, and after it."""
         s = state[name]
+        e0 = ctx.record_event()
         ctx.ref_gradient_into(dref, s["grad"])
-        ctx.prefilter_into(ddef, s["coeffs"])
         e1 = ctx.record_event()
-        ctx.solve_device(s["params"], dref, s["grad"], s["coeffs"], dxy.ptr, dini.ptr, n, s["out"].ptr)
+        ctx.prefilter_into(ddef, s["coeffs"])
         e2 = ctx.record_event()
-        return e1, e2
+        ctx.solve_device(s["params"], dref, s["grad"], s["coeffs"], dxy.ptr, dini.ptr, n, s["out"].ptr)
+        e3 = ctx.record_event()
+        return e0, e1, e2, e3
 
     timing = {}
     for name in ("fp64", "mixed"):
@@ -560,14 +697,17 @@ def main():
         ctx.synchronize()
         h1 = time.perf_counter()
         total_ms = ctx.elapsed_ms(es, ee)
-        kern_ms = [ctx.elapsed_ms(a, b) for a, b in kev]
-        for a, b in kev:
-            ctx.destroy_event(a)
-            ctx.destroy_event(b)
+        grad_ms = [ctx.elapsed_ms(e[0], e[1]) for e in kev]
+        pre_ms = [ctx.elapsed_ms(e[1], e[2]) for e in kev]
+        kern_ms = [ctx.elapsed_ms(e[2], e[3]) for e in kev]
+        for e in kev:
+            for x in e:
+                ctx.destroy_event(x)
         ctx.destroy_event(es)
         ctx.destroy_event(ee)
         timing[name] = dict(ms_per_step=total_ms / args.steps, kernel_ms=float(np.mean(kern_ms)),
-                            kernel_ms_min=float(np.min(kern_ms)), host_ms_per_step=(h1 - h0) * 1e3 / args.steps)
+                            kernel_ms_min=float(np.min(kern_ms)), gradient_ms=float(np.mean(grad_ms)),
+                            prefilter_ms=float(np.mean(pre_ms)), host_ms_per_step=(h1 - h0) * 1e3 / args.steps)
         log(f"{name}: {timing[name]}")
 
     res = {k: state[k]["out"].download(pkg.RESULT_DTYPE) for k in state}
@@ -588,44 +728,41 @@ def main():
                 "kernel": f"icgn_kernel<P=12,ZNSSD,quintic,{'f32' if name == 'mixed' else 'f64'}>",
                 "algorithmic_gflop_per_launch": round(fl / 1e9, 3), "kernel_ms": round(timing[name]["kernel_ms"], 4)}
 
+    def memory_phases(name):
+        """The two HBM-bound phases of the step against 8 TB/s, and the step's
+        time-weighted roofline fraction (each phase's fraction weighted by its
+        share of the step; IC-GN counted on its VALU roofline)."""
+        e = 8 if name == "fp64" else 4
+        H, W = wl.reference.shape
+        PW, PH = W + 6, H + 6
+        t = timing[name]
+        grad_bytes = 4 * W * H + 2 * e * W * H                  # image in, two gradient planes out
+        pre_bytes = 4 * W * H + e * PW * PH                     # compulsory: image in, coefficients out
+        pre_moved = (4 * W * H + 3 * 4 * PW * PH if e == 4 else  # image, FP32 intermediate out + in, coefficients
+                     4 * W * H + 4 * 8 * PW * PH + 8 * PW * PH)  # image, two FP64 intermediates out + in, coefficients
+        def frac(b, ms):
+            return b / (ms * 1e-3) / (PEAK_HBM_GBS * 1e9)
+        fg, fp_ = frac(grad_bytes, t["gradient_ms"]), frac(pre_bytes, t["prefilter_ms"])
+        fi = roofline(name, res[name])["frac"]
+        tot = t["gradient_ms"] + t["prefilter_ms"] + t["kernel_ms"]
+        return {
+            "gradient": {"ms": round(t["gradient_ms"], 4), "bytes": grad_bytes, "bound": "hbm",
+                         "achieved_gbs": round(grad_bytes / (t["gradient_ms"] * 1e-3) / 1e9, 1),
+                         "frac": round(fg, 4)},
+            "prefilter": {"ms": round(t["prefilter_ms"], 4), "bound": "hbm", "compulsory_bytes": pre_bytes,
+                          "moved_bytes": pre_moved,
+                          "achieved_gbs": round(pre_bytes / (t["prefilter_ms"] * 1e-3) / 1e9, 1),
+                          "frac": round(fp_, 4),
+                          "kernels": ("prefilter_rows_f32_kernel<24> + prefilter_cols_f32_kernel" if e == 4 else
+                                      "prefilter_rows_kernel + prefilter_cols_causal/anticausal_kernel")},
+            "icgn": {"ms": round(t["kernel_ms"], 4), "bound": "valu", "frac": fi},
+            "time_weighted_frac": round((fg * t["gradient_ms"] + fp_ * t["prefilter_ms"] + fi * t["kernel_ms"]) / tot,
+                                        4)}
+
     value = okmx.sum() / (timing["mixed"]["ms_per_step"] * 1e-3)
     value64 = ok64.sum() / (timing["fp64"]["ms_per_step"] * 1e-3)
 
-    cpu = None
-    if not args.no_cpu:
-        sys.path.insert(0, ROOT)
-        from oracle import oracle as orc
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
-        sel = np.arange(0, n, max(1, args.cpu_fraction))
-        c0 = time.perf_counter()
-        pre = orc.Precomputed(wl.deformed, degree=degree)
-        gx, gy = orc.reference_gradient_cache(wl.reference)
-        c1 = time.perf_counter()
-        cres = orc.icgn_solve(wl.reference, pre, gx, gy, wl.poi_xy[sel], wl.init[sel], radius=radius,
-                              max_iterations=30, threshold=1e-3, shape_order=2, threads=threads)
-        c2 = time.perf_counter()
-        one = sel[:: max(1, sel.size // 400)]
-        s0 = time.perf_counter()
-        orc.icgn_solve(wl.reference, pre, gx, gy, wl.poi_xy[one], wl.init[one], radius=radius, max_iterations=30,
-                       threshold=1e-3, shape_order=2, threads=1)
-        s1 = time.perf_counter()
-        t_job = (c1 - c0) + (c2 - c1) * n / sel.size
-        okc = (cres["status"] == 0) & (cres["valid"] == 1)
-        dgpu = np.maximum(np.abs(fp["p"][sel, 0] - cres["p"][:, 0]), np.abs(fp["p"][sel, 1] - cres["p"][:, 1]))
-        dmx = np.maximum(np.abs(mx["p"][sel, 0] - cres["p"][:, 0]), np.abs(mx["p"][sel, 1] - cres["p"][:, 1]))
-        cpu = {"value": round(okc.sum() * n / sel.size / t_job, 1), "unit": "converged POIs/s", "cores": threads,
-               "kind": "port",
-               "sample": (f"oracle FP64 restatement: full precompute (coefficients + per-pixel local blocks) of the "
-                          f"2048^2 frame, then " + (f"all {n} POIs" if sel.size == n else
-                                                    f"every {args.cpu_fraction}th POI ({sel.size} of {n})")
-                          + f" on {threads} threads" + ("" if sel.size == n else ", scaled to the full job")),
-               "precompute_s": round(c1 - c0, 3), "solve_sample_s": round(c2 - c1, 3),
-               "single_thread_pois_per_s": round(one.size / (s1 - s0), 1),
-               "fp64_gpu_vs_oracle_max_px": float(np.max(dgpu[okc], initial=0.0)),
-               "mixed_gpu_vs_oracle_max_px": float(np.max(dmx[okc], initial=0.0)),
-               "status_agreement": float(np.mean(cres["status"] == fp["status"][sel]))}
-        log(f"cpu baseline: {cpu}")
-
+    cpu = None if args.no_cpu else cpu_baseline(wl, fp, mx, radius, degree, args.cpu_fraction)
     batched = None if args.no_batched else batched_leg(pkg, ctx, synth)
     serving = None if args.no_serving else serving_leg(pkg, ctx, synth, duration_s=args.serving_seconds)
     stereo = None if args.no_stereo else stereo_leg(pkg, ctx, synth)
@@ -654,12 +791,19 @@ def main():
         "hit_max_iterations": int(np.sum(mx["iterations"] >= 30)),
         "mean_iterations": round(float(mx["iterations"].mean()), 3),
         "roofline": roofline("mixed", mx),
+        "step_phases": memory_phases("mixed"),
         "fp64": {"value": round(float(value64), 1), "ms_per_step": round(timing["fp64"]["ms_per_step"], 4),
                  "converged": int(ok64.sum()), "roofline": roofline("fp64", fp),
+                 "step_phases": memory_phases("fp64"),
                  "rms_error_vs_truth_px": round(float(err_truth), 5)},
-        "request_latency_ms": (dict(serving["request_latency_ms"], source="serving leg")
-                               if serving else None),
+        "request_latency_ms": (dict(serving["stream"]["mixed_lag4"]["request_latency_ms"],
+                                    source="serving leg, live stream, mixed, guess lag 4") if serving else None),
         "mixed_vs_fp64_speedup": round(float(value / value64), 3),
+        "mixed_vs_fp64_by_workload": {
+            "large_field_2048": round(float(value / value64), 3),
+            "batched_64_pairs": batched["vs_fp64_drop_in"] if batched else None,
+            "serving_saturated": serving["saturated"]["mixed_vs_fp64"] if serving else None,
+            "stereo_2448x2048": (round(stereo["value"] / stereo["fp64_value"], 3) if stereo else None)},
         "mixed_vs_fp64_max_px": float(np.max(dmix[ok64], initial=0.0)),
         "cpu_baseline": cpu,
         "batched": batched,

@@ -278,19 +278,24 @@ void dic_host_free(void *ptr);
  *
  * A request is an incremental frame sequence against one reference frame:
  * frame f is correlated with the reference at the request's POIs, starting
- * from the request's initial guesses (f = 0) or from frame f-1's results
- * (u, v, gradients; confidence = correlation).  dic_server_submit returns at
- * once; a worker thread admits pending requests in arrival order while the
- * number of active requests stays below max_active and the device bytes they
- * hold (reference, gradients, two frame and coefficient buffers, POI,
- * initial-guess and result arrays) stay within hbm_budget.  Each tick is one
- * launch over the next frame of every active request; a request that arrives
- * while a tick runs joins the next tick, and a finished request leaves and
- * frees its slot.  Two ticks are in flight: uploads and prefilters of tick
- * k+1 (copy engines, prep stream) overlap the solve of tick k.  Each POI's
- * result depends only on its own inputs, so a request's results are the same
- * served alone or in any mix.  The caller keeps every array of a request
- * valid until dic_server_wait reports it done. ---- */
+ * from the request's initial guesses (f < K) or from frame f-K's results
+ * (u, v, gradients; confidence = correlation), K = guess_lag.  dic_server_submit
+ * returns at once; a worker thread admits pending requests in arrival order
+ * while the number of active requests stays below max_active and the device
+ * bytes they hold (reference, gradients, two frame buffers, 3K coefficient,
+ * guess and result buffers, POIs, the request's guesses) stay within
+ * hbm_budget.  Each tick is one launch over the next frame of every active
+ * request and, while the tick holds fewer than tick_pois POIs, up to K - 1
+ * more of each; a request that arrives while a tick runs joins the next tick,
+ * and a finished request leaves and frees its slot.  Three ticks are in
+ * flight: uploads and prefilters of the next ticks (copy engines, prep
+ * stream) and the copy-out of the previous one overlap the solve of the
+ * current one.  Each POI's result depends only on its own inputs and the
+ * frame its guesses come from, which K alone fixes, so a request's results
+ * are the same served alone or in any mix.  The caller keeps every array of
+ * a request valid until dic_server_wait reports it done. ---- */
+#define DIC_SERVER_MAX_GUESS_LAG 16
+#define DIC_SERVER_DEFAULT_TICK_POIS (1 << 21)
 typedef struct dic_server_config {
     uint64_t hbm_budget;          /* device bytes held by admitted requests (0 = no cap) */
     int32_t max_active;           /* requests served at once */
@@ -303,6 +308,14 @@ typedef struct dic_server_config {
     int32_t precision;            /* DIC_PRECISION_* */
     int32_t objective;            /* DIC_OBJECTIVE_* */
     int32_t optimizer;            /* DIC_OPTIMIZER_*: IC-GN, or FGN (ForwardGaussNewtonSolver) */
+    int32_t guess_lag;            /* K in [0, DIC_SERVER_MAX_GUESS_LAG] (0 means 1): frame f starts from
+                                   * frame f - K's results, frames below K from the request's guesses;
+                                   * up to K frames of one request share a tick.  The reference's own
+                                   * sequence workflow correlates every frame against the reference
+                                   * (python/traditional_dic/workflows/subset_2d.py:152-227). */
+    int32_t tick_pois;            /* a tick takes the next frame of every active request, then more
+                                   * frames (up to K per request, arrival order) while it holds fewer
+                                   * POIs than this (0: DIC_SERVER_DEFAULT_TICK_POIS) */
 } dic_server_config;
 
 typedef struct dic_request {

@@ -287,6 +287,13 @@ int orc_precompute(const float *img, int W, int H, int d, int border, int exact,
     return 0;
 }
 
+void orc_set_threads(int n)
+{
+    static int default_threads = 0; /* the calling thread's count before the first call */
+    if (default_threads == 0) default_threads = omp_get_max_threads();
+    omp_set_num_threads(n > 0 ? n : default_threads);
+}
+
 void orc_precomputed_free(orc_precomputed *pre)
 {
     free(pre->coeffs);

@@ -80,6 +80,10 @@ int orc_form_coefficients(const float *image, int width, int height, int degree,
 int orc_precompute(const float *image, int width, int height, int degree, int border,
                    int exact, int with_blocks, orc_precomputed *out);
 void orc_precomputed_free(orc_precomputed *pre);
+/* Threads of the OpenMP regions without an explicit thread count (the
+ * per-pixel block build of orc_precompute, orc_value_batch) started from the
+ * calling thread; n <= 0 restores the OpenMP default. */
+void orc_set_threads(int n);
 double orc_value(const orc_precomputed *pre, double x, double y);                 /* :458-474 */
 void orc_gradient(const orc_precomputed *pre, double x, double y, double *gxgy);  /* :476-500 */
 void orc_value_batch(const orc_precomputed *pre, const double *xy, int n, double *val, double *grad);

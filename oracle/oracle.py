@@ -115,6 +115,7 @@ def load():
     lib.orc_precompute.argtypes = [vp, ci, ci, ci, ci, ci, ci, ctypes.POINTER(OrcPrecomputed)]
     lib.orc_precompute.restype = ci
     lib.orc_precomputed_free.argtypes = [ctypes.POINTER(OrcPrecomputed)]
+    lib.orc_set_threads.argtypes = [ci]
     lib.orc_value.argtypes = [ctypes.POINTER(OrcPrecomputed), cd, cd]
     lib.orc_value.restype = cd
     lib.orc_value_batch.argtypes = [ctypes.POINTER(OrcPrecomputed), vp, ci, vp, vp]
@@ -231,6 +232,12 @@ class Precomputed:
         grad = np.empty((n, 2))
         load().orc_value_batch(ctypes.byref(self.s), _p(xy), n, _p(val), _p(grad))
         return val, grad
+
+
+def set_threads(n: int) -> None:
+    """Threads of the per-pixel block build (Precomputed) and value_batch
+    regions started from this thread; n <= 0 restores the default."""
+    load().orc_set_threads(int(n))
 
 
 def reference_gradient_cache(img: np.ndarray):

@@ -3,8 +3,9 @@ sequences with HBM-budget admission.
 
 Each POI's result depends only on its own inputs, so a request's results
 must be bit-identical served alone and served in a mix of requests that
-arrive while it runs.  Frame f's guesses are frame f-1's results, so the
-served FP64 results are checked against the same chain of direct
+arrive while it runs.  Frame f's guesses are frame f-K's results (K the
+guess lag; frames below K start from the request's guesses), so the served
+FP64 results are checked against the same chain of direct
 dic_icgn_solve_batch calls (identical statuses, |du|, |dv| <= 1e-9 px) and
 frame 0 against the CPU oracle (|du|, |dv| <= 1e-5 px).
 """
@@ -29,36 +30,46 @@ def seqs(synth, pkg):
     return out
 
 
-def _direct_chain(pkg, ctx, seq, precision, optimizer="icgn", objective="znssd", order=1):
+def _guesses_from(pkg, r):
+    g = np.zeros(r.shape[0], pkg.INITIAL_DTYPE)
+    for k, name in enumerate(("u", "v", "du_dx", "du_dy", "dv_dx", "dv_dy")):
+        g[name] = r["p"][:, k]
+    g["confidence"] = r["correlation"]
+    return g
+
+
+def _direct_chain(pkg, ctx, seq, precision, optimizer="icgn", objective="znssd", order=1, lag=1):
+    """Frame f solved directly from frame f - lag's results (the request's
+    guesses below lag)."""
     ref, frames, xy, ini = seq
     cfg = pkg.SubsetConfig(subset_radius=15, optimizer=optimizer, objective=objective,
                            shape_function="second_order" if order == 2 else "first_order")
     params = pkg.make_params(cfg, precision)
     dref = ctx.upload_image(ref)
     grad = ctx.ref_gradient(dref, 0, 5, 3, precision)
-    res, cur = [], ini.copy()
-    for f in frames:
-        co = ctx.prefilter(ctx.upload_image(f), 5, 3, True, precision)
-        r = ctx.solve(params, dref, grad, co, xy, cur)
-        res.append(r)
-        cur = np.zeros(xy.shape[0], pkg.INITIAL_DTYPE)
-        for k, name in enumerate(("u", "v", "du_dx", "du_dy", "dv_dx", "dv_dy")):
-            cur[name] = r["p"][:, k]
-        cur["confidence"] = r["correlation"]
+    res = []
+    for f, frame in enumerate(frames):
+        co = ctx.prefilter(ctx.upload_image(frame), 5, 3, True, precision)
+        cur = ini if f < lag else _guesses_from(pkg, res[f - lag])
+        res.append(ctx.solve(params, dref, grad, co, xy, cur))
     return np.stack(res)
 
 
-def test_served_fp64_matches_direct_chain_and_oracle(pkg, ctx, orc, seqs):
+@pytest.mark.parametrize("lag", [1, 4])
+def test_served_fp64_matches_direct_chain_and_oracle(pkg, ctx, orc, seqs, lag):
     srv = ctx.server(radius=15, shape_order=1, degree=5, precision=pkg.PRECISION_FP64, max_active=8,
-                     max_width=512, max_height=512)
+                     max_width=512, max_height=512, guess_lag=lag)
     req = srv.submit(*seqs[0])
     got = req.wait(60.0)
+    st = srv.stats()
     srv.close()
-    want = _direct_chain(pkg, ctx, seqs[0], pkg.PRECISION_FP64)
+    n_frames = len(seqs[0][1])
+    assert st["ticks"] == -(-n_frames // lag)  # K frames of the one request share a tick
+    want = _direct_chain(pkg, ctx, seqs[0], pkg.PRECISION_FP64, lag=lag)
     assert np.array_equal(got["status"], want["status"])
     ok = want["status"] == 0
     d = np.abs(got["p"][..., :2] - want["p"][..., :2])[ok].max(initial=0)
-    _report(f"served vs direct chain: {ok.mean():.4f} ok, max |du| {d:.2e}")
+    _report(f"served (guess lag {lag}) vs direct chain: {ok.mean():.4f} ok, max |du| {d:.2e}, {st['ticks']} ticks")
     assert d <= 1e-9 and ok.mean() > 0.98
     ref, frames, xy, ini = seqs[0]
     pre = orc.Precomputed(frames[0], degree=5)
@@ -110,9 +121,10 @@ def test_served_other_configs_match_direct_chain(pkg, ctx, seqs, objective, orde
     assert got.tobytes() == want.tobytes()
 
 
-def test_alone_and_in_mix_are_identical(pkg, ctx, seqs):
+@pytest.mark.parametrize("lag", [1, 4])
+def test_alone_and_in_mix_are_identical(pkg, ctx, seqs, lag):
     srv = ctx.server(radius=15, shape_order=1, degree=5, precision=pkg.PRECISION_MIXED, max_active=8,
-                     max_width=512, max_height=512)
+                     max_width=512, max_height=512, guess_lag=lag)
     alone = srv.submit(*seqs[0]).wait(60.0).copy()
     # the same request again, in a mix: others are submitted before and after it
     r1 = srv.submit(*seqs[1])
@@ -129,9 +141,22 @@ def test_alone_and_in_mix_are_identical(pkg, ctx, seqs):
     srv.close()
 
 
+def _slab_bytes(w, h, n, elem=4, lag=1):
+    """Device bytes a request holds: reference, two frame buffers and two
+    gradient planes, POIs and the request's guesses, and 3K coefficient,
+    guess and result buffers (three ticks in flight, K = the guess lag);
+    each array rounded up to 256 bytes."""
+    def a(v):
+        return (v + 255) // 256 * 256
+    px, cpx = w * h, (w + 6) * (h + 6)
+    return (3 * a(4 * px) + 2 * a(elem * px) + a(16 * n) + a(56 * n)
+            + 3 * lag * (a(elem * cpx) + a(56 * n) + a(136 * n)))
+
+
 def test_budget_admission_and_refusal(pkg, ctx, seqs):
     # room for one 256 x 256 request at a time: requests are served one after another
-    one = 256 * 256 * (3 * 4 + 2 * 4) + 2 * 262 * 262 * 4 + 1000 * (16 + 56 + 2 * 136) + 64 * 1024
+    assert seqs[0][2].shape[0] == seqs[2][2].shape[0]
+    one = _slab_bytes(256, 256, seqs[0][2].shape[0])
     srv = ctx.server(radius=15, precision=pkg.PRECISION_MIXED, max_active=8, max_width=512, max_height=512,
                      hbm_budget=one + 64 * 1024)
     reqs = [srv.submit(*seqs[0]), srv.submit(*seqs[2])]
@@ -152,3 +177,6 @@ def test_server_rejects_bad_config(pkg, ctx):
         ctx.server(shape_order=3)
     with pytest.raises(pkg.DicError):
         ctx.server(max_active=0)
+    for lag in (-1, 17):
+        with pytest.raises(pkg.DicError, match="guess_lag"):
+            ctx.server(guess_lag=lag)

@@ -1,8 +1,11 @@
 // server.hip -- the serving loop (dic_server_*): admission by HBM budget and
-// slots, continuous batching of one frame per active request per tick, two
-// ticks in flight over three streams (prep: uploads + prefilters, compute:
-// initial guesses + IC-GN, copy: results back to pinned host memory).
+// slots, continuous batching of up to K frames per active request per tick
+// (K = the guess lag: frame f starts from frame f - K's results, frames below
+// K from the request's guesses), three ticks in flight over three streams
+// (prep: uploads + prefilters, compute: initial guesses + IC-GN, copy: results
+// back to pinned host memory).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -26,17 +29,25 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 // Device and pinned buffers of one request shape (W, H, n); pooled for reuse
 // so the loop never calls hipFree (which would wait for the whole device).
+// Per-frame buffers form rings of kDepth * K slots, frame f in slot
+// f mod (kDepth * K): the kDepth ticks in flight hold at most kDepth * K
+// consecutive frames of a request, frame f's guess source f - K lies in an
+// earlier tick on the same compute stream, and a slot is reused only by a
+// tick built after the one that used it was finalized.
+constexpr int kDepth = 3; // ticks in flight
 struct Slab {
     int W = 0, H = 0, n = 0;
     size_t bytes = 0;       // device bytes
     char *dev = nullptr;    // one allocation, carved below
     float *ref = nullptr, *frame[2] = {nullptr, nullptr};
-    void *gx = nullptr, *gy = nullptr, *coef[2] = {nullptr, nullptr};
+    void *gx = nullptr, *gy = nullptr;
     size_t coef_bytes = 0;
     double *xy = nullptr;
-    dic_initial *init = nullptr;
-    dic_result *out[2] = {nullptr, nullptr};
-    dic_result *pinned[2] = {nullptr, nullptr}; // host, pinned
+    dic_initial *init0 = nullptr;       // the request's guesses (frames below K)
+    std::vector<void *> coef;           // kDepth * K rings
+    std::vector<dic_initial *> init;    // guesses of frame f from frame f - K
+    std::vector<dic_result *> out;
+    std::vector<dic_result *> pinned;   // host, pinned
 };
 
 struct Req {
@@ -62,14 +73,18 @@ struct dic_server {
     dic_ctx *ctx = nullptr;
     dic_server_config cfg{};
     int border = 3, pitch = 0, rows_max = 0;
+    int lag = 1;   // K
+    int ring = kDepth; // kDepth * K
+    long tick_pois = 0;   // extra frames join a tick while it holds fewer POIs
+    int copy_threads = 1; // threads of the result copy-out
     const int *table = nullptr;
     int nfull = 0;
     hipStream_t prep = nullptr, compute = nullptr, copy = nullptr;
-    hipEvent_t prep_done[2] = {nullptr, nullptr}, solved[2] = {nullptr, nullptr};
+    hipEvent_t prep_done[kDepth] = {}, solved[kDepth] = {};
     double *tmp_a = nullptr, *tmp_b = nullptr; // prefilter temporaries (prep stream)
     size_t tmp_bytes = 0;
-    BatchPair *h_pairs[2] = {nullptr, nullptr}, *d_pairs[2] = {nullptr, nullptr};
-    int *h_off[2] = {nullptr, nullptr}, *d_off[2] = {nullptr, nullptr};
+    BatchPair *h_pairs[kDepth] = {}, *d_pairs[kDepth] = {};
+    int *h_off[kDepth] = {}, *d_off[kDepth] = {};
 
     std::thread worker;
     std::mutex mu;
@@ -88,6 +103,7 @@ struct dic_server {
     std::chrono::steady_clock::time_point t0;
 
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+    size_t max_pairs() const { return (size_t)cfg.max_active * (size_t)lag; } // frames in one tick
 };
 
 namespace {
@@ -96,10 +112,10 @@ size_t slab_device_bytes(const dic_server *s, int W, int H, int n)
 {
     const int b = s->border;
     const size_t px = (size_t)W * H, cpx = (size_t)(W + 2 * b) * (H + 2 * b);
-    const size_t e = elem_bytes(s->cfg.precision);
-    return 3 * align256(px * sizeof(float)) + 2 * align256(px * e) + 2 * align256(cpx * e) +
-           align256(16 * (size_t)n) + align256(sizeof(dic_initial) * (size_t)n) +
-           2 * align256(sizeof(dic_result) * (size_t)n);
+    const size_t e = elem_bytes(s->cfg.precision), R = (size_t)s->ring;
+    return 3 * align256(px * sizeof(float)) + 2 * align256(px * e) + align256(16 * (size_t)n) +
+           align256(sizeof(dic_initial) * (size_t)n) +
+           R * (align256(cpx * e) + align256(sizeof(dic_initial) * (size_t)n) + align256(sizeof(dic_result) * (size_t)n));
 }
 
 int fail(dic_server *s, int code, const std::string &msg)
@@ -163,13 +179,18 @@ Slab *take_slab(dic_server *s, int W, int H, int n)
     sl->gx = carve(px * e);
     sl->gy = carve(px * e);
     sl->coef_bytes = cpx * e;
-    sl->coef[0] = carve(cpx * e);
-    sl->coef[1] = carve(cpx * e);
     sl->xy = (double *)carve(16 * (size_t)n);
-    sl->init = (dic_initial *)carve(sizeof(dic_initial) * (size_t)n);
-    sl->out[0] = (dic_result *)carve(sizeof(dic_result) * (size_t)n);
-    sl->out[1] = (dic_result *)carve(sizeof(dic_result) * (size_t)n);
-    for (int k = 0; k < 2; ++k)
+    sl->init0 = (dic_initial *)carve(sizeof(dic_initial) * (size_t)n);
+    sl->coef.assign(s->ring, nullptr);
+    sl->init.assign(s->ring, nullptr);
+    sl->out.assign(s->ring, nullptr);
+    sl->pinned.assign(s->ring, nullptr);
+    for (int k = 0; k < s->ring; ++k) {
+        sl->coef[k] = carve(cpx * e);
+        sl->init[k] = (dic_initial *)carve(sizeof(dic_initial) * (size_t)n);
+        sl->out[k] = (dic_result *)carve(sizeof(dic_result) * (size_t)n);
+    }
+    for (int k = 0; k < s->ring; ++k)
         if (hipHostMalloc((void **)&sl->pinned[k], sizeof(dic_result) * (size_t)std::max(n, 1), 0) != hipSuccess) {
             free_slab(s, sl);
             return nullptr;
@@ -244,7 +265,7 @@ int setup_request(dic_server *s, Req *q)
         rc = hipcheck(s, hipMemcpyAsync(sl->xy, q->r.poi_xy, 16 * (size_t)q->r.n, hipMemcpyHostToDevice, s->prep),
                       "POI upload");
         if (rc == DIC_OK)
-            rc = hipcheck(s, hipMemcpyAsync(sl->init, q->r.init, sizeof(dic_initial) * (size_t)q->r.n,
+            rc = hipcheck(s, hipMemcpyAsync(sl->init0, q->r.init, sizeof(dic_initial) * (size_t)q->r.n,
                                             hipMemcpyHostToDevice, s->prep),
                           "initial-guess upload");
     }
@@ -255,39 +276,66 @@ int build_tick(dic_server *s, Tick &t, int slot)
 {
     t.slot = slot;
     t.frames.clear();
-    for (Req *q : s->active)
-        if (q->next < q->r.n_frames) t.frames.push_back({q, q->next++});
+    // the next frame of every active request, then (arrival order) up to K - 1
+    // more consecutive frames of each while the tick holds fewer than
+    // tick_pois POIs.  At most K frames of a request share a tick, so frame
+    // f's source f - K lies in an earlier one, already ordered before this
+    // one on the compute stream; which tick a frame joins changes no result.
+    long pois = 0;
+    std::vector<char> open(s->active.size(), 0); // took a frame in the previous pass
+    for (size_t i = 0; i < s->active.size(); ++i) {
+        Req *q = s->active[i];
+        if (q->next < q->r.n_frames) {
+            t.frames.push_back({q, q->next++});
+            pois += q->r.n;
+            open[i] = 1;
+        }
+    }
+    for (int j = 1; j < s->lag && pois < s->tick_pois; ++j)
+        for (size_t i = 0; i < s->active.size(); ++i) {
+            Req *q = s->active[i];
+            if (!open[i]) continue;
+            if (pois >= s->tick_pois || q->next >= q->r.n_frames) {
+                open[i] = 0;
+                continue;
+            }
+            t.frames.push_back({q, q->next++});
+            pois += q->r.n;
+        }
     if (t.frames.empty()) return DIC_OK;
-    // prep: frame uploads and prefilters into the frame's buffer slot
+    // prep: frame uploads and prefilters into the frame's ring slot
     for (auto &fr : t.frames) {
         Req *q = fr.first;
         Slab *sl = q->slab;
-        const int f = fr.second, k = f & 1;
+        const int f = fr.second, k = f % s->ring;
         const size_t px = (size_t)q->r.width * q->r.height;
-        int rc = hipcheck(s, hipMemcpyAsync(sl->frame[k], q->r.frames[f], px * sizeof(float), hipMemcpyHostToDevice,
-                                            s->prep),
+        int rc = hipcheck(s, hipMemcpyAsync(sl->frame[f & 1], q->r.frames[f], px * sizeof(float),
+                                            hipMemcpyHostToDevice, s->prep),
                           "frame upload");
         if (rc == DIC_OK)
-            rc = hipcheck(s, launch_prefilter(sl->frame[k], q->r.width, q->r.height, s->cfg.degree, s->border, 1,
-                                              s->cfg.precision, sl->coef[k], s->tmp_a, s->tmp_b, s->prep),
+            rc = hipcheck(s, launch_prefilter(sl->frame[f & 1], q->r.width, q->r.height, s->cfg.degree, s->border,
+                                              1, s->cfg.precision, sl->coef[k], s->tmp_a, s->tmp_b, s->prep),
                           "frame prefilter");
         if (rc != DIC_OK) return rc;
     }
     int rc = hipcheck(s, hipEventRecord(s->prep_done[slot], s->prep), "event");
     if (rc == DIC_OK) rc = hipcheck(s, hipStreamWaitEvent(s->compute, s->prep_done[slot], 0), "event");
     if (rc != DIC_OK) return rc;
-    // compute: guesses from the previous frame, then one launch over all frames
+    // compute: guesses from frame f - K, then one launch over all frames
     BatchPair *hp = s->h_pairs[slot];
     int *ho = s->h_off[slot];
     int total = 0;
     for (size_t k = 0; k < t.frames.size(); ++k) {
         Req *q = t.frames[k].first;
         Slab *sl = q->slab;
-        const int f = t.frames[k].second;
-        if (f > 0) {
-            rc = hipcheck(s, launch_init_from_results(sl->out[(f - 1) & 1], sl->init, q->r.n, s->compute),
-                          "initial guesses from previous frame");
+        const int f = t.frames[k].second, r = f % s->ring;
+        const dic_initial *guess = sl->init0;
+        if (f >= s->lag) {
+            rc = hipcheck(s, launch_init_from_results(sl->out[(f - s->lag) % s->ring], sl->init[r], q->r.n,
+                                                      s->compute),
+                          "initial guesses from an earlier frame");
             if (rc != DIC_OK) return rc;
+            guess = sl->init[r];
         }
         BatchPair bp{};
         IcgnArgs &a = bp.a;
@@ -298,7 +346,7 @@ int build_tick(dic_server *s, Tick &t, int slot)
         a.ref = sl->ref;
         a.gx = sl->gx;
         a.gy = sl->gy;
-        a.coeffs = sl->coef[f & 1];
+        a.coeffs = sl->coef[r];
         a.roi = nullptr;
         a.table = s->table;
         a.nfull = s->nfull;
@@ -309,8 +357,8 @@ int build_tick(dic_server *s, Tick &t, int slot)
         a.min_needed = s->cfg.shape_order == 2 ? 12 : 6;
         a.fgn = s->cfg.optimizer == DIC_OPTIMIZER_FGN ? 1 : 0;
         a.xy = sl->xy;
-        a.init = sl->init;
-        a.out = sl->out[f & 1];
+        a.init = guess;
+        a.out = sl->out[r];
         a.n = q->r.n;
         bp.shape_order = s->cfg.shape_order;
         hp[k] = bp;
@@ -342,23 +390,62 @@ int build_tick(dic_server *s, Tick &t, int slot)
         if (rc != DIC_OK) break;
         Slab *sl = fr.first->slab;
         const int f = fr.second;
-        rc = hipcheck(s, hipMemcpyAsync(sl->pinned[f & 1], sl->out[f & 1], sizeof(dic_result) * fr.first->r.n,
-                                        hipMemcpyDeviceToHost, s->copy),
+        rc = hipcheck(s, hipMemcpyAsync(sl->pinned[f % s->ring], sl->out[f % s->ring],
+                                        sizeof(dic_result) * fr.first->r.n, hipMemcpyDeviceToHost, s->copy),
                       "result download");
     }
     if (rc == DIC_OK) rc = hipcheck(s, hipEventRecord(t.copied, s->copy), "event");
     return rc;
 }
 
+// The tick's results, from the pinned staging slots into the callers' arrays
+// (worker thread, no lock: only the worker touches slabs, and a request's
+// arrays stay valid until it is reported done).  Writes into freshly
+// allocated host arrays fault their pages in and run at a few GB/s per core,
+// so a large tick is split in pieces over up to copy_threads threads.
+void copy_results(dic_server *s, const Tick &t)
+{
+    struct Piece {
+        char *dst;
+        const char *src;
+        size_t bytes;
+    };
+    constexpr size_t kPiece = (size_t)8 << 20;
+    std::vector<Piece> pieces;
+    for (auto &fr : t.frames) {
+        const Req *q = fr.first;
+        const int f = fr.second;
+        const size_t n = (size_t)q->r.n, bytes = sizeof(dic_result) * n;
+        char *dst = reinterpret_cast<char *>(q->r.results + (size_t)f * n);
+        const char *src = reinterpret_cast<const char *>(q->slab->pinned[f % s->ring]);
+        for (size_t o = 0; o < bytes; o += kPiece) pieces.push_back({dst + o, src + o, std::min(kPiece, bytes - o)});
+    }
+    std::atomic<size_t> next{0};
+    auto body = [&] {
+        for (size_t i; (i = next.fetch_add(1)) < pieces.size();) std::memcpy(pieces[i].dst, pieces[i].src, pieces[i].bytes);
+    };
+    std::vector<std::thread> helpers;
+    const size_t want = std::min<size_t>((size_t)s->copy_threads, pieces.size());
+    for (size_t k = 1; k < want; ++k) {
+        try {
+            helpers.emplace_back(body);
+        } catch (...) {
+            break; // fewer helpers; the calling thread copies the rest
+        }
+    }
+    body();
+    for (auto &h : helpers) h.join();
+}
+
 void finalize_tick(dic_server *s, Tick &t)
 {
     if (hipcheck(s, hipEventSynchronize(t.copied), "tick completion") != DIC_OK) return;
+    copy_results(s, t);
     const double now = s->now();
     std::lock_guard<std::mutex> lk(s->mu);
     for (auto &fr : t.frames) {
         Req *q = fr.first;
         const int f = fr.second, n = q->r.n;
-        std::memcpy(q->r.results + (size_t)f * n, q->slab->pinned[f & 1], sizeof(dic_result) * (size_t)n);
         if (q->r.frame_done_s) q->r.frame_done_s[f] = now;
         q->done = f + 1;
         s->st.frames_done += 1;
@@ -379,7 +466,7 @@ void finalize_tick(dic_server *s, Tick &t)
 void worker_main(dic_server *s)
 {
     (void)hipSetDevice(s->ctx->device);
-    Tick ticks[2];
+    Tick ticks[kDepth];
     for (auto &t : ticks)
         if (hipcheck(s, hipEventCreateWithFlags(&t.copied, hipEventDisableTiming), "event") != DIC_OK) return;
     std::deque<int> inflight; // tick slots in launch order
@@ -399,15 +486,15 @@ void worker_main(dic_server *s)
         bool ok = true;
         for (Req *q : admitted) ok = ok && setup_request(s, q) == DIC_OK;
         if (!ok) break;
-        // a new tick while fewer than two are in flight and some frame is waiting
+        // a new tick while fewer than kDepth are in flight and some frame is waiting
         bool waiting = false;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (Req *q : s->active) waiting = waiting || q->next < q->r.n_frames;
         }
-        if (waiting && inflight.size() < 2) {
+        if (waiting && inflight.size() < (size_t)kDepth) {
             const int slot = next_slot;
-            next_slot ^= 1;
+            next_slot = (next_slot + 1) % kDepth;
             if (build_tick(s, ticks[slot], slot) != DIC_OK) break;
             if (!ticks[slot].frames.empty()) inflight.push_back(slot);
             continue;
@@ -439,11 +526,11 @@ void destroy_resources(dic_server *s)
     for (Slab *sl : s->slabs) free_slab(s, sl);
     s->slabs.clear();
     s->pool.clear();
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < kDepth; ++k) {
         if (s->h_pairs[k]) (void)hipHostFree(s->h_pairs[k]);
         if (s->h_off[k]) (void)hipHostFree(s->h_off[k]);
-        if (s->d_pairs[k]) ctx_free(s->ctx, s->d_pairs[k], sizeof(BatchPair) * (size_t)s->cfg.max_active);
-        if (s->d_off[k]) ctx_free(s->ctx, s->d_off[k], sizeof(int) * ((size_t)s->cfg.max_active + 1));
+        if (s->d_pairs[k]) ctx_free(s->ctx, s->d_pairs[k], sizeof(BatchPair) * s->max_pairs());
+        if (s->d_off[k]) ctx_free(s->ctx, s->d_off[k], sizeof(int) * (s->max_pairs() + 1));
         if (s->prep_done[k]) (void)hipEventDestroy(s->prep_done[k]);
         if (s->solved[k]) (void)hipEventDestroy(s->solved[k]);
     }
@@ -487,6 +574,10 @@ int dic_server_create(dic_ctx *ctx, const dic_server_config *cfg, dic_server **o
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "server: unknown objective");
     if (c.optimizer != DIC_OPTIMIZER_ICGN && c.optimizer != DIC_OPTIMIZER_FGN)
         return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "server: unknown optimizer");
+    if (c.tick_pois < 0) return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "server: tick_pois must be >= 0");
+    if (c.guess_lag < 0 || c.guess_lag > DIC_SERVER_MAX_GUESS_LAG)
+        return set_error(ctx, DIC_ERR_INVALID_ARGUMENT,
+                         "server: guess_lag must be in [0, " + std::to_string(DIC_SERVER_MAX_GUESS_LAG) + "]");
     const int b = 3;
     int rc0 = check_prefilter_size(ctx, c.max_width, c.max_height, b, c.precision, "server: largest frame");
     if (rc0 != DIC_OK) return rc0;
@@ -495,6 +586,10 @@ int dic_server_create(dic_ctx *ctx, const dic_server_config *cfg, dic_server **o
     s->ctx = ctx;
     s->cfg = c;
     s->border = b;
+    s->lag = c.guess_lag < 1 ? 1 : c.guess_lag;
+    s->ring = kDepth * s->lag;
+    s->tick_pois = c.tick_pois > 0 ? c.tick_pois : DIC_SERVER_DEFAULT_TICK_POIS;
+    s->copy_threads = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
     s->t0 = std::chrono::steady_clock::now();
     int rc = get_subset_table(ctx, c.subset_radius, &s->table, &s->nfull);
     if (rc == DIC_OK)
@@ -510,11 +605,11 @@ int dic_server_create(dic_ctx *ctx, const dic_server_config *cfg, dic_server **o
     s->tmp_bytes = sizeof(double) * (size_t)(c.max_width + 2 * b) * (c.max_height + 2 * b);
     hc(ctx_malloc(ctx, (void **)&s->tmp_a, s->tmp_bytes), "prefilter temporaries");
     hc(ctx_malloc(ctx, (void **)&s->tmp_b, s->tmp_bytes), "prefilter temporaries");
-    for (int k = 0; k < 2; ++k) {
-        hc(hipHostMalloc((void **)&s->h_pairs[k], sizeof(BatchPair) * (size_t)c.max_active, 0), "pinned descriptors");
-        hc(hipHostMalloc((void **)&s->h_off[k], sizeof(int) * ((size_t)c.max_active + 1), 0), "pinned offsets");
-        hc(ctx_malloc(ctx, (void **)&s->d_pairs[k], sizeof(BatchPair) * (size_t)c.max_active), "descriptors");
-        hc(ctx_malloc(ctx, (void **)&s->d_off[k], sizeof(int) * ((size_t)c.max_active + 1)), "offsets");
+    for (int k = 0; k < kDepth; ++k) {
+        hc(hipHostMalloc((void **)&s->h_pairs[k], sizeof(BatchPair) * s->max_pairs(), 0), "pinned descriptors");
+        hc(hipHostMalloc((void **)&s->h_off[k], sizeof(int) * (s->max_pairs() + 1), 0), "pinned offsets");
+        hc(ctx_malloc(ctx, (void **)&s->d_pairs[k], sizeof(BatchPair) * s->max_pairs()), "descriptors");
+        hc(ctx_malloc(ctx, (void **)&s->d_off[k], sizeof(int) * (s->max_pairs() + 1)), "offsets");
         hc(hipEventCreateWithFlags(&s->prep_done[k], hipEventDisableTiming), "event");
         hc(hipEventCreateWithFlags(&s->solved[k], hipEventDisableTiming), "event");
     }
