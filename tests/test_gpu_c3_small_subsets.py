@@ -13,7 +13,10 @@ stationary point it is heading for.  The test pins that with the oracle:
     before its iteration limit;
   * every POI where the two precisions part (or their statuses differ) is
     one the oracle runs to its iteration limit, one where the oracle's
-    iteration diverges and fails, or one whose oracle
+    iteration diverges and fails (there the FP64 status may differ from the
+    oracle's only if the oracle itself reaches it from a guess 1e-9 px
+    away: such an iteration is chaotic, and the order of FP64 roundings
+    decides where it ends), or one whose oracle
     result lies more than 1e-3 px from the stationary point (the oracle run
     to ||dp|| < 1e-12); mixed lies no farther from that point than twice the
     oracle's own distance plus 1e-3 px.
@@ -73,6 +76,20 @@ def test_small_second_order_subsets_at_full_size(pkg, ctx, orc, synth):
         ran_out = at["iterations"][k] == 30
         if ran_out:
             continue  # no tolerance holds for an iteration that never met its own test
+        if g64["status"][i] != at["status"][k]:
+            # a chaotic POI: the oracle's own status changes when its guess
+            # moves by 1e-9 px; FP64 must reach a status the oracle reaches
+            flips = set()
+            for du, dv in ((1e-9, 0.0), (-1e-9, 0.0), (0.0, 1e-9), (0.0, -1e-9)):
+                pi = ini[[i]].copy()
+                pi["u"] += du
+                pi["v"] += dv
+                flips.add(int(orc.icgn_solve(c.reference, pre, gx, gy, xy[[i]], pi, radius=c.radius,
+                                             max_iterations=30, threshold=1e-3, shape_order=2)["status"][0]))
+            print(f"  POI {i}: FP64 status {g64['status'][i]}, oracle {at['status'][k]} after {at['iterations'][k]} "
+                  f"iterations; oracle statuses with the guess moved by 1e-9 px: {sorted(flips)}", flush=True)
+            assert int(g64["status"][i]) in flips, i
+            continue
         assert g64["status"][i] == at["status"][k], i  # FP64 is the oracle there too
         if at["status"][k] != 0:
             # a diverging iteration (it leaves the image or its deformed norm

@@ -93,155 +93,202 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
         }
     }
 
-    // ---- reference statistics: icgn.cpp:485-530 ----
+    // ---- reference-side sums, once per POI (icgn.cpp:485-536) ----
+    // Column walk of the circular subset: lane l takes subset column
+    // dx = l - r + 64 g (column group g; one group for r <= 31) and runs down
+    // it from the centre row, the samples at dy = +t and -t together (the disc
+    // is symmetric about its centre row).  Every sum the solver needs is a
+    // moment sum_{dx,dy} Q dx^p dy^q: the Hessian's with Q in {gx^2, gx gy, gy^2},
+    // p + q <= 4 (P = 12) or 2, and the FP32 kernels' one-pass sums with Q in
+    // {gx, gy, (f - mean) gx, (f - mean) gy}, p + q <= 2 or 1.  A lane forms the
+    // column sums C_q = sum_dy Q dy^q (Q(+t) +- Q(-t) for even / odd q), the
+    // moments are the lane sums of dx^p C_q, reduced a few dozen at a time by
+    // wave_reduce_store (one walk for the Hessian moments and the norm, one for
+    // the one-pass sums, so that few column sums are live at once).  FP64 in
+    // both precisions: at small radii
+    // these moments span many orders of magnitude (dx^4 against 1), and FP32
+    // lane sums put errors into the ill-conditioned Hessian directions.
+    constexpr int QH = (P == 12) ? 4 : 2, QG = (P == 12) ? 2 : 1; // largest p + q
+    constexpr int NH = (QH + 1) * (QH + 2) / 2, NG = (QG + 1) * (QG + 2) / 2;
+    constexpr bool REFSUMS = ZNSSD && !TWOPASS; // the FP32 kernels' one-pass ZNSSD sums
+    // reduced sums in mred: [moments of gx^2 | norm | gx gy | gy^2 | sum gx, gy moments | sum (f - mean) |
+    // sum (f - mean) gx, gy moments]
+    constexpr int kNorm = NH, kRefG = 3 * NH + 1, kSumF = kRefG + 2 * NG, kRefF = kSumF + 1;
+    auto hmom = [](int T, int k) { return T * NH + (T > 0 ? 1 : 0) + k; };
+    const int ngroups = (2 * a.radius + 64) >> 6;
+    // column of lane `lane` in group g and its half height h (-1: no column)
+    auto column = [&](int g, int &dx, int &h) {
+        dx = lane + 64 * g - a.radius;
+        const int rem = a.radius * a.radius - dx * dx;
+        int hh = rem < 0 ? -1 : (int)sqrtf((float)rem);
+        if (rem >= 0 && (hh + 1) * (hh + 1) <= rem) ++hh;
+        if (rem >= 0 && hh * hh > rem) --hh;
+        h = hh;
+    };
+    // moment index k = d (d + 1) / 2 + q of the monomial dx^p dy^q, d = p + q
+    auto mono_pq = [](int k, int &p, int &q) {
+        int d = 0;
+        while ((d + 1) * (d + 2) / 2 <= k) ++d;
+        q = k - d * (d + 1) / 2;
+        p = d - q;
+    };
+    double *mred = sh.Hinv; // reduced moments, until the solve operator is formed
     if (!early) {
-        double fsum_l = 0.0;
+        double fs = 0.0;
         int cnt = 0;
-        for (int s = lane; s < a.nfull; s += 64) {
-            int dx, dy;
-            decode(a.table[s], dx, dy);
-            const int x = cx + dx, y = cy + dy;
-            if (!usable(x, y)) continue;
-            fsum_l += (double)a.ref[(size_t)y * W + x];
-            ++cnt;
+        for (int g = 0; g < ngroups; ++g) {
+            int dx, h;
+            column(g, dx, h);
+            const int x = cx + dx;
+            for (int t = 0; t <= h; ++t) {
+                if (usable(x, cy + t)) {
+                    fs += (double)a.ref[(size_t)(cy + t) * W + x];
+                    ++cnt;
+                }
+                if (t > 0 && usable(x, cy - t)) {
+                    fs += (double)a.ref[(size_t)(cy - t) * W + x];
+                    ++cnt;
+                }
+            }
         }
-        const int ns = wave_sum_i(cnt);
+        const int ns = MASKED ? wave_sum_i(cnt) : a.nfull;
         if (ns < a.min_needed) {
             early = true;
-        } else if constexpr (ZNSSD) {
-            const double fmean = wave_sum(fsum_l) / (double)ns;
-            double acc = 0.0;
-            for (int s = lane; s < a.nfull; s += 64) {
-                int dx, dy;
-                decode(a.table[s], dx, dy);
-                const int x = cx + dx, y = cy + dy;
-                if (!usable(x, y)) continue;
-                const double df = (double)a.ref[(size_t)y * W + x] - fmean;
-                acc += df * df;
+        } else {
+            const double fmean = ZNSSD ? wave_sum(fs) / (double)ns : 0.0;
+            // column walk: calls visit(t, usable at +t, at -t, pixel offsets) per row pair
+            auto walk = [&](int x, int h, auto &&visit) {
+                for (int t = 0; t <= h; ++t) {
+                    const bool u1 = usable(x, cy + t), u2 = t > 0 && usable(x, cy - t);
+                    visit((double)t, u1, u2, (size_t)(cy + t) * W + x, (size_t)(cy - t) * W + x);
+                }
+            };
+            // c_q += (Q(+t) + Q(-t)) t^q for even q, (Q(+t) - Q(-t)) t^q for odd q
+            auto acc = [](double *c, auto qmax, double t1, double q1, double q2) {
+                const double sv = q1 + q2, dv = q1 - q2, t2 = t1 * t1;
+                c[0] += sv;
+                c[1] += dv * t1;
+                if constexpr (decltype(qmax)::value >= 2) c[2] += sv * t2;
+                if constexpr (decltype(qmax)::value >= 3) c[3] += dv * (t2 * t1);
+                if constexpr (decltype(qmax)::value >= 4) c[4] += sv * (t2 * t2);
+            };
+            constexpr std::integral_constant<int, QH> qh{};
+            constexpr std::integral_constant<int, QG> qg{};
+            for (int g = 0; g < ngroups; ++g) {
+                int dx, h;
+                column(g, dx, h);
+                const int x = cx + dx;
+                const double d1 = (double)dx, d2 = d1 * d1;
+                auto dxp = [&](int p) { return p == 0 ? 1.0 : (p == 1 ? d1 : (p == 2 ? d2 : (p == 3 ? d2 * d1 : d2 * d2))); };
+                // moment k of column sums c: sum over lanes of dx^p c[q]
+                auto moment = [&](const double *c, int k) {
+                    int p, q;
+                    mono_pq(k, p, q);
+                    return dxp(p) * c[q];
+                };
+                {   // Hessian moments (and the norm), one reduction per gradient product
+                    double cH[3][QH + 1], n2 = 0.0;
+#pragma unroll
+                    for (int q = 0; q <= QH; ++q) cH[0][q] = cH[1][q] = cH[2][q] = 0.0;
+                    walk(x, h, [&](double t1, bool u1, bool u2, size_t o1, size_t o2) {
+                        double gx1 = 0.0, gy1 = 0.0, gx2 = 0.0, gy2 = 0.0;
+                        if (u1) {
+                            gx1 = (double)gxp[o1];
+                            gy1 = (double)gyp[o1];
+                            if constexpr (ZNSSD) {
+                                const double f = (double)a.ref[o1] - fmean;
+                                n2 += f * f;
+                            }
+                        }
+                        if (u2) {
+                            gx2 = (double)gxp[o2];
+                            gy2 = (double)gyp[o2];
+                            if constexpr (ZNSSD) {
+                                const double f = (double)a.ref[o2] - fmean;
+                                n2 += f * f;
+                            }
+                        }
+                        acc(cH[0], qh, t1, gx1 * gx1, gx2 * gx2);
+                        acc(cH[1], qh, t1, gx1 * gy1, gx2 * gy2);
+                        acc(cH[2], qh, t1, gy1 * gy1, gy2 * gy2);
+                    });
+                    wave_reduce_store<NH + 1>([&](int i) { return i < NH ? moment(cH[0], i) : n2; }, lane, mred,
+                                              g > 0);
+                    wave_reduce_store<NH>([&](int i) { return moment(cH[1], i); }, lane, mred + NH + 1, g > 0);
+                    wave_reduce_store<NH>([&](int i) { return moment(cH[2], i); }, lane, mred + 2 * NH + 1, g > 0);
+                }
+                if constexpr (REFSUMS) { // sum sd, sum (f - mean) sd, sum (f - mean)
+                    double cG[2][QG + 1], cF[2][QG + 1], sf = 0.0;
+#pragma unroll
+                    for (int q = 0; q <= QG; ++q) cG[0][q] = cG[1][q] = cF[0][q] = cF[1][q] = 0.0;
+                    walk(x, h, [&](double t1, bool u1, bool u2, size_t o1, size_t o2) {
+                        double gx1 = 0.0, gy1 = 0.0, f1 = 0.0, gx2 = 0.0, gy2 = 0.0, f2 = 0.0;
+                        if (u1) {
+                            gx1 = (double)gxp[o1];
+                            gy1 = (double)gyp[o1];
+                            f1 = (double)a.ref[o1] - fmean;
+                        }
+                        if (u2) {
+                            gx2 = (double)gxp[o2];
+                            gy2 = (double)gyp[o2];
+                            f2 = (double)a.ref[o2] - fmean;
+                        }
+                        acc(cG[0], qg, t1, gx1, gx2);
+                        acc(cG[1], qg, t1, gy1, gy2);
+                        acc(cF[0], qg, t1, f1 * gx1, f2 * gx2);
+                        acc(cF[1], qg, t1, f1 * gy1, f2 * gy2);
+                        sf += f1 + f2;
+                    });
+                    wave_reduce_store<2 * NG + 1>(
+                        [&](int i) { return i < NG ? moment(cG[0], i) : (i < 2 * NG ? moment(cG[1], i - NG) : sf); },
+                        lane, mred + kRefG, g > 0);
+                    wave_reduce_store<2 * NG>(
+                        [&](int i) { return i < NG ? moment(cF[0], i) : moment(cF[1], i - NG); }, lane,
+                        mred + kRefF, g > 0);
+                }
             }
-            const double fnorm = sqrt(wave_sum(acc));
-            if (fnorm <= kEps) {
-                early = true;
-                status = DIC_NUMERICAL_FAILURE;
+            WAVE_SYNC();
+            double fnorm = 1.0;
+            if constexpr (ZNSSD) {
+                fnorm = sqrt(mred[kNorm]);
+                if (fnorm <= kEps) {
+                    early = true;
+                    status = DIC_NUMERICAL_FAILURE;
+                }
             }
             if (lane == 0) {
                 sh.fmean = fmean;
                 sh.fnorm = fnorm;
+                sh.ns = ns;
             }
-        } else {
-            if (lane == 0) {
-                sh.fmean = 0.0;
-                sh.fnorm = 1.0;
+            if (!early) {
+                // H = c * sum sd sd^T (icgn.cpp:532-536): entry (a, b) is c_a c_b times
+                // the moment of g_t(a) g_t(b) dx^(p_a + p_b) dy^(q_a + q_b)
+                const double hscale = ZNSSD ? 2.0 / (fnorm * fnorm) : 1.0;
+                for (int e = lane; e < P * P; e += 64) {
+                    const int ia = e / P, ib = e % P;
+                    const ParamMonomial ma = param_monomial(ia), mb = param_monomial(ib);
+                    const int qq = ma.q + mb.q, d = ma.p + mb.p + qq;
+                    sh.H[e] = hscale * sd_half(ia) * sd_half(ib) * mred[hmom(ma.t + mb.t, d * (d + 1) / 2 + qq)];
+                }
+                if constexpr (REFSUMS) {
+                    const double fin = 1.0 / fnorm;
+                    if (lane < P) {
+                        const ParamMonomial m = param_monomial(lane);
+                        const int d = m.p + m.q, k = d * (d + 1) / 2 + m.q;
+                        sh.ssd[lane] = sd_half(lane) * mred[kRefG + m.t * NG + k];
+                        sh.sfsd[lane] = sd_half(lane) * fin * mred[kRefF + m.t * NG + k];
+                    }
+                    if (lane == 0) {
+                        sh.sfh = mred[kSumF] * fin;
+                        sh.sfh2 = mred[kNorm] * (fin * fin);
+                    }
+                }
             }
         }
-        if (lane == 0) sh.ns = ns;
     }
 
-    // ---- Hessian H = c * sum sd sd^T (icgn.cpp:532-536), two rows at a time
-    // for P = 12; the one-pass variant also needs sum sd, sum fhat sd,
-    // sum fhat, sum fhat^2 ----
     if (!early) {
-        WAVE_SYNC();
-        const double fmean = sh.fmean, fnorm = sh.fnorm;
-        const double hscale = ZNSSD ? 2.0 / (fnorm * fnorm) : 1.0;
-        if constexpr (P == 6) {
-#pragma unroll
-            for (int r0 = 0; r0 < P; r0 += 6) {
-                Acc<double> acc[6][P]; // FP64 in both precisions
-                for (int s = lane; s < a.nfull; s += 64) {
-                    int dx, dy;
-                    decode(a.table[s], dx, dy);
-                    const int x = cx + dx, y = cy + dy;
-                    if (!usable(x, y)) continue;
-                    double sd[P];
-                    steepest_descent<P, double>(gxp[(size_t)y * W + x], gyp[(size_t)y * W + x], (double)dx,
-                                                (double)dy, sd);
-#pragma unroll
-                    for (int i = 0; i < 6; ++i)
-#pragma unroll
-                        for (int j = i; j < P; ++j) acc[i][j].add(sd[i] * sd[j]);
-                }
-#pragma unroll
-                for (int i = 0; i < 6; ++i)
-#pragma unroll
-                    for (int j = i; j < P; ++j) {
-                        const double h = wave_sum(acc[i][j].value()) * hscale;
-                        if (lane == 0) {
-                            sh.H[i * P + j] = h;
-                            sh.H[j * P + i] = h;
-                        }
-                    }
-            }
-        } else {
-            // Second order: sd_a = g_t(a) * c_a dx^p_a dy^q_a, so every Hessian
-            // entry is c_a c_b * sum G_T dx^p dy^q with T in {gx^2, gx gy, gy^2}
-            // and p + q <= 4: three passes over 15 moments each.
-#pragma unroll
-            // Accumulated in FP64 in both precisions: at small radii these
-            // moments span many orders of magnitude (dx^4 against 1), and FP32
-            // lane sums put errors into the ill-conditioned Hessian directions.
-            for (int T = 0; T < 3; ++T) {
-                double m[15];
-#pragma unroll
-                for (int k = 0; k < 15; ++k) m[k] = 0.0;
-                for (int s = lane; s < a.nfull; s += 64) {
-                    int dx, dy;
-                    decode(a.table[s], dx, dy);
-                    const int x = cx + dx, y = cy + dy;
-                    if (!usable(x, y)) continue;
-                    const double gx = gxp[(size_t)y * W + x], gy = gyp[(size_t)y * W + x];
-                    const double g = (T == 0) ? gx * gx : ((T == 1) ? gx * gy : gy * gy);
-                    const double X = dx, Y = dy, X2 = X * X, Y2 = Y * Y, XY = X * Y;
-                    const double mono[15] = {1.0, X, Y, X2, XY, Y2, X2 * X, X2 * Y, X * Y2, Y2 * Y,
-                                             X2 * X2, X2 * XY, X2 * Y2, XY * Y2, Y2 * Y2};
-#pragma unroll
-                    for (int k = 0; k < 15; ++k) m[k] += g * mono[k];
-                }
-#pragma unroll
-                for (int k = 0; k < 15; ++k) {
-                    const double v = wave_sum(m[k]);
-                    if (lane == 0) sh.mom[T][k] = v;
-                }
-            }
-            WAVE_SYNC();
-            for (int e = lane; e < 144; e += 64) {
-                const int ia = e / 12, ib = e % 12;
-                const ParamMonomial ma = param_monomial(ia), mb = param_monomial(ib);
-                const int qq = ma.q + mb.q, d = ma.p + mb.p + qq;
-                sh.H[e] = hscale * sd_half(ia) * sd_half(ib) * sh.mom[ma.t + mb.t][d * (d + 1) / 2 + qq];
-            }
-        }
-        if constexpr (!TWOPASS && ZNSSD) { // FP32 kernels (FP64 ZNSSD is two-pass); reference-side sums in FP64
-            Acc<double> a_sd[P], a_fsd[P], a_f, a_f2;
-            const double fin = 1.0 / fnorm;
-            for (int s = lane; s < a.nfull; s += 64) {
-                int dx, dy;
-                decode((int)__builtin_amdgcn_raw_buffer_load_b32(trs, 4 * s, 0, 0), dx, dy);
-                const int x = cx + dx, y = cy + dy;
-                if (!usable(x, y)) continue;
-                const int vo = (__mul24(y, W) + x) * 4;
-                const double gx = buffer_f32(grx, vo), gy = buffer_f32(gry, vo);
-                const double fh = ((double)buffer_f32(frs, vo) - fmean) * fin;
-                const double lx = dx, ly = dy;
-                a_f.add(fh);
-                a_f2.add(fh * fh);
-                acc_sd<P, double>(a_sd, gx, gy, lx, ly, lx * lx, lx * ly, ly * ly);
-                acc_sd<P, double>(a_fsd, fh * gx, fh * gy, lx, ly, lx * lx, lx * ly, ly * ly);
-            }
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-                const double v1 = wave_sum(a_sd[j].value()) * sd_half(j), v2 = wave_sum(a_fsd[j].value()) * sd_half(j);
-                if (lane == 0) {
-                    sh.ssd[j] = v1;
-                    sh.sfsd[j] = v2;
-                }
-            }
-            const double v1 = wave_sum(a_f.value()), v2 = wave_sum(a_f2.value());
-            if (lane == 0) {
-                sh.sfh = v1;
-                sh.sfh2 = v2;
-            }
-        }
         WAVE_SYNC();
         int ok, pos;
         if constexpr (P == 12) {

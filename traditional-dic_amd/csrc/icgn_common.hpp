@@ -63,6 +63,83 @@ __device__ __forceinline__ bool le_int(double t, int n)
     return t <= -1e9 || (t < 1e9 && (int)ceil(t) <= n);
 }
 
+// ---- many sums over the wave at once (transposed butterfly) ----
+// permlane32 swap of a pair (lo, hi): lanes 0-31 get their partner's lo in hi,
+// lanes 32-63 their partner's hi in lo (gfx950 V_PERMLANE32_SWAP, both dwords)
+__device__ __forceinline__ void swap_halves32(double &lo, double &hi)
+{
+    const auto a = __builtin_amdgcn_permlane32_swap(__double2loint(lo), __double2loint(hi), false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(__double2hiint(lo), __double2hiint(hi), false, false);
+    lo = __hiloint2double((int)b[0], (int)a[0]);
+    hi = __hiloint2double((int)b[1], (int)a[1]);
+}
+// the same between 16-lane rows 0<->1 and 2<->3 (V_PERMLANE16_SWAP)
+__device__ __forceinline__ void swap_rows16(double &lo, double &hi)
+{
+    const auto a = __builtin_amdgcn_permlane16_swap(__double2loint(lo), __double2loint(hi), false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(__double2hiint(lo), __double2hiint(hi), false, false);
+    lo = __hiloint2double((int)b[0], (int)a[0]);
+    hi = __hiloint2double((int)b[1], (int)a[1]);
+}
+// v of the DPP partner lane (row_mirror 0x140, row_half_mirror 0x141, quad permutations)
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sums N per-lane values gen(0..N-1) over the 64 lanes and stores total i in
+// dst[i] (adds it to dst[i] when `add`).  Instead of N butterflies of six
+// exchanges each, every exchange level halves the list a lane is responsible
+// for: lanes 0-31 keep the first half of the (padded) list and receive their
+// partner's copy of it, lanes 32-63 the second half (permlane32 swap), then
+// rows 0/2 against rows 1/3 (permlane16 swap), then lanes l / 15-l of a row
+// (DPP row mirror); the last NP/8 values are summed over each group of 8
+// lanes by full butterflies, after which the 8 lanes hold bit-identical
+// totals and the first lane of each group stores them.  Called by the whole
+// wave with i compile-time in gen; the caller orders LDS (WAVE_SYNC) before
+// reading dst.
+template <int N, typename Gen>
+__device__ __forceinline__ void wave_reduce_store(Gen &&gen, int lane, double *dst, bool add)
+{
+    constexpr int NP = (N + 7) & ~7, H1 = NP / 2, H2 = NP / 4, H3 = NP / 8;
+    double a[H1];
+#pragma unroll
+    for (int i = 0; i < H1; ++i) {
+        double lo = (i < N) ? gen(i) : 0.0;
+        double hi = (i + H1 < N) ? gen(i + H1) : 0.0;
+        swap_halves32(lo, hi);
+        a[i] = lo + hi;
+    }
+    double b[H2];
+#pragma unroll
+    for (int i = 0; i < H2; ++i) {
+        double lo = a[i], hi = a[i + H2];
+        swap_rows16(lo, hi);
+        b[i] = lo + hi;
+    }
+    const bool up = (lane & 8) != 0;
+    double c[H3];
+#pragma unroll
+    for (int i = 0; i < H3; ++i) {
+        const double keep = up ? b[i + H3] : b[i], send = up ? b[i] : b[i + H3];
+        c[i] = keep + dpp_f64<0x140>(send);
+    }
+#pragma unroll
+    for (int i = 0; i < H3; ++i) {
+        c[i] += dpp_f64<0x141>(c[i]);
+        c[i] += dpp_f64<0x4E>(c[i]);
+        c[i] += dpp_f64<0xB1>(c[i]);
+    }
+    if ((lane & 7) == 0) {
+        const int base = ((lane >> 5) & 1) * H1 + ((lane >> 4) & 1) * H2 + ((lane >> 3) & 1) * H3;
+#pragma unroll
+        for (int i = 0; i < H3; ++i)
+            if (base + i < N) dst[base + i] = add ? dst[base + i] + c[i] : c[i];
+    }
+}
+
 __device__ __forceinline__ double uniform_f64(double v)
 {
     const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
