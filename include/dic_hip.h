@@ -361,15 +361,27 @@ void dic_server_destroy(dic_server *s);
  * (Success, valid, correlation < max_znssd, |du|, |dv| < step from the guess,
  * finite; :359-368); rejected, unvisited and out-of-ROI points keep the
  * default record (status InvalidInput, valid 0, x and y set).  The reference
- * pops one point at a time from a heap ordered by correlation; here every
- * frontier (all points accepted in the previous wave) is expanded at once,
- * best correlation first, and its new neighbours are solved in one launch,
- * so which parent seeds a neighbour, and hence which points a strict
- * max_znssd rejects, can differ from the reference's order.  params->masked
- * plays truncate_roi_subsets (solve_with_mask).  roi is host memory, W x H. */
+ * pops one point at a time from a heap ordered by correlation:
+ *   DIC_PROPAGATE_HEAP     -- that order exactly: the host replays the heap
+ *                             walk (the same std::priority_queue operations)
+ *                             and takes each solve from speculative launches
+ *                             over the heap's points in pop order, so every
+ *                             point is solved from the parent the reference
+ *                             would use and the valid set is the reference's
+ *                             at any max_znssd;
+ *   DIC_PROPAGATE_FRONTIER -- every frontier (all points accepted in the
+ *                             previous wave) is expanded at once, best
+ *                             correlation first, and its new neighbours are
+ *                             solved in one launch: fewer, larger launches,
+ *                             but which parent seeds a neighbour, and hence
+ *                             which points a strict max_znssd rejects, can
+ *                             differ from the reference's order.
+ * params->masked plays truncate_roi_subsets (solve_with_mask).  roi is host
+ * memory, W x H. */
+enum { DIC_PROPAGATE_HEAP = 0, DIC_PROPAGATE_FRONTIER = 1 };
 typedef struct dic_propagation_params {
     int32_t spacing;      /* propagation_spacing (>= 0) */
-    int32_t reserved;
+    int32_t order;        /* DIC_PROPAGATE_* */
     double max_znssd;     /* propagation_max_znssd */
 } dic_propagation_params;
 

@@ -153,3 +153,47 @@ def test_gpu_propagation_strict_threshold_and_bad_seed(pkg, ctx, stereo):
     outside = _seed(pkg, 5.0, 5.0, 0.0, 0.0, (0, 0, 0, 0), 0.0)  # outside the ROI
     _, grid, n = pkg.propagate(ref, dfm, stereo["roi"], outside, solver=solver)
     assert n == 0 and grid["valid"].sum() == 0 and np.all(grid["status"] == 2)
+
+
+def _window_walks(pkg, ctx, orc, stereo, max_znssd, order):
+    ref, dfm, roi, seed_row, golden = crop_case(stereo, "left_temporal")
+    pre = orc.Precomputed(dfm, degree=5)
+    gx, gy = orc.reference_gradient_cache(ref)
+    cpu, ncpu = orc.propagate(ref, pre, gx, gy, roi, seed_row, radius=41, max_iterations=50, threshold=1e-6,
+                              spacing=3, max_znssd=max_znssd)
+    s = seed_row[0]
+    seed = _seed(pkg, s[0], s[1], s[2], s[3], s[4:8], s[8])
+    cfg = _cfg(pkg)
+    cfg.propagation_max_znssd = max_znssd
+    cfg.propagation_order = order
+    solver = pkg.ICGNSolver(cfg, precision="fp64", context=ctx)
+    _, gpu, ngpu = pkg.propagate(ref, dfm, roi, seed, solver=solver)
+    return cpu, ncpu, gpu, ngpu, len(golden)
+
+
+@pytest.mark.parametrize("max_znssd", [0.5, 0.05, 0.0082, 0.0077])
+def test_gpu_heap_order_walk_equals_oracle_walk(pkg, ctx, orc, stereo, max_znssd):
+    """The heap order replays the reference's priority-queue walk
+    (reliability_propagation.cpp:277-389): at any acceptance limit, including
+    limits that reject part of the window (its golden ZNSSD spans 0.006 to
+    0.0098, median 0.0077), every point is solved from the parent the oracle's best-first walk
+    uses, so the valid set, the count and u, v (1e-5 px) equal the oracle's."""
+    cpu, ncpu, gpu, ngpu, lenient = _window_walks(pkg, ctx, orc, stereo, max_znssd, "heap")
+    v = cpu["valid"] == 1
+    d = np.abs(gpu["p"][v][:, :2] - cpu["p"][v][:, :2]).max(axis=1) if v.any() else np.zeros(1)
+    _report(f"heap walk, max ZNSSD {max_znssd}: gpu {ngpu} / oracle {ncpu} valid ({lenient} at the golden "
+            f"limit), max |du| {d.max():.2e}")
+    assert ngpu == ncpu and np.array_equal(gpu["valid"], cpu["valid"])
+    assert d.max() <= 1e-5
+    if max_znssd < 0.01:
+        assert ncpu < lenient  # the limit rejects points, so the walk order decides the valid set
+
+
+def test_gpu_frontier_order_accepts_what_heap_order_accepts_at_lenient_limit(pkg, ctx, orc, stereo):
+    """The frontier order at the golden limit (2.0: every solved point passes)
+    gives the heap order's valid set; values may come from other parents."""
+    _, _, heap, nh, _ = _window_walks(pkg, ctx, orc, stereo, 2.0, "heap")
+    _, _, front, nf, _ = _window_walks(pkg, ctx, orc, stereo, 2.0, "frontier")
+    assert nh == nf and np.array_equal(heap["valid"], front["valid"])
+    v = heap["valid"] == 1
+    assert np.abs(heap["p"][v][:, :2] - front["p"][v][:, :2]).max() <= 1e-4

@@ -159,8 +159,11 @@ class ServerStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+PROPAGATION_ORDERS = {"heap": 0, "frontier": 1}  # DIC_PROPAGATE_HEAP, DIC_PROPAGATE_FRONTIER
+
+
 class PropagationParams(ctypes.Structure):
-    _fields_ = [("spacing", ctypes.c_int32), ("reserved", ctypes.c_int32), ("max_znssd", ctypes.c_double)]
+    _fields_ = [("spacing", ctypes.c_int32), ("order", ctypes.c_int32), ("max_znssd", ctypes.c_double)]
 
 
 SEED_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("displacement", INITIAL_DTYPE)])
@@ -468,15 +471,18 @@ class Context:
         return out
 
     def propagate(self, params: IcgnParams, spacing: int, max_znssd: float, ref: "DeviceImage", grad,
-                  coeffs: "Coefficients", roi: np.ndarray, seeds: np.ndarray):
-        """dic_propagate: returns (grid results (gh, gw), points_computed)."""
+                  coeffs: "Coefficients", roi: np.ndarray, seeds: np.ndarray, order: str = "heap"):
+        """dic_propagate: returns (grid results (gh, gw), points_computed).
+        order: "heap" (the reference's priority-queue order) or "frontier"."""
+        if order not in PROPAGATION_ORDERS:
+            raise ValueError(f"propagation order must be one of {sorted(PROPAGATION_ORDERS)}, not {order!r}")
         r = _checked_roi(roi, ref)
         sd = np.ascontiguousarray(seeds, SEED_DTYPE)
         step = max(int(spacing), 0) + 1
         W, H = ref.width, ref.height
         gw, gh = (W + step - 1) // step, (H + step - 1) // step
         out = np.zeros(gw * gh, RESULT_DTYPE)
-        pp = PropagationParams(int(spacing), 0, float(max_znssd))
+        pp = PropagationParams(int(spacing), PROPAGATION_ORDERS[order], float(max_znssd))
         a, b, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self.check(self.lib.dic_propagate(self.handle, ctypes.byref(params), ctypes.byref(pp), ref.handle,
                                           grad.handle if grad is not None else None, coeffs.handle, _ptr(r),

@@ -1,8 +1,11 @@
 // propagate.hip -- reliability-guided propagation over the POI grid
 // (dic_propagate): the reference's seed alignment, neighbour rules, guess
-// extrapolation and acceptance test, with each frontier solved in one launch.
+// extrapolation and acceptance test.  Two walk orders: the reference's heap
+// order (speculative launches, replayed on the host) and frontier waves
+// (each frontier solved in one launch).
 #include <algorithm>
 #include <cmath>
+#include <unordered_map>
 #include <vector>
 
 #include "dic_internal.hpp"
@@ -67,6 +70,143 @@ int solve_host(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref
     if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(s), "propagation sync");
     return rc;
 }
+
+// Heap order (reliability_propagation.cpp:277-389): the reference pops one
+// point at a time from a std::priority_queue ordered by correlation and
+// solves the popped point's unvisited neighbours with guesses extrapolated
+// from it; whether a neighbour is accepted, and so what is pushed next,
+// depends on that solve.  Here the host replays exactly that walk, with the
+// same container operations (std::push_heap / std::pop_heap under the
+// reference's comparison), and takes every solve from a cache keyed by
+// (neighbour, parent): a solve's result depends on nothing but the
+// neighbour's position and the parent's parameters.  When the point on top
+// of the heap needs a solve the cache lacks, one launch solves, for the
+// heap's points in their pop order, every neighbour that is not yet visited
+// nor taken by an earlier point of that order; points accepted after the
+// launch can claim neighbours first, so some of these solves go unused, and
+// the next launch covers what the replay then needs.
+struct HeapWalk {
+    dic_ctx *ctx;
+    const dic_icgn_params *params;
+    const dic_image *ref;
+    const dic_grad *grad;
+    const dic_coeffs *def;
+    DeviceScratch &d;
+    const uint8_t *roi;
+    int W, H, step, gw, gh, radius;
+    double max_znssd;
+    std::vector<uint8_t> &calculated;
+    dic_result *grid_out;
+    int &computed;
+    std::unordered_map<long long, dic_result> cache;
+    std::vector<int> taken; // launch number that took a neighbour in speculate()
+    int launches = 0;
+    std::vector<double> xy;
+    std::vector<dic_initial> ini;
+    std::vector<dic_result> res;
+
+    static bool later(const Item &a, const Item &b) { return a.corr > b.corr; } // QueueItem::operator<
+    long long key(int nidx, const Item &p) const { return (long long)nidx * gw * gh + (p.gy * gw + p.gx); }
+
+    // neighbour q of p (up, right, down, left) as grid index, or -1 when the
+    // bounds test of :296-303 skips it; roi_ok / margin_ok: the tests of :300-311
+    int neighbour(const Item &p, int q, int &nx, int &ny, bool &usable) const
+    {
+        const int px = p.gx * step, py = p.gy * step;
+        const int nb[4][2] = {{px, py - step}, {px + step, py}, {px, py + step}, {px - step, py}};
+        nx = nb[q][0];
+        ny = nb[q][1];
+        if (!(nx >= 0 && ny >= 0 && nx < W && ny < H)) return -1;
+        const int gnx = nx / step, gny = ny / step;
+        if (gnx < 0 || gny < 0 || gnx >= gw || gny >= gh) return -1;
+        usable = roi[(size_t)ny * W + nx] &&
+                 !(nx - radius < 0 || ny - radius < 0 || nx + radius >= W || ny + radius >= H);
+        return gny * gw + gnx;
+    }
+    dic_initial guess(const Item &p, int nx, int ny) const
+    {
+        const double ddx = (double)(nx - p.gx * step), ddy = (double)(ny - p.gy * step);
+        return {p.u + p.ux * ddx + p.uy * ddy, p.v + p.vx * ddx + p.vy * ddy, p.ux, p.uy, p.vx, p.vy, 0.0};
+    }
+    bool needs_launch(const Item &p) const
+    {
+        for (int q = 0; q < 4; ++q) {
+            int nx, ny;
+            bool usable = false;
+            const int nidx = neighbour(p, q, nx, ny, usable);
+            if (nidx >= 0 && !calculated[nidx] && usable && !cache.count(key(nidx, p))) return true;
+        }
+        return false;
+    }
+    // one launch: the neighbours of the heap's points in their pop order
+    int speculate(const std::vector<Item> &heap)
+    {
+        constexpr size_t kMaxLaunch = 8192;
+        std::vector<Item> h(heap);
+        if (taken.empty()) taken.assign((size_t)gw * gh, 0);
+        ++launches;
+        xy.clear();
+        ini.clear();
+        std::vector<long long> keys;
+        while (!h.empty() && ini.size() + 4 <= kMaxLaunch) {
+            std::pop_heap(h.begin(), h.end(), later);
+            const Item p = h.back();
+            h.pop_back();
+            for (int q = 0; q < 4; ++q) {
+                int nx, ny;
+                bool usable = false;
+                const int nidx = neighbour(p, q, nx, ny, usable);
+                if (nidx < 0 || calculated[nidx] || !usable) continue;
+                if (taken[nidx] == launches) continue;
+                taken[nidx] = launches;
+                const long long k = key(nidx, p);
+                if (cache.count(k)) continue;
+                keys.push_back(k);
+                xy.push_back((double)nx);
+                xy.push_back((double)ny);
+                ini.push_back(guess(p, nx, ny));
+            }
+        }
+        const int rc = solve_host(ctx, params, ref, grad, def, d, xy, ini, res);
+        if (rc != DIC_OK) return rc;
+        for (size_t c = 0; c < keys.size(); ++c) cache.emplace(keys[c], res[c]);
+        return DIC_OK;
+    }
+    int walk(const Item &seed)
+    {
+        std::vector<Item> heap{seed};
+        while (!heap.empty()) {
+            if (needs_launch(heap.front())) {
+                const int rc = speculate(heap);
+                if (rc != DIC_OK) return rc;
+            }
+            std::pop_heap(heap.begin(), heap.end(), later);
+            const Item p = heap.back();
+            heap.pop_back();
+            for (int q = 0; q < 4; ++q) {
+                int nx, ny;
+                bool usable = false;
+                const int nidx = neighbour(p, q, nx, ny, usable);
+                if (nidx < 0 || calculated[nidx]) continue;
+                calculated[nidx] = 1; // marked even when outside the ROI or the margin (:300-311, :357)
+                if (!usable) continue;
+                const dic_initial g = guess(p, nx, ny);
+                const dic_result &r = cache.at(key(nidx, p));
+                const bool accepted = r.valid && r.status == DIC_SUCCESS && r.correlation < max_znssd &&
+                                      std::fabs(g.u - r.p[0]) < (double)step &&
+                                      std::fabs(g.v - r.p[1]) < (double)step && finite6(r);
+                if (!accepted) continue;
+                grid_out[nidx] = r;
+                grid_out[nidx].x = (double)nx;
+                grid_out[nidx].y = (double)ny;
+                ++computed;
+                heap.push_back({r.correlation, nx / step, ny / step, r.p[0], r.p[1], r.p[2], r.p[3], r.p[4], r.p[5]});
+                std::push_heap(heap.begin(), heap.end(), later);
+            }
+        }
+        return DIC_OK;
+    }
+};
 
 } // namespace
 
@@ -151,8 +291,16 @@ extern "C" int dic_propagate(dic_ctx *ctx, const dic_icgn_params *params, const 
         al.status = DIC_SUCCESS;
         grid_out[sidx] = al;
         ++computed;
-        std::vector<Item> frontier{{std::max(0.0, al.correlation), gxs, gys, al.p[0], al.p[1], al.p[2], al.p[3],
-                                    al.p[4], al.p[5]}};
+        const Item seed_item{std::max(0.0, al.correlation), gxs, gys, al.p[0], al.p[1], al.p[2], al.p[3], al.p[4],
+                             al.p[5]};
+        if (prop->order == DIC_PROPAGATE_HEAP) {
+            HeapWalk hw{ctx, params, ref, grad, def, d, roi, W, H, step, gw, gh, radius, max_znssd, calculated,
+                        grid_out, computed, {}, {}, 0, {}, {}, {}};
+            rc = hw.walk(seed_item);
+            if (rc != DIC_OK) return rc;
+            continue;
+        }
+        std::vector<Item> frontier{seed_item};
         while (!frontier.empty()) {
             // best (lowest correlation) parents claim their neighbours first
             std::stable_sort(frontier.begin(), frontier.end(),

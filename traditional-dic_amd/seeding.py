@@ -244,7 +244,7 @@ class SubsetDIC:
             try:
                 params = make_params(cfg, self.precision, masked=bool(cfg.truncate_roi_subsets))
                 grid, n = self.ctx.propagate(params, cfg.propagation_spacing, cfg.propagation_max_znssd, pair.ref,
-                                             pair.grad, pair.coeffs, mask, sd)
+                                             pair.grad, pair.coeffs, mask, sd, order=cfg.propagation_order)
             finally:
                 pair.free()
         finally:
@@ -266,10 +266,13 @@ def subset(reference: np.ndarray, deformed: np.ndarray, config: str | Path | dic
            seed_subset_radius: int = 10, sift_enabled: bool = False, pyramid_enabled: bool = True,
            subpixel_refinement: bool = True, seed_count: int = 64, max_znssd: float = 0.3,
            min_displacement_norm: float = 0.0, min_texture_std: float = 0.02, propagation_spacing: int = 5,
-           propagation_max_znssd: float = 0.5, precision: str = "fp64", device: int = 0) -> dict:
+           propagation_max_znssd: float = 0.5, precision: str = "fp64", device: int = 0,
+           propagation_order: str = "heap") -> dict:
     """Run 2D Subset-DIC with reliability-guided propagation
     (python/traditional_dic/subset.py:16-164, same arguments and defaults):
     a YAML path or dict config takes precedence over the keyword arguments.
+    precision, device and propagation_order ("heap": the reference's walk
+    order; "frontier": whole frontiers per launch) are this build's own.
     Returns the dict of bind_subset.cpp:102-153 (x, y, u, v, du_dx, du_dy,
     dv_dx, dv_dy, correlation, valid)."""
     if not isinstance(reference, np.ndarray) or reference.ndim != 2:
@@ -302,7 +305,8 @@ def subset(reference: np.ndarray, deformed: np.ndarray, config: str | Path | dic
             },
             "seed_selection": {"seed_count": seed_count, "max_znssd": max_znssd,
                                "min_displacement_norm": min_displacement_norm, "min_texture_std": min_texture_std},
-            "reliability_propagation": {"spacing": propagation_spacing, "max_znssd": propagation_max_znssd},
+            "reliability_propagation": {"spacing": propagation_spacing, "max_znssd": propagation_max_znssd,
+                                        "order": propagation_order},
         }
     if roi is not None and (not isinstance(roi, np.ndarray) or roi.dtype not in (np.bool_, np.uint8)):
         raise RuntimeError("roi must be a 2D numpy array (bool or uint8) or None.")

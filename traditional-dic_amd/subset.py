@@ -87,6 +87,10 @@ class SubsetConfig:
     propagation_spacing: int = 5
     propagation_max_znssd: float = 0.5
     truncate_roi_subsets: bool = False
+    # walk order of the propagation (not a field of the reference's config):
+    # "heap" follows its priority queue exactly, "frontier" expands whole
+    # frontiers per launch (faster; a strict max_znssd can reject other points)
+    propagation_order: str = "heap"
     min_valid_sample_ratio: float = 0.5
     min_valid_samples: int = 12
     image_precompute: BSplinePrecomputeConfig = field(default_factory=BSplinePrecomputeConfig)
@@ -179,6 +183,7 @@ def config_from_dict(d: dict | None) -> SubsetConfig:
     if isinstance(rp, dict):
         cfg.propagation_spacing = int(rp.get("spacing", cfg.propagation_spacing))
         cfg.propagation_max_znssd = float(rp.get("max_znssd", cfg.propagation_max_znssd))
+        cfg.propagation_order = str(rp.get("order", cfg.propagation_order))
     sf = d.get("shape_function")
     if isinstance(sf, dict) and "order" in sf:
         order = str(sf["order"])
@@ -391,7 +396,8 @@ def propagate(reference: np.ndarray, deformed: np.ndarray, roi: np.ndarray, seed
     try:
         params = make_params(cfg, solver.precision, masked=bool(cfg.truncate_roi_subsets))
         grid, n = solver.ctx.propagate(params, cfg.propagation_spacing, cfg.propagation_max_znssd, pair.ref,
-                                       pair.grad, pair.coeffs, np.ascontiguousarray(roi, np.uint8), seeds)
+                                       pair.grad, pair.coeffs, np.ascontiguousarray(roi, np.uint8), seeds,
+                                       order=cfg.propagation_order)
     finally:
         if owned:
             pair.free()
