@@ -51,8 +51,9 @@ def test_ring_workflow_reproduces_golden_stats(pkg, tmp_path):
         Image.fromarray(a.astype(np.uint8), "L").save(tmp_path / name)
         paths[key] = tmp_path / name
     print("ring workflow: 1280x1280, golden config without SIFT", flush=True)
-    art = pkg.workflow.run_subset_2d(paths["img_reference"], [paths["img_deformed"]], paths["roi"], RING_CONFIG,
-                                     tmp_path / "result" / "subset")
+    run = pkg.workflow.run_subset_2d(paths["img_reference"], [paths["img_deformed"]], paths["roi"], RING_CONFIG,
+                                     tmp_path / "result" / "subset", run_id="ring")
+    art = run.artifacts
     assert art["stats"] == [str(tmp_path / "result" / "subset" / "002_znssd_icgn_1st" / "stats.json")]
     got = json.loads(open(art["stats"][0]).read())
     print({k: (got[k], golden[k]) for k in golden}, flush=True)
@@ -67,3 +68,9 @@ def test_ring_workflow_reproduces_golden_stats(pkg, tmp_path):
     assert rows.shape[0] == golden["total_points"] and int(rows["valid"].sum()) == golden["valid_points"]
     m = pkg.workflow.derive_subset_metrics(art["stats"])
     assert m["valid_fraction"] == golden["valid_points"] / golden["total_points"]
+    # the run contract beside the frame folder (run_contract.py:779-839)
+    files = pkg.run_contract.validate_run_contract(tmp_path / "result" / "subset")
+    assert files["status"]["execution_status"] == "SUCCESS" and files["status"]["quality_status"] == "QUALITY_OK"
+    assert files["metrics"]["valid_points"] == golden["valid_points"] and files["metrics"]["run_id"] == "ring"
+    assert files["result"]["primary_artifacts"] == ["displacements:002_znssd_icgn_1st__displacements.csv"]
+    assert files["result"]["secondary_artifacts"] == ["stats:002_znssd_icgn_1st__stats.json"]

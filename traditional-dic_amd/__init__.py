@@ -16,7 +16,7 @@ from .subset import (ICGNSolver, SubsetConfig, BSplinePrecomputeConfig, config_f
                      SeedInitializationConfig, SeedSelectionConfig)
 from .seeding import (IntegerSearchInitializer, SeedSelector, SeedSelectionResult, SubsetDIC, subset,
                       recommended_subset_padding, seed_params, search_params)
-from . import workflow
+from . import workflow, run_contract
 
 __all__ = [
     "Context", "DicError", "INITIAL_DTYPE", "RESULT_DTYPE", "IcgnParams", "load_library", "device_count",
@@ -29,5 +29,5 @@ __all__ = [
     "ESTIMATE_DTYPE", "SEED_EVAL_DTYPE", "SeedIntegerSearchConfig", "SeedSubpixelRefinementConfig",
     "SeedInitializationConfig", "SeedSelectionConfig", "IntegerSearchInitializer", "SeedSelector",
     "SeedSelectionResult", "SubsetDIC", "subset", "recommended_subset_padding", "seed_params", "search_params",
-    "workflow",
+    "workflow", "run_contract",
 ]

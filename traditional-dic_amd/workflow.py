@@ -7,6 +7,7 @@ Reference interfaces mirrored (paths inside the Traditional-DIC tree):
   subset_method_tag             python/traditional_dic/config.py:77-92
   the per-frame loop            workflows/subset_2d.py:152-227 (result/<frame>_<tag>/)
   derive_metrics (subset_2d)    python/traditional_dic/run_contract.py:392-411
+  the run contract files        run_contract.py (module run_contract here)
 
 The figures, legends and the least-squares strain post-processing of the
 reference workflow are not written here.
@@ -99,26 +100,46 @@ def subset_method_tag(config: dict | None) -> str:
 
 
 def run_subset_2d(reference_path, deformed_paths, roi_path, config: dict, result_root, precision: str = "fp64",
-                  device: int = 0) -> dict[str, list[str]]:
+                  device: int = 0, run_id: str | None = None, repository_root=None):
     """For each deformed frame: subset() on the GPU, then
     result_root/<frame>_<tag>/displacements.csv and stats.json
-    (workflows/subset_2d.py:152-227 without figures and strain)."""
+    (workflows/subset_2d.py:152-227 without figures and strain), and the run
+    contract's manifest.json / status.json / metrics.json / result.json in
+    result_root (run_contract.py:779-839): status RUNNING while the frames are
+    solved, EXECUTION_FAILED (and the exception re-raised) if a frame fails.
+    Returns a WorkflowRunResult with the artifacts and the contract paths."""
     from .seeding import subset
-    reference = read_gray(reference_path)
-    roi = read_mask(roi_path)
+    from . import run_contract as rc
     root = Path(result_root)
-    artifacts: dict[str, list[str]] = {"displacements": [], "stats": []}
-    tag = subset_method_tag(config)
-    criterion = str((config.get("correlation", {}) or {}).get("criterion", "znssd")).strip().lower()
-    for dp in deformed_paths:
-        out = root / (Path(dp).stem + (f"_{tag}" if tag else ""))
-        out.mkdir(parents=True, exist_ok=True)
-        result = subset(reference, read_gray(dp), config=config, roi=roi, precision=precision, device=device)
-        save_subset_csv(result, out / "displacements.csv")
-        artifacts["displacements"].append(str(out / "displacements.csv"))
-        write_stats(result, out / "stats.json", metric="ssd" if criterion == "ssd" else "znssd")
-        artifacts["stats"].append(str(out / "stats.json"))
-    return artifacts
+    ws = rc.RunWorkspace.create(root, "subset_2d", repository_root=repository_root, run_id=run_id)
+    try:
+        reference = read_gray(reference_path)
+        roi = read_mask(roi_path)
+        artifacts: dict[str, list[str]] = {"displacements": [], "stats": []}
+        tag = subset_method_tag(config)
+        criterion = str((config.get("correlation", {}) or {}).get("criterion", "znssd")).strip().lower()
+        for dp in deformed_paths:
+            out = root / (Path(dp).stem + (f"_{tag}" if tag else ""))
+            out.mkdir(parents=True, exist_ok=True)
+            result = subset(reference, read_gray(dp), config=config, roi=roi, precision=precision, device=device)
+            save_subset_csv(result, out / "displacements.csv")
+            artifacts["displacements"].append(str(out / "displacements.csv"))
+            write_stats(result, out / "stats.json", metric="ssd" if criterion == "ssd" else "znssd")
+            artifacts["stats"].append(str(out / "stats.json"))
+    except Exception as exc:
+        rc.write_execution_failure(ws, exc)
+        raise
+    wr = rc.WorkflowRunResult("subset_2d", root, artifacts)
+    try:
+        contract = rc.finalize_run_contract(ws, wr, config=config)
+        rc.validate_run_contract(ws.root)
+    except Exception as exc:
+        rc.write_execution_failure(ws, exc, stage="contract_finalization")
+        raise
+    wr.run_id, wr.contract = ws.run_id, contract
+    wr.manifest_path, wr.status_path = ws.manifest_path, ws.status_path
+    wr.metrics_path, wr.result_path = ws.metrics_path, ws.result_path
+    return wr
 
 
 def derive_subset_metrics(stats_paths) -> dict[str, Any]:
