@@ -291,20 +291,23 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
     if (!early) {
         WAVE_SYNC();
         int ok, pos;
-        if constexpr (P == 12) {
-            ldlt_factor_wave<P>(sh, lane, ok, pos);
-        } else { // 6 x 6: the serial form on one lane is as fast
-            if (lane == 0) {
-                ldlt_factor<P>(sh.H, sh.tr, sh.x, &ok, &pos);
-                sh.state = ok && pos;
+        const bool fast = ldlt_operator_fast<P>(sh, lane, ok, pos);
+        if (!fast) { // tied or NaN diagonal: Eigen's exchanges step by step
+            if constexpr (P == 12) {
+                ldlt_factor_wave<P>(sh, lane, ok, pos);
+            } else { // 6 x 6: the serial form on one lane is as fast
+                if (lane == 0) {
+                    ldlt_factor<P>(sh.H, sh.tr, sh.x, &ok, &pos);
+                    sh.state = ok && pos;
+                }
+                WAVE_SYNC();
+                ok = pos = sh.state;
             }
-            WAVE_SYNC();
-            ok = pos = sh.state;
         }
         if (!(ok && pos)) {
             early = true;
             status = DIC_NUMERICAL_FAILURE;
-        } else {
+        } else if (!fast) {
             ldlt_operator_column<P>(sh, lane);
             WAVE_SYNC();
         }

@@ -229,6 +229,7 @@ struct IcgnShared {
     double fmean, fnorm, sfh, sfh2;
     double mom[3][15]; // second-order Hessian moments (gx^2, gx gy, gy^2) x dx^p dy^q
     int tr[12];
+    int perm[12];      // pivot order of the LDL^T: position k holds row perm[k] of H
     int ns, state;
 };
 
@@ -392,6 +393,128 @@ template <int P> __device__ void ldlt_factor_wave(IcgnShared &sh, int lane, int 
     ok_out = ret;
     pos_out = (sign == 0 || sign == 2);
     WAVE_SYNC();
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// Eigen 3.4 LDLT of sh.H and its solve operator H+ (sh.Hinv), when the
+// |diagonal| entries of H are distinct and none is NaN.  Eigen's pivot at
+// step k is the largest |diagonal| among the rows not yet factored, and
+// those rows still carry their original diagonal (step k updates only
+// row k's), so the pivots sort the diagonal by magnitude, descending; with
+// distinct entries the position of row i is its rank.  The permutation is
+// applied once -- lane i loads row perm[i] of H with permuted columns -- and
+// the left-looking factorisation then needs no exchanges: every L and D entry
+// comes out of the same operations in the same order as in the stepwise
+// pivoted form (ldlt_factor_wave, ldlt_factor), with row k of L broadcast by
+// readlane instead of shuffles.  info / isPositive follow the same rules.
+// Lane j then forms column j of H+ = P^T L^-T D^+ L^-1 P e_j in registers
+// (D^+ by reciprocal pivots, zero where |d| <= DBL_MIN).  Writes L, D (in
+// pivot order) to sh.H, the reciprocal pivots to sh.x, the order to sh.perm
+// and, when ok and positive, H+ to sh.Hinv.  Returns false, having written
+// nothing, on a tie or a NaN: the order of Eigen's exchanges then decides, and
+// the caller runs the stepwise factorisation.  Called by the whole wave.
+template <int P> __device__ bool ldlt_operator_fast(IcgnShared &sh, int lane, int &ok_out, int &pos_out)
+{
+    const bool own = lane < P;
+    const double di = own ? fabs(sh.H[lane * P + lane]) : 0.0;
+    int rank = 0;
+    bool bad = own && !(di == di);
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const double dj = fabs(sh.H[j * P + j]);
+        rank += (dj > di) ? 1 : 0;
+        bad = bad || (own && j != lane && dj == di);
+    }
+    if (__any(bad)) return false;
+    if (own) sh.perm[rank] = lane;
+    WAVE_SYNC();
+    double row[P];
+    {
+        const int pi = own ? sh.perm[lane] : 0;
+#pragma unroll
+        for (int j = 0; j < P; ++j) row[j] = own ? sh.H[pi * P + sh.perm[j]] : 0.0;
+    }
+    WAVE_SYNC();
+    double D[P];
+    int sign = 2 /*zero*/, found_zero = 0, ret = 1;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        if (k > 0) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < k; ++j) acc += row[j] * (D[j] * readlane_f64(row[j], k));
+            row[k] -= acc;
+        }
+        const double akk = readlane_f64(row[k], k);
+        const bool valid = fabs(akk) > 0.0;
+        if (k == 0 && !valid) { // an all-zero diagonal: nothing is factored
+            sign = 2;
+            ret = 0;
+            break;
+        }
+        if (k + 1 < P) {
+            if (valid) {
+                if (lane > k) row[k] /= akk;
+            } else {
+                ret = ret && !__any(own && lane > k && row[k] != 0.0);
+            }
+        }
+        if (found_zero && valid) ret = 0;
+        else if (!valid) found_zero = 1;
+        if (sign == 0) {
+            if (akk < 0.0) sign = 3;
+        } else if (sign == 1) {
+            if (akk > 0.0) sign = 3;
+        } else if (sign == 2) {
+            if (akk > 0.0) sign = 0;
+            else if (akk < 0.0) sign = 1;
+        }
+        D[k] = akk;
+    }
+    ok_out = ret;
+    pos_out = (sign == 0 || sign == 2);
+    if (!(ret && pos_out)) return true;
+    if (own) { // L (strict lower) and D in pivot order; lane k divides its own pivot
+        double dk = 0.0;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            if (j <= lane) sh.H[lane * P + j] = row[j];
+            if (j == lane) dk = D[j];
+        }
+        sh.x[lane] = fabs(dk) > DBL_MIN ? 1.0 / dk : 0.0;
+    }
+    WAVE_SYNC();
+    if (own) {
+        // row by row, each behind a compiler barrier, so that at most one row
+        // of L is loaded ahead (hoisting all P(P-1)/2 loads would spill)
+        double x[P];
+#pragma unroll
+        for (int i = 0; i < P; ++i) { // L^-1 P e_j: the unit vector at the pivot position of row j
+            WAVE_SYNC();
+            double s = (i == rank) ? 1.0 : 0.0;
+#pragma unroll
+            for (int l = 0; l < i; ++l) s -= sh.H[i * P + l] * x[l];
+            x[i] = s;
+        }
+#pragma unroll
+        for (int i = P - 1; i >= 0; --i) {
+            WAVE_SYNC();
+            double s = x[i] * sh.x[i];
+#pragma unroll
+            for (int l = i + 1; l < P; ++l) s -= sh.H[l * P + i] * x[l];
+            x[i] = s;
+        }
+        WAVE_SYNC();
+#pragma unroll
+        for (int k = 0; k < P; ++k) sh.Hinv[sh.perm[k] * P + lane] = x[k];
+    }
+    WAVE_SYNC();
+    return true;
 }
 
 // Column `lane` of the LDLT solve operator H+ (Eigen LDLT::_solve_impl: P b,
