@@ -182,13 +182,13 @@ def _serving_pools(ctx, synth, seed):
     return pools
 
 
-def _serve(pkg, ctx, pools, arrivals, precision, lag, paced, against=None, keep=False):
+def _serve(pkg, ctx, pools, arrivals, precision, lag, paced, against=None, keep=False, extrapolate=False):
     """One pass of the request list through a fresh server.  paced: submit at
     the arrival times (a live stream); else all at t = 0 (saturation).
     against: per-request (u, v, ok) of another pass, to count POIs within
     1e-3 px of it; keep: return this pass's (u, v, ok) per request."""
     srv = ctx.server(radius=15, shape_order=1, degree=5, precision=precision, max_active=256, max_width=2448,
-                     max_height=2048, hbm_budget=200 * 10**9, guess_lag=lag)
+                     max_height=2048, hbm_budget=200 * 10**9, guess_lag=lag, guess_extrapolate=extrapolate)
     ref, frames, xy = pools[(1024, 1024)]
     srv.submit(ref, frames[:2], xy, np.zeros(xy.shape[0], pkg.INITIAL_DTYPE)).wait(120.0)  # warm
     reqs = []
@@ -201,7 +201,7 @@ def _serve(pkg, ctx, pools, arrivals, precision, lag, paced, against=None, keep=
         ref, frames, xy = pools[size]
         reqs.append(srv.submit(ref, frames[:nf], xy, np.zeros(xy.shape[0], pkg.INITIAL_DTYPE)))
     frame_lat, req_lat, kept = [], [], []
-    pois = ok_n = conv = 0
+    pois = ok_n = conv = iters = 0
     for i, r in enumerate(reqs):
         res = r.wait(900.0)
         frame_lat.append(r.frame_done_s - r.arrival)
@@ -209,6 +209,7 @@ def _serve(pkg, ctx, pools, arrivals, precision, lag, paced, against=None, keep=
         ok = (res["status"] == pkg.SUCCESS) & (res["valid"] == 1)
         pois += res.size
         ok_n += int(ok.sum())
+        iters += int(res["iterations"].sum())
         if against is not None:
             u, v, ok2 = against[i]
             d = np.maximum(np.abs(res["p"][..., 0] - u), np.abs(res["p"][..., 1] - v))
@@ -229,6 +230,7 @@ def _serve(pkg, ctx, pools, arrivals, precision, lag, paced, against=None, keep=
                                 "p99": round(float(np.percentile(frame_lat, 99)) * 1e3, 2)},
            "request_latency_ms": {"p50": round(float(np.percentile(req_lat, 50)) * 1e3, 2),
                                   "p99": round(float(np.percentile(req_lat, 99)) * 1e3, 2)},
+           "mean_iterations": round(iters / max(pois, 1), 3),
            "ticks": st["ticks"], "max_active_seen": st["max_active_seen"],
            "peak_hbm_gb": round(st["peak_bytes_held"] / 1e9, 2)}
     if against is not None:
@@ -242,14 +244,17 @@ def serving_leg(pkg, ctx, synth, duration_s=6.0, rate=20.0, seed=4000):
     2448x2048 (50/50) against its resident reference frame; radius 15, first
     order, POI step 7, quintic; 200 GB admission budget.
 
-    Live stream (requests submitted at their arrival times): mixed precision
-    with guess lag K = 1 (frame f from frame f - 1) and K = 4 (frame f from
-    f - 4, up to 4 frames of a request per tick), and the plain FP64 drop-in
-    at K = 4: frame and request latency p50/p99; mixed K = 1 and 4 again at a
-    quarter of the rate.  Saturation (the same
-    requests all submitted at t = 0, K = 4): the FP64 pass, then the mixed
-    pass, whose converged POIs are those within 1e-3 px of FP64; their
-    converged POIs/s ratio is the served mixed-vs-FP64 figure."""
+    Guesses: frame f starts from frame f - K's results (K = 1 or 4; up to K
+    frames of a request share a tick) or, extrapolated, from 2 r(f - K) -
+    r(f - 2K).  Live stream (requests submitted at their arrival times):
+    mixed precision K = 1 plain, K = 1 and K = 4 extrapolated, and the plain
+    FP64 drop-in at K = 1 extrapolated: frame and request latency p50/p99;
+    the mixed K = 1 / K = 4 extrapolated passes again at a quarter of the
+    rate.  Saturation (the same requests all submitted at t = 0, K = 1
+    extrapolated): the FP64 pass, then the mixed pass, whose converged POIs
+    are those within 1e-3 px of FP64; their converged POIs/s ratio is the
+    served mixed-vs-FP64 figure.  The headline latency is the live mixed
+    K = 1 pass."""
     t0 = time.time()
     pools = _serving_pools(ctx, synth, seed)
     log(f"serving frame pools rendered in {time.time() - t0:.1f}s")
@@ -267,15 +272,16 @@ def serving_leg(pkg, ctx, synth, duration_s=6.0, rate=20.0, seed=4000):
     arrivals, light = poisson(rate), poisson(rate / 4)
     mixed, fp64 = pkg.PRECISION_MIXED, pkg.PRECISION_FP64
     stream, stream_light = {}, {}
-    for name, prec, lag in (("mixed_lag1", mixed, 1), ("mixed_lag4", mixed, 4), ("fp64_lag4", fp64, 4)):
-        stream[name], _ = _serve(pkg, ctx, pools, arrivals, prec, lag, paced=True)
+    for name, prec, lag, ex in (("mixed_lag1", mixed, 1, False), ("mixed_lag1_extrapolated", mixed, 1, True),
+                                ("mixed_lag4_extrapolated", mixed, 4, True), ("fp64_lag1_extrapolated", fp64, 1, True)):
+        stream[name], _ = _serve(pkg, ctx, pools, arrivals, prec, lag, paced=True, extrapolate=ex)
         log(f"serving stream {name}: {stream[name]}")
-    for name, lag in (("mixed_lag1", 1), ("mixed_lag4", 4)):
-        stream_light[name], _ = _serve(pkg, ctx, pools, light, mixed, lag, paced=True)
+    for name, lag in (("mixed_lag1_extrapolated", 1), ("mixed_lag4_extrapolated", 4)):
+        stream_light[name], _ = _serve(pkg, ctx, pools, light, mixed, lag, paced=True, extrapolate=True)
         log(f"serving light stream {name}: {stream_light[name]}")
-    sat64, ref64 = _serve(pkg, ctx, pools, arrivals, fp64, 4, paced=False, keep=True)
+    sat64, ref64 = _serve(pkg, ctx, pools, arrivals, fp64, 1, paced=False, keep=True, extrapolate=True)
     log(f"serving saturated fp64: {sat64}")
-    satmx, _ = _serve(pkg, ctx, pools, arrivals, mixed, 4, paced=False, against=ref64)
+    satmx, _ = _serve(pkg, ctx, pools, arrivals, mixed, 1, paced=False, against=ref64, extrapolate=True)
     log(f"serving saturated mixed: {satmx}")
     del ref64
     mx_rate = satmx["converged_vs_other"] / satmx["span_s"]
@@ -284,9 +290,10 @@ def serving_leg(pkg, ctx, synth, duration_s=6.0, rate=20.0, seed=4000):
                         f"1024^2 / 2448x2048, radius 15, first order, step 7, quintic, 200 GB budget"),
            "requests": len(arrivals), "frames": int(sum(a[2] for a in arrivals)), "pois": satmx["pois"],
            "offered_pois_per_s": round(satmx["pois"] / duration_s, 1),
+           "headline": "stream.mixed_lag1",
            "stream": stream,
            "stream_quarter_rate": dict(stream_light, requests=len(light), rate_req_per_s=rate / 4),
-           "saturated": {"fp64_lag4": sat64, "mixed_lag4": satmx,
+           "saturated": {"fp64_lag1_extrapolated": sat64, "mixed_lag1_extrapolated": satmx,
                          "mixed_converged_pois_per_s": round(mx_rate, 1),
                          "fp64_converged_pois_per_s": round(fp_rate, 1),
                          "mixed_vs_fp64": round(mx_rate / fp_rate, 3)}}
@@ -393,10 +400,11 @@ def fgn_leg(pkg, ctx, wl, steps=5, stride=16):
 def propagation_leg(pkg, ctx, reps=3):
     """Reliability-guided propagation over the reference's stereo-plate case
     (1440x1080, ROI, radius 41, quintic, ||dp|| < 1e-6, spacing 3, one seed),
-    left_temporal pair, plain FP64: the GPU frontier walk (all 7654 ROI grid
-    points) timed end to end from host images (upload, gradient cache,
-    prefilter, walk, download, as the reference's propagate builds its cache
-    and coefficients inside the call); the oracle's sequential
+    left_temporal pair, plain FP64: the GPU walk (all 7654 ROI grid points)
+    in the reference's heap order and in frontier order, each timed end to
+    end from host images (upload, gradient cache, prefilter, walk, download,
+    as the reference's propagate builds its cache and coefficients inside
+    the call); the oracle's sequential
     best-first walk timed on a 340x320 window of the same case (2688 POIs)
     on one host thread, as the reference runs it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -416,14 +424,18 @@ def propagation_leg(pkg, ctx, reps=3):
         "subset": {"radius": 41}, "optimization": {"method": "icgn", "max_iterations": 50,
                                                   "convergence_threshold": 1e-6},
         "interpolation": {"degree": 5}, "reliability_propagation": {"spacing": 3, "max_znssd": 2.0}})
-    solver = pkg.ICGNSolver(cfg, precision="fp64", context=ctx)
-    pkg.propagate(ref, dfm, st["roi"], seed, solver=solver)  # warm (module load, first launches)
-    ts = []  # each call uploads the pair and builds its gradient cache and coefficients, as :182-183 do
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        _, grid, n = pkg.propagate(ref, dfm, st["roi"], seed, solver=solver)
-        ts.append(time.perf_counter() - t0)
-    t_gpu = float(np.median(ts))
+    walks = {}
+    for order in ("heap", "frontier"):
+        cfg.propagation_order = order
+        solver = pkg.ICGNSolver(cfg, precision="fp64", context=ctx)
+        pkg.propagate(ref, dfm, st["roi"], seed, solver=solver)  # warm (module load, first launches)
+        ts = []  # each call uploads the pair and builds its gradient cache and coefficients, as :182-183 do
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            _, grid, n = pkg.propagate(ref, dfm, st["roi"], seed, solver=solver)
+            ts.append(time.perf_counter() - t0)
+        walks[order] = (float(np.median(ts)), int(n), bool(int(grid["valid"].sum()) == rows.shape[0]))
+    t_gpu, n, same = walks["heap"]
     from oracle import oracle as orc
     from test_oracle_propagation import crop_case
     cref, cdfm, croi, cseed, _ = crop_case(st, "left_temporal")
@@ -435,10 +447,14 @@ def propagation_leg(pkg, ctx, reps=3):
     t_cpu = time.perf_counter() - t0
     out = {"workload": "stereo-plate left_temporal (reference case), 1440x1080, ROI, radius 41, quintic, "
                        "thr 1e-6, spacing 3, one seed",
+           "order": "heap (the reference's priority-queue order)",
            "gpu_points": int(n), "gpu_s": round(t_gpu, 4), "gpu_points_per_s": round(n / t_gpu, 1),
+           "frontier_order": {"gpu_points": walks["frontier"][1], "gpu_s": round(walks["frontier"][0], 4),
+                              "gpu_points_per_s": round(walks["frontier"][1] / walks["frontier"][0], 1),
+                              "valid_equals_golden": walks["frontier"][2]},
            "cpu_oracle_points": int(ncpu), "cpu_oracle_s": round(t_cpu, 3),
            "cpu_oracle_points_per_s": round(ncpu / t_cpu, 1), "cpu_threads": 1,
-           "valid_equals_golden": bool(int(grid["valid"].sum()) == rows.shape[0])}
+           "valid_equals_golden": same}
     log(f"propagation: {out}")
     return out
 
@@ -796,8 +812,9 @@ def main():
                  "converged": int(ok64.sum()), "roofline": roofline("fp64", fp),
                  "step_phases": memory_phases("fp64"),
                  "rms_error_vs_truth_px": round(float(err_truth), 5)},
-        "request_latency_ms": (dict(serving["stream"]["mixed_lag4"]["request_latency_ms"],
-                                    source="serving leg, live stream, mixed, guess lag 4") if serving else None),
+        "request_latency_ms": (dict(serving["stream"]["mixed_lag1"]["request_latency_ms"],
+                                    source="serving leg, live stream at 20 req/s, mixed, frame f from frame "
+                                           "f - 1's results") if serving else None),
         "mixed_vs_fp64_speedup": round(float(value / value64), 3),
         "mixed_vs_fp64_by_workload": {
             "large_field_2048": round(float(value / value64), 3),

@@ -279,7 +279,8 @@ void dic_host_free(void *ptr);
  * A request is an incremental frame sequence against one reference frame:
  * frame f is correlated with the reference at the request's POIs, starting
  * from the request's initial guesses (f < K) or from frame f-K's results
- * (u, v, gradients; confidence = correlation), K = guess_lag.  dic_server_submit
+ * (u, v, gradients; confidence = correlation), K = guess_lag, or from their
+ * extrapolation along the sequence (guess_extrapolate).  dic_server_submit
  * returns at once; a worker thread admits pending requests in arrival order
  * while the number of active requests stays below max_active and the device
  * bytes they hold (reference, gradients, two frame buffers, 3K coefficient,
@@ -291,7 +292,7 @@ void dic_host_free(void *ptr);
  * flight: uploads and prefilters of the next ticks (copy engines, prep
  * stream) and the copy-out of the previous one overlap the solve of the
  * current one.  Each POI's result depends only on its own inputs and the
- * frame its guesses come from, which K alone fixes, so a request's results
+ * frames its guesses come from, which K and guess_extrapolate fix, so a request's results
  * are the same served alone or in any mix.  The caller keeps every array of
  * a request valid until dic_server_wait reports it done. ---- */
 #define DIC_SERVER_MAX_GUESS_LAG 16
@@ -316,6 +317,10 @@ typedef struct dic_server_config {
     int32_t tick_pois;            /* a tick takes the next frame of every active request, then more
                                    * frames (up to K per request, arrival order) while it holds fewer
                                    * POIs than this (0: DIC_SERVER_DEFAULT_TICK_POIS) */
+    int32_t guess_extrapolate;    /* 1: frames f >= 2K start from the linear extrapolation
+                                   * 2 r(f-K) - r(f-2K) of u, v and the gradients where both earlier
+                                   * results are Success (r(f-K) otherwise); 0: from r(f-K) */
+    int32_t reserved;
 } dic_server_config;
 
 typedef struct dic_request {

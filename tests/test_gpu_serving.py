@@ -4,7 +4,8 @@ sequences with HBM-budget admission.
 Each POI's result depends only on its own inputs, so a request's results
 must be bit-identical served alone and served in a mix of requests that
 arrive while it runs.  Frame f's guesses are frame f-K's results (K the
-guess lag; frames below K start from the request's guesses), so the served
+guess lag; frames below K start from the request's guesses) or, with
+extrapolation, 2 r(f-K) - r(f-2K) from frame 2K on, so the served
 FP64 results are checked against the same chain of direct
 dic_icgn_solve_batch calls (identical statuses, |du|, |dv| <= 1e-9 px) and
 frame 0 against the CPU oracle (|du|, |dv| <= 1e-5 px).
@@ -38,9 +39,21 @@ def _guesses_from(pkg, r):
     return g
 
 
-def _direct_chain(pkg, ctx, seq, precision, optimizer="icgn", objective="znssd", order=1, lag=1):
+def _extrapolated(pkg, a, b):
+    """Guesses 2 a - b (u, v, gradients) where a and b are both Success, a's
+    values otherwise; confidence = a's correlation."""
+    g = _guesses_from(pkg, a)
+    lin = (a["status"] == 0) & (b["status"] == 0)
+    for k, name in enumerate(("u", "v", "du_dx", "du_dy", "dv_dx", "dv_dy")):
+        g[name] = np.where(lin, 2.0 * a["p"][:, k] - b["p"][:, k], a["p"][:, k])
+    return g
+
+
+def _direct_chain(pkg, ctx, seq, precision, optimizer="icgn", objective="znssd", order=1, lag=1,
+                  extrapolate=False):
     """Frame f solved directly from frame f - lag's results (the request's
-    guesses below lag)."""
+    guesses below lag), or from the extrapolation of frames f - lag and
+    f - 2 lag where f >= 2 lag."""
     ref, frames, xy, ini = seq
     cfg = pkg.SubsetConfig(subset_radius=15, optimizer=optimizer, objective=objective,
                            shape_function="second_order" if order == 2 else "first_order")
@@ -50,26 +63,32 @@ def _direct_chain(pkg, ctx, seq, precision, optimizer="icgn", objective="znssd",
     res = []
     for f, frame in enumerate(frames):
         co = ctx.prefilter(ctx.upload_image(frame), 5, 3, True, precision)
-        cur = ini if f < lag else _guesses_from(pkg, res[f - lag])
+        if f < lag:
+            cur = ini
+        elif extrapolate and f >= 2 * lag:
+            cur = _extrapolated(pkg, res[f - lag], res[f - 2 * lag])
+        else:
+            cur = _guesses_from(pkg, res[f - lag])
         res.append(ctx.solve(params, dref, grad, co, xy, cur))
     return np.stack(res)
 
 
-@pytest.mark.parametrize("lag", [1, 4])
-def test_served_fp64_matches_direct_chain_and_oracle(pkg, ctx, orc, seqs, lag):
+@pytest.mark.parametrize("lag,extrapolate", [(1, False), (4, False), (1, True), (2, True)])
+def test_served_fp64_matches_direct_chain_and_oracle(pkg, ctx, orc, seqs, lag, extrapolate):
     srv = ctx.server(radius=15, shape_order=1, degree=5, precision=pkg.PRECISION_FP64, max_active=8,
-                     max_width=512, max_height=512, guess_lag=lag)
+                     max_width=512, max_height=512, guess_lag=lag, guess_extrapolate=extrapolate)
     req = srv.submit(*seqs[0])
     got = req.wait(60.0)
     st = srv.stats()
     srv.close()
     n_frames = len(seqs[0][1])
     assert st["ticks"] == -(-n_frames // lag)  # K frames of the one request share a tick
-    want = _direct_chain(pkg, ctx, seqs[0], pkg.PRECISION_FP64, lag=lag)
+    want = _direct_chain(pkg, ctx, seqs[0], pkg.PRECISION_FP64, lag=lag, extrapolate=extrapolate)
     assert np.array_equal(got["status"], want["status"])
     ok = want["status"] == 0
     d = np.abs(got["p"][..., :2] - want["p"][..., :2])[ok].max(initial=0)
-    _report(f"served (guess lag {lag}) vs direct chain: {ok.mean():.4f} ok, max |du| {d:.2e}, {st['ticks']} ticks")
+    _report(f"served (guess lag {lag}, extrapolate {extrapolate}) vs direct chain: {ok.mean():.4f} ok, "
+            f"max |du| {d:.2e}, {st['ticks']} ticks, mean iterations {got['iterations'].mean():.2f}")
     assert d <= 1e-9 and ok.mean() > 0.98
     ref, frames, xy, ini = seqs[0]
     pre = orc.Precomputed(frames[0], degree=5)
@@ -121,10 +140,10 @@ def test_served_other_configs_match_direct_chain(pkg, ctx, seqs, objective, orde
     assert got.tobytes() == want.tobytes()
 
 
-@pytest.mark.parametrize("lag", [1, 4])
-def test_alone_and_in_mix_are_identical(pkg, ctx, seqs, lag):
+@pytest.mark.parametrize("lag,extrapolate", [(1, False), (4, False), (1, True), (4, True)])
+def test_alone_and_in_mix_are_identical(pkg, ctx, seqs, lag, extrapolate):
     srv = ctx.server(radius=15, shape_order=1, degree=5, precision=pkg.PRECISION_MIXED, max_active=8,
-                     max_width=512, max_height=512, guess_lag=lag)
+                     max_width=512, max_height=512, guess_lag=lag, guess_extrapolate=extrapolate)
     alone = srv.submit(*seqs[0]).wait(60.0).copy()
     # the same request again, in a mix: others are submitted before and after it
     r1 = srv.submit(*seqs[1])

@@ -128,6 +128,8 @@ class ServerConfig(ctypes.Structure):
         ("optimizer", ctypes.c_int32),
         ("guess_lag", ctypes.c_int32),
         ("tick_pois", ctypes.c_int32),
+        ("guess_extrapolate", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -587,14 +589,17 @@ class Context:
 
     def server(self, radius=15, shape_order=1, degree=5, precision=PRECISION_MIXED, objective=OBJECTIVE_ZNSSD,
                max_active=64, max_width=4096, max_height=4096, hbm_budget=0, max_iterations=30,
-               convergence_threshold=1e-3, optimizer=OPTIMIZER_ICGN, guess_lag=1, tick_pois=0) -> "Server":
+               convergence_threshold=1e-3, optimizer=OPTIMIZER_ICGN, guess_lag=1, tick_pois=0,
+               guess_extrapolate=False) -> "Server":
         """guess_lag K: frame f starts from frame f - K's results (frames below
         K from the request's guesses), and up to K frames of a request share a
         tick while the tick holds fewer than tick_pois POIs (0: the library's
-        default)."""
+        default).  guess_extrapolate: frames f >= 2K start from 2 r(f - K) -
+        r(f - 2K) where both are Success."""
         cfg = ServerConfig(int(hbm_budget), int(max_active), int(max_width), int(max_height), int(radius),
                            int(shape_order), int(max_iterations), float(convergence_threshold), int(degree),
-                           int(precision), int(objective), int(optimizer), int(guess_lag), int(tick_pois))
+                           int(precision), int(objective), int(optimizer), int(guess_lag), int(tick_pois),
+                           int(bool(guess_extrapolate)), 0)
         h = _vp()
         self.check(self.lib.dic_server_create(self.handle, ctypes.byref(cfg), ctypes.byref(h)))
         return Server(self, h)

@@ -174,6 +174,9 @@ struct RangeLaunch {
 };
 hipError_t launch_range(const RangeLaunch &L, hipStream_t s);
 hipError_t launch_init_from_results(const dic_result *prev, dic_initial *init, int n, hipStream_t s);
+// guesses 2 a - b (u, v, gradients) where both a and b are Success, a otherwise
+hipError_t launch_init_extrapolated(const dic_result *a, const dic_result *b, dic_initial *init, int n,
+                                    hipStream_t s);
 hipError_t add_batch_nodes(const BatchLaunch &L, hipGraph_t g);
 
 struct IcgnLaunch {

@@ -922,6 +922,22 @@ __global__ void init_from_results_kernel(const dic_result *__restrict__ prev, di
     init[i] = o;
 }
 
+// Next frame's guesses extrapolated along the sequence: a = results K frames
+// back, b = 2K frames back; u, v and the four gradients of 2 a - b where both
+// records are Success, a's values otherwise; confidence = a's correlation.
+__global__ void init_extrapolated_kernel(const dic_result *__restrict__ a, const dic_result *__restrict__ b,
+                                         dic_initial *__restrict__ init, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const dic_result ra = a[i], rb = b[i];
+    const bool lin = ra.status == DIC_SUCCESS && rb.status == DIC_SUCCESS;
+    double q[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) q[j] = lin ? 2.0 * ra.p[j] - rb.p[j] : ra.p[j];
+    init[i] = dic_initial{q[0], q[1], q[2], q[3], q[4], q[5], ra.correlation};
+}
+
 __global__ void fill_u32_kernel(unsigned *p, int n, unsigned v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1858,6 +1874,14 @@ hipError_t launch_init_from_results(const dic_result *prev, dic_initial *init, i
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(init_from_results_kernel, dim3((n + 255) / 256), dim3(256), 0, s, prev, init, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_extrapolated(const dic_result *a, const dic_result *b, dic_initial *init, int n,
+                                    hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(init_extrapolated_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a, b, init, n);
     return hipGetLastError();
 }
 

@@ -74,6 +74,7 @@ struct dic_server {
     dic_server_config cfg{};
     int border = 3, pitch = 0, rows_max = 0;
     int lag = 1;   // K
+    bool extrapolate = false; // guesses 2 r(f-K) - r(f-2K)
     int ring = kDepth; // kDepth * K
     long tick_pois = 0;   // extra frames join a tick while it holds fewer POIs
     int copy_threads = 1; // threads of the result copy-out
@@ -331,9 +332,17 @@ int build_tick(dic_server *s, Tick &t, int slot)
         const int f = t.frames[k].second, r = f % s->ring;
         const dic_initial *guess = sl->init0;
         if (f >= s->lag) {
-            rc = hipcheck(s, launch_init_from_results(sl->out[(f - s->lag) % s->ring], sl->init[r], q->r.n,
-                                                      s->compute),
-                          "initial guesses from an earlier frame");
+            // frame f - 2K's result slot is reused only by frame f + K, whose
+            // solve follows this tick on the compute stream
+            if (s->extrapolate && f >= 2 * s->lag)
+                rc = hipcheck(s, launch_init_extrapolated(sl->out[(f - s->lag) % s->ring],
+                                                          sl->out[(f - 2 * s->lag) % s->ring], sl->init[r], q->r.n,
+                                                          s->compute),
+                              "initial guesses extrapolated from earlier frames");
+            else
+                rc = hipcheck(s, launch_init_from_results(sl->out[(f - s->lag) % s->ring], sl->init[r], q->r.n,
+                                                          s->compute),
+                              "initial guesses from an earlier frame");
             if (rc != DIC_OK) return rc;
             guess = sl->init[r];
         }
@@ -587,6 +596,7 @@ int dic_server_create(dic_ctx *ctx, const dic_server_config *cfg, dic_server **o
     s->cfg = c;
     s->border = b;
     s->lag = c.guess_lag < 1 ? 1 : c.guess_lag;
+    s->extrapolate = c.guess_extrapolate != 0;
     s->ring = kDepth * s->lag;
     s->tick_pois = c.tick_pois > 0 ? c.tick_pois : DIC_SERVER_DEFAULT_TICK_POIS;
     s->copy_threads = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
