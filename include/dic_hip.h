@@ -231,13 +231,14 @@ enum { DIC_BATCH_ORDER_NATURAL = 0, DIC_BATCH_ORDER_COST = 1, DIC_BATCH_ORDER_IN
  *   mapping DIC_BATCH_GRID: one workgroup (one 64-lane wave) per POI;
  *   mapping DIC_BATCH_PERSISTENT: waves_per_cu x CUs workgroups take `chunk`
  *     POIs at a time from a queue counter until the queue is empty;
- *   mapping DIC_BATCH_TILE: first-order items of a mixed-precision batch run
- *     four neighbouring POIs of one item per workgroup (one per wave) over a
- *     tile of the coefficients staged in LDS around their starting subsets
- *     (workgroups dealt over the 8 XCDs in contiguous runs); there the POIs
- *     of an item keep their natural order and `order` only ranks the items
- *     (by cost under COST).  Second-order and FP64 items run as GRID in the
- *     given order.
+ *   mapping DIC_BATCH_TILE: first-order IC-GN items of subset radius <= 24
+ *     in a mixed-precision batch (those whose four neighbouring subsets can
+ *     share one tile, the per-pair launch's rule) run four neighbouring POIs
+ *     of one item per workgroup (one per wave) over a tile of the
+ *     coefficients staged in LDS around their starting subsets (workgroups
+ *     dealt over the 8 XCDs in contiguous runs); there the POIs of an item
+ *     keep their natural order and `order` only ranks the items (by cost
+ *     under COST).  All other items run as GRID in the given order.
  *   order: NATURAL = item after item; COST = items by descending estimated
  *     cost per POI (subset samples x shape order); INTERLEAVE = round robin
  *     over the items in runs of 256 POIs; XCD = natural order with each

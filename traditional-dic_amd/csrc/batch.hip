@@ -71,31 +71,36 @@ int valid_tuning(dic_ctx *ctx, const dic_batch_tuning &t)
     return DIC_OK;
 }
 
-// Launch groups: descriptors of one optimizer and shape order (FGN, then
-// second order first); where the solve keeps per-sample values in LDS (FP64
-// ZNSSD IC-GN, FGN) also of one LDS size class (power of two of the
-// subset's bytes), so a small subset is not limited by the LDS the largest
-// one needs.
-std::vector<std::vector<int>> make_groups(const dic_batch *b)
+// Whether descriptor p runs the tile kernel under `mapping`: under TILE, the
+// first-order FP32 IC-GN descriptors whose four neighbouring subsets can share
+// one tile (r <= 24, the per-pair launch's rule); the others keep one POI per
+// workgroup.
+bool tile_item(const dic_batch *b, const BatchPair &p, int mapping)
 {
-    std::map<std::tuple<int, int, int>, std::vector<int>> m; // (-fgn, -order, -class) -> descriptors
+    return mapping == DIC_BATCH_TILE && !p.a.fgn && p.shape_order == 1 && b->precision != DIC_PRECISION_FP64 &&
+           grouped_radius(p.a.radius);
+}
+
+// Launch groups: descriptors of one optimizer and shape order (FGN, then
+// second order first), tile descriptors apart; where the solve keeps
+// per-sample values in LDS (FP64 ZNSSD IC-GN, FGN) also of one LDS size class
+// (power of two of the subset's bytes), so a small subset is not limited by
+// the LDS the largest one needs.
+std::vector<std::vector<int>> make_groups(const dic_batch *b, int mapping)
+{
+    // (-fgn, -order, tile, -class) -> descriptors
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> m;
     for (int p = 0; p < (int)b->pairs.size(); ++p) {
         const IcgnArgs &a = b->pairs[p].a;
         const size_t bytes = solve_dynamic_lds(a.fgn != 0, b->precision, b->objective, a.nfull);
         int cls = 0;
         if (bytes > 0)
             while ((size_t(1) << cls) < bytes) ++cls;
-        m[{-a.fgn, -b->pairs[p].shape_order, -cls}].push_back(p);
+        m[{-a.fgn, -b->pairs[p].shape_order, tile_item(b, b->pairs[p], mapping) ? 1 : 0, -cls}].push_back(p);
     }
     std::vector<std::vector<int>> g;
     for (auto &kv : m) g.push_back(kv.second);
     return g;
-}
-
-// Whether group G runs the tile kernel under the TILE mapping (first order, FP32).
-bool tile_group(const dic_batch *b, int shape_order, int mapping)
-{
-    return mapping == DIC_BATCH_TILE && shape_order == 1 && b->precision != DIC_PRECISION_FP64;
 }
 
 // (descriptor, POI) list, group after group, each group in the requested
@@ -108,11 +113,12 @@ std::vector<int2> make_items(const dic_batch *b, int order, int mapping, std::ve
     std::vector<int2> items;
     items.reserve((size_t)b->total);
     groups.clear();
-    for (std::vector<int> ids : make_groups(b)) {
+    for (std::vector<int> ids : make_groups(b, mapping)) {
         const size_t start = items.size();
-        BatchGroup G{b->pairs[ids[0]].shape_order, 0, (int)start, 0, b->pairs[ids[0]].a.fgn};
+        BatchGroup G{b->pairs[ids[0]].shape_order, 0, (int)start, 0, b->pairs[ids[0]].a.fgn,
+                     tile_item(b, b->pairs[ids[0]], mapping) ? 1 : 0};
         for (int p : ids) G.max_nfull = std::max(G.max_nfull, b->pairs[p].a.nfull);
-        if (!G.fgn && tile_group(b, G.shape_order, mapping)) {
+        if (G.tile) {
             if (order == DIC_BATCH_ORDER_COST)
                 std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) { return b->cost[x] > b->cost[y]; });
             for (int p : ids)
@@ -192,7 +198,6 @@ int prepare(dic_batch *b)
     L.groups = b->groups.data();
     L.ngroups = (int)b->groups.size();
     L.persistent = b->tuning.mapping == DIC_BATCH_PERSISTENT;
-    L.tile = b->tuning.mapping == DIC_BATCH_TILE;
     L.grid = std::max(1, b->tuning.waves_per_cu * b->cus);
     L.chunk = std::max(1, b->tuning.chunk);
     L.counter = b->d_counter;
@@ -390,7 +395,7 @@ int dic_batch_autotune(dic_batch *b, const dic_batch_tuning *candidates, int n_c
                     cand.push_back({DIC_BATCH_PERSISTENT, wpc, chunk, order});
         bool first_fp32 = false;
         for (const BatchPair &p : b->pairs)
-            first_fp32 = first_fp32 || (!p.a.fgn && tile_group(b, p.shape_order, DIC_BATCH_TILE));
+            first_fp32 = first_fp32 || tile_item(b, p, DIC_BATCH_TILE);
         if (first_fp32)
             for (int order : {DIC_BATCH_ORDER_NATURAL, DIC_BATCH_ORDER_COST, DIC_BATCH_ORDER_XCD})
                 cand.push_back({DIC_BATCH_TILE, 0, 1, order});

@@ -143,7 +143,8 @@ struct BatchGroup {
     int shape_order;
     int max_nfull;
     int first, count;
-    int fgn; // forward Gauss-Newton descriptors (one POI per workgroup, per-sample LDS buffer)
+    int fgn;  // forward Gauss-Newton descriptors (one POI per workgroup, per-sample LDS buffer)
+    int tile; // first-order FP32 descriptors of r <= 24 under the TILE mapping: four POIs per workgroup
 };
 constexpr int kMaxBatchGroups = 64; // queue counters in the batch's 256-byte counter buffer
 
@@ -154,7 +155,6 @@ struct BatchLaunch {
     const BatchGroup *groups;
     int ngroups;
     bool persistent;   // work-queue kernels on a fixed grid, else one workgroup per item
-    bool tile;         // first-order FP32 groups: four items per workgroup over an LDS tile
     int grid;          // persistent workgroups
     int chunk;         // items per queue grab
     unsigned *counter; // one queue counter per group (persistent)

@@ -1718,7 +1718,7 @@ hipError_t batch_launch_of(const BatchLaunch &L, int g, Launch1 &o)
             return hipFuncSetAttribute(o.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)o.lds);
         return hipSuccess;
     }
-    if (L.tile && G.shape_order == 1 && L.precision != DIC_PRECISION_FP64) {
+    if (G.tile) {
         o.func = L.objective == DIC_OBJECTIVE_SSD ? batch_group_func_deg<DIC_OBJECTIVE_SSD>(L.degree)
                                                   : batch_group_func_deg<DIC_OBJECTIVE_ZNSSD>(L.degree);
         if (!o.func) return hipErrorInvalidValue;
