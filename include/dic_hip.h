@@ -160,12 +160,10 @@ int dic_bspline_sample(dic_ctx *ctx, const dic_coeffs *c, const double *xy, int 
  * (src/subset/solver/forward_gauss_newton.cpp:117-175 -> :188-1941); grad is
  * then unused and may be NULL.
  * Largest subset: one POI's workgroup holds a 3.4 KiB solver block in LDS,
- * plus one double per sample for FP64 ZNSSD IC-GN (two-pass statistics) and
- * the value and two gradient terms per sample for FGN (8 B each in FP64, 4 B
- * mixed).  On a 160 KiB-LDS device (MI355X) that is radius <= 79 for FP64
- * ZNSSD IC-GN, <= 46 for FP64 FGN and <= 65 for mixed FGN; beyond it the call
- * returns DIC_ERR_UNSUPPORTED before any launch.  The reference has no such
- * limit.
+ * plus one double per sample for FP64 ZNSSD IC-GN (two-pass statistics).
+ * On a 160 KiB-LDS device (MI355X) that is radius <= 79 for FP64 ZNSSD
+ * IC-GN; beyond it the call returns DIC_ERR_UNSUPPORTED before any launch.
+ * The reference has no such limit; nor do the other solves here.
  * Host-pointer form: copies in, runs, copies out, synchronises. ---- */
 int dic_icgn_solve_batch(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
                          const dic_coeffs *def, const uint8_t *roi_host, const double *poi_xy,

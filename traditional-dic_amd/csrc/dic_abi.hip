@@ -28,8 +28,7 @@ int check_hip(dic_ctx *ctx, hipError_t e, const char *what)
 
 size_t solve_dynamic_lds(bool fgn, int precision, int objective, int nfull)
 {
-    const size_t elem = precision == DIC_PRECISION_FP64 ? sizeof(double) : sizeof(float);
-    if (fgn) return 3 * elem * (size_t)nfull;
+    if (fgn) return 0; // FGN keeps no per-sample values
     const bool twopass = precision == DIC_PRECISION_FP64 && objective == DIC_OBJECTIVE_ZNSSD;
     return twopass ? sizeof(double) * (size_t)nfull : 0;
 }

@@ -79,9 +79,9 @@ int launch_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *r
 int noalloc_setup(dic_ctx *ctx);
 
 // LDS of one solve workgroup: the static per-POI block (icgn.hip) plus the
-// per-sample buffer of FP64 two-pass ZNSSD IC-GN (one value) or of FGN (value
-// and two gradient terms); check_solve_lds turns a subset too large for the
-// device into DIC_ERR_UNSUPPORTED before any launch
+// per-sample buffer of FP64 two-pass ZNSSD IC-GN (one value; FGN keeps none);
+// check_solve_lds turns a subset too large for the device into
+// DIC_ERR_UNSUPPORTED before any launch
 size_t solve_static_lds();
 size_t solve_dynamic_lds(bool fgn, int precision, int objective, int nfull);
 int check_solve_lds(dic_ctx *ctx, bool fgn, int precision, int objective, int radius, int nfull);
@@ -143,7 +143,7 @@ struct BatchGroup {
     int shape_order;
     int max_nfull;
     int first, count;
-    int fgn;  // forward Gauss-Newton descriptors (one POI per workgroup, per-sample LDS buffer)
+    int fgn;  // forward Gauss-Newton descriptors (one POI per workgroup)
     int tile; // first-order FP32 descriptors of r <= 24 under the TILE mapping: four POIs per workgroup
 };
 constexpr int kMaxBatchGroups = 64; // queue counters in the batch's 256-byte counter buffer

@@ -951,12 +951,20 @@ __global__ void fill_u32_kernel(unsigned *p, int n, unsigned v)
 // point; the steepest-descent vector is that gradient through dW/dz of the
 // current warp (:44-60, :986-1019); H, G are rebuilt, LDLT-factored, and
 // dp = +H^-1 G is capped and composed as W(p) o W(dp) (:64-76, :1025-1053).
-// Per-sample g, dW/dz-projected gradients (ge, he) live in LDS between the
-// passes: A (warp + sample), B (deformed norm, ZNSSD), C (G and C), then the
-// Hessian from 3 x NM moments of {ge^2, ge he, he^2} times monomials.
-// Variant constants as the reference has them: step cap 0.5 / 0.35 / 0.3,
-// which norm the convergence test reads, Levenberg-Marquardt on three of the
-// four second-order variants.
+// One pass over the samples per iteration, nothing stored per sample: every
+// quantity the iteration needs is a sum over the subset, accumulated per lane
+// and reduced once.  The Hessian is 3 x NM moments of {ge^2, ge he, he^2}
+// times the monomials dx^p dy^q (ge, he: the projected gradient); G and C
+// of ZNSSD come from the sums of d = g - mean_f, d^2, f^ d and of f^ sd,
+// d sd, sd (f^ the normalised reference):
+//   e = f^ - (d - mean_d) / |d - mean_d|,  G = 2 / |f - mean_f| sum e sd
+//     = 2 / |f - mean_f| (sum f^ sd - (sum d sd - mean_d sum sd) / |d - mean_d|),
+//   C = sum f^2 - 2 (sum f^ d - mean_d sum f^) / |d - mean_d| + 1,
+// the same quantities as the reference's two-pass forms (:31-39, :335-350),
+// rounded differently.  The solve is the LDLT operator of the IC-GN setup,
+// formed per iteration.  Variant constants as the reference has them: step
+// cap 0.5 / 0.35 / 0.3, which norm the convergence test reads,
+// Levenberg-Marquardt on three of the four second-order variants.
 // ---------------------------------------------------------------------------
 template <int P, int OBJ, bool MASKED> struct FgnRules {
     static constexpr bool znssd = OBJ == DIC_OBJECTIVE_ZNSSD;
@@ -967,49 +975,6 @@ template <int P, int OBJ, bool MASKED> struct FgnRules {
     // H_ii += 1e-4 max_i H_ii (:1242-1245, :1682-1685, :1886-1889)
     static constexpr bool lm = P == 12 && !(znssd && MASKED);
 };
-
-// Eigen LDLT::solve of one right-hand side in place (lane 0): P b, L^-1,
-// D^+ (|d| > DBL_MIN), L^-T, P^T.
-template <int P> __device__ void ldlt_solve_vec(const IcgnShared &sh, double *x)
-{
-#pragma unroll 1
-    for (int k = 0; k < P; ++k) {
-        const int t = sh.tr[k];
-        if (t != k) {
-            const double v = x[k];
-            x[k] = x[t];
-            x[t] = v;
-        }
-    }
-#pragma unroll 1
-    for (int i = 0; i < P; ++i) {
-        double acc = x[i];
-#pragma unroll 1
-        for (int j = 0; j < i; ++j) acc -= sh.H[i * P + j] * x[j];
-        x[i] = acc;
-    }
-#pragma unroll 1
-    for (int i = 0; i < P; ++i) {
-        const double d = sh.H[i * P + i];
-        x[i] = fabs(d) > DBL_MIN ? x[i] / d : 0.0;
-    }
-#pragma unroll 1
-    for (int i = P - 1; i >= 0; --i) {
-        double acc = x[i];
-#pragma unroll 1
-        for (int j = i + 1; j < P; ++j) acc -= sh.H[j * P + i] * x[j];
-        x[i] = acc;
-    }
-#pragma unroll 1
-    for (int k = P - 1; k >= 0; --k) {
-        const int t = sh.tr[k];
-        if (t != k) {
-            const double v = x[k];
-            x[k] = x[t];
-            x[t] = v;
-        }
-    }
-}
 
 // compose_warp: W(x; pn) = W(x; p) o W(x; dp) (:64-76 first order,
 // :1025-1053 second order, first order in dp)
@@ -1068,8 +1033,12 @@ __device__ __forceinline__ void sample_vg(__amdgpu_buffer_rsrc_t rs, int pitch, 
     }
 }
 
+// Occupancy targets of the FGN kernels (waves per SIMD), set by the per-lane
+// sums they keep: 3 x NM Hessian moments plus 3P + 3 (ZNSSD) or P + 1 (SSD).
+template <int P, typename R> constexpr int kFgnMinWaves = sizeof(R) == 4 ? (P == 12 ? 3 : 5) : (P == 12 ? 2 : 3);
+
 template <int P, int OBJ, int D, typename R, bool MASKED>
-__device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, IcgnShared &sh, R *lbuf)
+__device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, IcgnShared &sh)
 {
     constexpr bool ZNSSD = OBJ == DIC_OBJECTIVE_ZNSSD;
     constexpr bool FP64 = sizeof(R) == 8;
@@ -1081,7 +1050,6 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
     const int W = a.W, Hh = a.H;
     const double px = a.xy[2 * poi], py = a.xy[2 * poi + 1];
     const int cx = (int)round(px), cy = (int)round(py);
-    R *gv = lbuf, *gev = lbuf + a.nfull, *hev = lbuf + 2 * a.nfull;
     auto usable = [&](int x, int y) -> bool {
         if constexpr (MASKED) return x >= 0 && y >= 0 && x < W && y < Hh && a.roi[(size_t)y * W + x];
         return true;
@@ -1095,7 +1063,8 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
                                        cy + a.radius >= Hh;
         else early = !usable(cx, cy);
     }
-    // reference statistics (forward_gauss_newton.cpp:233-274)
+    // reference statistics (forward_gauss_newton.cpp:233-274), and for ZNSSD
+    // the sums of f^ and f^2 in the precision the iterations form f^ in
     if (!early) {
         double fsum_l = 0.0;
         int cnt = 0;
@@ -1107,7 +1076,7 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
             ++cnt;
         }
         const int ns = wave_sum_i(cnt);
-        double fmean = 0.0, fnorm = 1.0;
+        double fmean = 0.0, fnorm = 1.0, sfh = 0.0, sfh2 = 0.0;
         if (ns < a.min_needed) {
             early = true;
         } else if constexpr (ZNSSD) {
@@ -1124,11 +1093,26 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
             if (fnorm <= kEps) {
                 early = true;
                 status = DIC_NUMERICAL_FAILURE;
+            } else {
+                const R fm = (R)fmean, fin = (R)(1.0 / fnorm);
+                double h1 = 0.0, h2 = 0.0;
+                for (int s = lane; s < a.nfull; s += 64) {
+                    int dx, dy;
+                    decode(a.table[s], dx, dy);
+                    if (!usable(cx + dx, cy + dy)) continue;
+                    const double fh = (double)(((R)a.ref[(size_t)(cy + dy) * W + cx + dx] - fm) * fin);
+                    h1 += fh;
+                    h2 += fh * fh;
+                }
+                sfh = wave_sum(h1);
+                sfh2 = wave_sum(h2);
             }
         }
         if (lane == 0) {
             sh.fmean = fmean;
             sh.fnorm = fnorm;
+            sh.sfh = sfh;
+            sh.sfh2 = sfh2;
             sh.ns = ns;
         }
     }
@@ -1147,10 +1131,14 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
 #pragma unroll
             for (int j = 6; j < 12; ++j) sh.p[j] = 0.0;
         }
-        __syncthreads();
+        WAVE_SYNC();
         const double Wm1 = (double)(W - 1), Hm1 = (double)(Hh - 1);
         const int ns = sh.ns;
         const double fmean = uniform_f64(sh.fmean), fnorm = uniform_f64(sh.fnorm);
+        const R fm = (R)fmean, fin = (R)(1.0 / fnorm);
+        // raw sums of the iteration, reduced into the operator's space (the
+        // operator is formed only after they are read)
+        double *const raw = sh.Hinv;
         bool converged = false, failed = false;
         for (int it = 0; it < a.max_it; ++it) {
             iterations = it + 1;
@@ -1164,16 +1152,19 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
             R q[12];
 #pragma unroll
             for (int j = 2; j < 12; ++j) q[j] = (R)p[j];
-            // ---- pass A: warp, bounds (:283-300), value and gradient at the warped point ----
             bool oob = wild;
-            Acc<R> sg;
+            // ZNSSD: s1 = sum d, s2 = sum d^2, s3 = sum f^ d; ga = sum f^ sd,
+            // gb = sum d sd, gc = sum sd.  SSD: s2 = sum e^2, ga = sum e sd.
+            Acc<R> s1, s2, s3, ga[P], gb[P], gc[P], hm[3 * NM];
             for (int s = wild ? a.nfull : lane; s < a.nfull; s += 64) {
                 int dx, dy;
                 decode(a.table[s], dx, dy);
-                if (!usable(cx + dx, cy + dy)) continue;
+                const int x = cx + dx, y = cy + dy;
+                if (!usable(x, y)) continue;
                 int ix, iy;
                 R tx, ty;
                 const R lx = (R)dx, ly = (R)dy;
+                // warp and bounds (:283-300)
                 if constexpr (FP64) {
                     double wx = (double)cx + lx + p[0] + p[2] * lx + p[3] * ly;
                     double wy = (double)cy + ly + p[1] + p[4] * lx + p[5] * ly;
@@ -1210,6 +1201,7 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
                         continue;
                     }
                 }
+                const R f = (R)a.ref[(size_t)y * W + x];
                 R g, gx, gy;
                 sample_vg<D, R>(crs, a.pitch, a.b, ix, iy, tx, ty, g, gx, gy);
                 // dW/dz of the current warp at (lx, ly) (:994-998)
@@ -1220,87 +1212,83 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
                     a10 += q[9] * lx + q[10] * ly;
                     a11 += q[10] * lx + q[11] * ly;
                 }
-                gv[s] = g;
-                gev[s] = gx * a00 + gy * a10;
-                hev[s] = gx * a01 + gy * a11;
-                sg.add(g);
+                const R ge = gx * a00 + gy * a10, he = gx * a01 + gy * a11;
+                const R xx = lx * lx, xy = lx * ly, yy = ly * ly;
+                if constexpr (ZNSSD) {
+                    const R fh = (f - fm) * fin, d = g - fm;
+                    s1.add(d);
+                    s2.add(d * d);
+                    s3.add(fh * d);
+                    acc_sd<P, R>(ga, fh * ge, fh * he, lx, ly, xx, xy, yy);
+                    acc_sd<P, R>(gb, d * ge, d * he, lx, ly, xx, xy, yy);
+                    acc_sd<P, R>(gc, ge, he, lx, ly, xx, xy, yy);
+                } else { // e = f - g
+                    const R e = f - g;
+                    s2.add(e * e);
+                    acc_sd<P, R>(ga, e * ge, e * he, lx, ly, xx, xy, yy);
+                }
+                const R Q[3] = {ge * ge, ge * he, he * he};
+                R mono[NM];
+                mono[0] = R(1);
+                mono[1] = lx;
+                mono[2] = ly;
+                mono[3] = xx;
+                mono[4] = xy;
+                mono[5] = yy;
+                if constexpr (P == 12) {
+                    mono[6] = xx * lx;
+                    mono[7] = xx * ly;
+                    mono[8] = lx * yy;
+                    mono[9] = yy * ly;
+                    mono[10] = xx * xx;
+                    mono[11] = xx * xy;
+                    mono[12] = xx * yy;
+                    mono[13] = xy * yy;
+                    mono[14] = yy * yy;
+                }
+#pragma unroll
+                for (int T = 0; T < 3; ++T)
+#pragma unroll
+                    for (int k = 0; k < NM; ++k) hm[T * NM + k].add(Q[T] * mono[k]);
             }
             if (__any(oob)) {
                 status = DIC_INVALID_INPUT;
                 failed = true;
                 break;
             }
-            // ---- pass B: deformed mean and norm (vector_norm, :31-39) ----
-            double gmean = 0.0, gnorm = 1.0;
+            // ---- the sums over the wave: Hessian moments, then G and C ----
+#pragma unroll
+            for (int T = 0; T < 3; ++T)
+                wave_reduce_store<NM>([&](int i) { return hm[T * NM + i].value(); }, lane, sh.mom[T], false);
             if constexpr (ZNSSD) {
-                gmean = wave_sum(sg.value()) / (double)ns;
-                const R gm = (R)gmean;
-                Acc<R> sq;
-                for (int s = lane; s < a.nfull; s += 64) {
-                    if constexpr (MASKED) {
-                        int dx, dy;
-                        decode(a.table[s], dx, dy);
-                        if (!usable(cx + dx, cy + dy)) continue;
-                    }
-                    const R d = gv[s] - gm;
-                    sq.add(d * d);
-                }
-                gnorm = sqrt(wave_sum(sq.value()));
-                if (gnorm <= kEps) {
+                wave_reduce_store<3 + P>(
+                    [&](int i) { return i == 0 ? s1.value() : (i == 1 ? s2.value() : (i == 2 ? s3.value() : ga[i - 3].value())); },
+                    lane, raw, false);
+                wave_reduce_store<P>([&](int i) { return gb[i].value(); }, lane, raw + 3 + P, false);
+                wave_reduce_store<P>([&](int i) { return gc[i].value(); }, lane, raw + 3 + 2 * P, false);
+            } else {
+                wave_reduce_store<1 + P>([&](int i) { return i == 0 ? s2.value() : ga[i - 1].value(); }, lane, raw,
+                                         false);
+            }
+            WAVE_SYNC();
+            if constexpr (ZNSSD) {
+                // deformed mean and norm (vector_norm, :31-39) from the shifted sums
+                const double S1 = raw[0], md = S1 / (double)ns, var = raw[1] - S1 * md;
+                const double gnorm = sqrt(var);
+                if (!(var > 0.0) || gnorm <= kEps) {
                     status = DIC_NUMERICAL_FAILURE;
                     failed = true;
                     break;
                 }
+                const double ig = 1.0 / gnorm, gsc = 2.0 / fnorm;
+                if (lane < P)
+                    sh.red[lane] = gsc * sd_half(lane) *
+                                   (raw[3 + lane] - (raw[3 + P + lane] - md * raw[3 + 2 * P + lane]) * ig);
+                if (lane == 0) sh.red[P] = sh.sfh2 - 2.0 * (raw[2] - md * sh.sfh) * ig + var * (ig * ig);
+            } else {
+                if (lane < P) sh.red[lane] = sd_half(lane) * raw[1 + lane];
+                if (lane == 0) sh.red[P] = raw[0];
             }
-            // ---- pass C: G and C (:335-350) ----
-            {
-                const R fm = (R)fmean, fin = (R)(1.0 / fnorm), gm = (R)gmean, gin = (R)(1.0 / gnorm);
-                Acc<R> ag[P], ac;
-                for (int s = lane; s < a.nfull; s += 64) {
-                    int dx, dy;
-                    decode(a.table[s], dx, dy);
-                    const int x = cx + dx, y = cy + dy;
-                    if (!usable(x, y)) continue;
-                    const R f = (R)a.ref[(size_t)y * W + x];
-                    const R e = ZNSSD ? (f - fm) * fin - (gv[s] - gm) * gin : f - gv[s];
-                    R sd[P];
-                    steepest_descent<P, R>(gev[s], hev[s], (R)dx, (R)dy, sd);
-#pragma unroll
-                    for (int j = 0; j < P; ++j) ag[j].add(e * sd[j]);
-                    ac.add(e * e);
-                }
-                const double gsc = ZNSSD ? 2.0 / fnorm : 1.0;
-#pragma unroll
-                for (int j = 0; j < P; ++j) {
-                    const double v = wave_sum(ag[j].value()) * gsc;
-                    if (lane == 0) sh.red[j] = v;
-                }
-                const double v = wave_sum(ac.value());
-                if (lane == 0) sh.red[P] = v;
-            }
-            // ---- Hessian from moments of {ge^2, ge he, he^2} x monomials ----
-#pragma unroll
-            for (int T = 0; T < 3; ++T) {
-                Acc<R> m[NM];
-                for (int s = lane; s < a.nfull; s += 64) {
-                    int dx, dy;
-                    decode(a.table[s], dx, dy);
-                    if (!usable(cx + dx, cy + dy)) continue;
-                    const R ge = gev[s], he = hev[s];
-                    const R w = (T == 0) ? ge * ge : ((T == 1) ? ge * he : he * he);
-                    const R X = (R)dx, Y = (R)dy, X2 = X * X, Y2 = Y * Y, XY = X * Y;
-                    const R mono[15] = {R(1), X, Y, X2, XY, Y2, X2 * X, X2 * Y, X * Y2, Y2 * Y,
-                                        X2 * X2, X2 * XY, X2 * Y2, XY * Y2, Y2 * Y2};
-#pragma unroll
-                    for (int k = 0; k < NM; ++k) m[k].add(w * mono[k]);
-                }
-#pragma unroll
-                for (int k = 0; k < NM; ++k) {
-                    const double v = wave_sum(m[k].value());
-                    if (lane == 0) sh.mom[T][k] = v;
-                }
-            }
-            __syncthreads();
             {
                 const double hsc = ZNSSD ? 2.0 / (fnorm * fnorm) : 1.0;
                 for (int e = lane; e < P * P; e += 64) {
@@ -1310,7 +1298,7 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
                     sh.H[e] = hsc * sd_half(ia) * sd_half(ib) * sh.mom[ma.t + mb.t][d * (d + 1) / 2 + qq];
                 }
             }
-            __syncthreads();
+            WAVE_SYNC();
             if constexpr (Rules::lm) {
                 if (lane == 0) {
                     double mx = sh.H[0];
@@ -1318,31 +1306,45 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
                     const double lambda = 1e-4 * mx;
                     for (int i = 0; i < P; ++i) sh.H[i * P + i] += lambda;
                 }
-                __syncthreads();
+                WAVE_SYNC();
             }
             int ok, pos;
-            if constexpr (P == 12) {
-                ldlt_factor_wave<P>(sh, lane, ok, pos);
-            } else {
-                if (lane == 0) {
-                    ldlt_factor<P>(sh.H, sh.tr, sh.x, &ok, &pos);
-                    sh.state = ok && pos;
+            const bool fast = ldlt_operator_fast<P>(sh, lane, ok, pos);
+            if (!fast) { // tied or NaN diagonal: Eigen's exchanges step by step
+                if constexpr (P == 12) {
+                    ldlt_factor_wave<P>(sh, lane, ok, pos);
+                } else {
+                    if (lane == 0) {
+                        ldlt_factor<P>(sh.H, sh.tr, sh.x, &ok, &pos);
+                        sh.state = ok && pos;
+                    }
+                    WAVE_SYNC();
+                    ok = pos = sh.state;
                 }
-                __syncthreads();
-                ok = pos = sh.state;
             }
             if (!(ok && pos)) {
                 status = DIC_NUMERICAL_FAILURE;
                 failed = true;
                 break;
             }
+            if (!fast) {
+                ldlt_operator_column<P>(sh, lane);
+                WAVE_SYNC();
+            }
             corr = sh.red[P];
-            // lane 0: dp = +H^-1 G, cap, compose (:351-390)
+            // dp = +H^-1 G (:351-355): lane i forms row i of the operator product
+            if (lane < P) {
+                double d = 0.0;
+#pragma unroll
+                for (int j = 0; j < P; ++j) d += sh.Hinv[lane * P + j] * sh.red[j];
+                sh.dp[lane] = d;
+            }
+            WAVE_SYNC();
+            // lane 0: cap and compose (:356-390)
             if (lane == 0) {
                 double dp[12] = {0.0};
 #pragma unroll
-                for (int j = 0; j < P; ++j) dp[j] = sh.red[j];
-                ldlt_solve_vec<P>(sh, dp);
+                for (int j = 0; j < P; ++j) dp[j] = sh.dp[j];
                 int st = 0; // 0 go on, 1 converged, 2 failed
                 bool fin = true;
                 double nrm = 0.0;
@@ -1372,7 +1374,7 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
                 }
                 sh.state = st;
             }
-            __syncthreads();
+            WAVE_SYNC();
             const int st = sh.state;
             if (st == 2) {
                 status = DIC_NUMERICAL_FAILURE;
@@ -1416,41 +1418,37 @@ __device__ __forceinline__ void fgn_solve_poi(const IcgnArgs &a, const int poi, 
 }
 
 template <int P, int OBJ, int D, typename R, bool MASKED>
-__global__ __launch_bounds__(64, (kMinWaves<P, R>)) void fgn_kernel(IcgnArgs a)
+__global__ __launch_bounds__(64, (kFgnMinWaves<P, R>)) void fgn_kernel(IcgnArgs a)
 {
     __shared__ IcgnShared sh;
-    extern __shared__ double fbuf[];
     const int per = (a.n + 7) >> 3;
     const int poi = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
     if (poi >= a.n) return;
-    fgn_solve_poi<P, OBJ, D, R, MASKED>(a, poi, sh, reinterpret_cast<R *>(fbuf));
+    fgn_solve_poi<P, OBJ, D, R, MASKED>(a, poi, sh);
 }
 
 // Batch FGN descriptors: one POI per workgroup, item after item (the
-// dispatch order the item table holds); the per-sample buffer is sized for
-// the group's largest subset.
+// dispatch order the item table holds).
 template <int P, int OBJ, int D, typename R>
-__global__ __launch_bounds__(64, (kMinWaves<P, R>)) void fgn_batch_kernel(const BatchPair *__restrict__ pairs,
-                                                                          const int2 *__restrict__ items, int nitems,
-                                                                          unsigned *, int)
+__global__ __launch_bounds__(64, (kFgnMinWaves<P, R>)) void fgn_batch_kernel(const BatchPair *__restrict__ pairs,
+                                                                             const int2 *__restrict__ items,
+                                                                             int nitems, unsigned *, int)
 {
     __shared__ IcgnShared sh;
-    extern __shared__ double fbuf[];
     if ((int)blockIdx.x >= nitems) return;
     const int2 it = items[blockIdx.x];
     const int pid = __builtin_amdgcn_readfirstlane(it.x);
     const int poi = __builtin_amdgcn_readfirstlane(it.y);
-    fgn_solve_poi<P, OBJ, D, R, false>(pairs[pid].a, poi, sh, reinterpret_cast<R *>(fbuf));
+    fgn_solve_poi<P, OBJ, D, R, false>(pairs[pid].a, poi, sh);
 }
 
-// Serving ticks with FGN: as icgn_range_kernel, one POI per workgroup, the
-// per-sample buffer sized for the server's subset.
+// Serving ticks with FGN: as icgn_range_kernel, one POI per workgroup.
 template <int P, int OBJ, int D, typename R>
-__global__ __launch_bounds__(64, (kMinWaves<P, R>)) void fgn_range_kernel(const BatchPair *__restrict__ pairs,
-                                                                          const int *__restrict__ offsets, int npairs)
+__global__ __launch_bounds__(64, (kFgnMinWaves<P, R>)) void fgn_range_kernel(const BatchPair *__restrict__ pairs,
+                                                                             const int *__restrict__ offsets,
+                                                                             int npairs)
 {
     __shared__ IcgnShared sh;
-    extern __shared__ double fbuf[];
     const int g = (int)blockIdx.x;
     if (g >= offsets[npairs]) return;
     int lo = 0, hi = npairs;
@@ -1460,19 +1458,13 @@ __global__ __launch_bounds__(64, (kMinWaves<P, R>)) void fgn_range_kernel(const 
         else hi = mid;
     }
     const int pid = __builtin_amdgcn_readfirstlane(lo);
-    fgn_solve_poi<P, OBJ, D, R, false>(pairs[pid].a, g - offsets[pid], sh, reinterpret_cast<R *>(fbuf));
+    fgn_solve_poi<P, OBJ, D, R, false>(pairs[pid].a, g - offsets[pid], sh);
 }
 
 template <int P, int OBJ, int D, typename R, bool MASKED>
 hipError_t launch_fgn_variant(const IcgnArgs &args, hipStream_t s)
 {
-    const size_t lds = 3 * sizeof(R) * (size_t)args.nfull;
-    auto kern = fgn_kernel<P, OBJ, D, R, MASKED>;
-    if (lds > 48 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(kern, dim3(8 * ((args.n + 7) / 8)), dim3(64), lds, s, args);
+    hipLaunchKernelGGL((fgn_kernel<P, OBJ, D, R, MASKED>), dim3(8 * ((args.n + 7) / 8)), dim3(64), 0, s, args);
     return hipGetLastError();
 }
 
