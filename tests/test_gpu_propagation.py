@@ -197,3 +197,59 @@ def test_gpu_frontier_order_accepts_what_heap_order_accepts_at_lenient_limit(pkg
     assert nh == nf and np.array_equal(heap["valid"], front["valid"])
     v = heap["valid"] == 1
     assert np.abs(heap["p"][v][:, :2] - front["p"][v][:, :2]).max() <= 1e-4
+
+
+@pytest.mark.parametrize("order,criterion,masked,limit", [(2, "znssd", False, 0.0082), (1, "ssd", False, 1.3),
+                                                          (1, "znssd", True, 0.0082), (2, "ssd", True, 1.3)])
+def test_gpu_heap_order_walk_other_variants_equal_oracle_walk(pkg, ctx, orc, stereo, order, criterion, masked,
+                                                              limit):
+    """The heap-order walk solves a few POIs per launch, each on a workgroup of
+    several waves that share its samples; over the solver variants the
+    reference's factory builds (first / second order, ZNSSD / SSD, with and
+    without ROI-truncated subsets), at acceptance limits that reject part of
+    the window (near the median correlation: ZNSSD 0.0082, SSD 1.3), the valid
+    set, the count and u, v (1e-5 px) equal the oracle's best-first walk."""
+    ref, dfm, roi, seed_row, _ = crop_case(stereo, "left_temporal")
+    pre = orc.Precomputed(dfm, degree=5)
+    gx, gy = orc.reference_gradient_cache(ref)
+    obj = 1 if criterion == "ssd" else 0
+    cpu, ncpu = orc.propagate(ref, pre, gx, gy, roi, seed_row, radius=41, max_iterations=50, threshold=1e-6,
+                              shape_order=order, objective=obj, spacing=3, max_znssd=limit, truncate_roi=masked)
+    s = seed_row[0]
+    seed = _seed(pkg, s[0], s[1], s[2], s[3], s[4:8], s[8])
+    cfg = pkg.config_from_dict({
+        "subset": {"radius": 41}, "shape_function": {"order": order},
+        "optimization": {"method": "icgn", "max_iterations": 50, "convergence_threshold": 1e-6},
+        "correlation": {"criterion": criterion}, "interpolation": {"degree": 5},
+        "reliability_propagation": {"spacing": 3, "max_znssd": limit}})
+    cfg.truncate_roi_subsets = masked
+    solver = pkg.ICGNSolver(cfg, precision="fp64", context=ctx)
+    _, gpu, ngpu = pkg.propagate(ref, dfm, roi, seed, solver=solver)
+    v = cpu["valid"] == 1
+    d = np.abs(gpu["p"][v][:, :2] - cpu["p"][v][:, :2]).max(axis=1) if v.any() else np.zeros(1)
+    _report(f"heap walk order {order} {criterion} masked={masked}: gpu {ngpu} / oracle {ncpu} valid, "
+            f"max |du| {d.max():.2e}")
+    assert ngpu == ncpu and ncpu > 100 and np.array_equal(gpu["valid"], cpu["valid"])
+    assert d.max() <= 1e-5
+
+
+def test_gpu_heap_order_walk_second_order_mixed_follows_fp64(pkg, ctx, orc, stereo):
+    """Second-order ZNSSD walks in mixed precision and FP64 at the golden
+    limit (2.0, every solved point accepted): the same valid set, u, v within
+    1e-3 px."""
+    ref, dfm, roi, seed_row, _ = crop_case(stereo, "left_temporal")
+    s = seed_row[0]
+    seed = _seed(pkg, s[0], s[1], s[2], s[3], s[4:8], s[8])
+    cfg = pkg.config_from_dict({
+        "subset": {"radius": 41}, "shape_function": {"order": 2},
+        "optimization": {"method": "icgn", "max_iterations": 50, "convergence_threshold": 1e-6},
+        "interpolation": {"degree": 5}, "reliability_propagation": {"spacing": 3, "max_znssd": 2.0}})
+    out = {}
+    for prec in ("fp64", "mixed"):
+        solver = pkg.ICGNSolver(cfg, precision=prec, context=ctx)
+        _, out[prec], n = pkg.propagate(ref, dfm, roi, seed, solver=solver)
+    v = out["fp64"]["valid"] == 1
+    d = np.abs(out["mixed"]["p"][v][:, :2] - out["fp64"]["p"][v][:, :2]).max(axis=1)
+    _report(f"second-order heap walk mixed vs fp64: {int(v.sum())} valid, max |du| {d.max():.2e}")
+    assert v.sum() > 1000 and np.array_equal(out["mixed"]["valid"], out["fp64"]["valid"])
+    assert d.max() <= 1e-3

@@ -194,10 +194,13 @@ struct IcgnLaunch {
     const dic_initial *init;
     dic_result *out;
     int n;
-    // first-order FP32 launch shape: 1 four POIs per workgroup over an LDS
-    // tile, 0 one POI per workgroup, -1 decide by the radius (grouped_radius)
+    // launch shape: kShapeTile four POIs per workgroup over an LDS tile
+    // (first-order FP32 IC-GN), kShapeWave one POI per 64-lane workgroup,
+    // kShapeCoop one POI per workgroup of several waves (IC-GN; a few POIs at
+    // a time), -1 decide by the radius (grouped_radius)
     int tile_hint = -1;
 };
+enum { kShapeWave = 0, kShapeTile = 1, kShapeCoop = 2 };
 hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s);
 
 

@@ -44,9 +44,19 @@ namespace {
 
 // One POI on the calling wave (64 lanes, the whole workgroup): sh is the
 // workgroup's LDS record, gbuf (FP64 ZNSSD only) holds nfull doubles.
-template <int P, int OBJ, int D, typename R, bool MASKED>
+// Per-wave totals of a POI solved by NW > 1 waves (one workgroup), summed in
+// wave order so that every wave holds the same totals.
+template <int NW> struct CoopSlots {
+    double sum[NW][16];
+    double mom[NW][72]; // each wave's share of the reference-side moments
+    int flag[NW];
+    int bcast[2];
+};
+
+template <int P, int OBJ, int D, typename R, bool MASKED, int NW = 1>
 __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, IcgnShared &sh, double *gbuf,
-                                          const Tile tile = Tile{nullptr, 0, 0, 0, 0})
+                                          const Tile tile = Tile{nullptr, 0, 0, 0, 0},
+                                          CoopSlots<NW> *cs = nullptr)
 {
     constexpr bool ZNSSD = OBJ == DIC_OBJECTIVE_ZNSSD;
     constexpr bool FP64 = sizeof(R) == 8;
@@ -55,7 +65,34 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
     // data path; the second-order ones by the vector ALU)
     constexpr bool TILED = !FP64 && !MASKED && P == 6;
 
-    const int lane = threadIdx.x & 63; // one POI per wave
+    const int lane = threadIdx.x & 63; // one POI per wave, or per workgroup of NW waves
+    // NW > 1: wave 0 forms the reference-side sums, the factorisation, the
+    // solve and the update; every wave takes a share of the samples in each
+    // iteration, and the per-wave totals meet in cs
+    const int wv = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
+    const bool leader = NW > 1 ? threadIdx.x == 0 : lane == 0;
+    auto sync = [] {
+        if constexpr (NW > 1) __syncthreads();
+        else WAVE_SYNC();
+    };
+    // totals over the workgroup of n per-wave totals v[0..n) (uniform in each wave)
+    auto coop_sum = [&](double *v, auto n) {
+        if constexpr (NW > 1) {
+            if (lane == 0) {
+#pragma unroll
+                for (int k = 0; k < decltype(n)::value; ++k) cs->sum[wv][k] = v[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < decltype(n)::value; ++k) {
+                double t = 0.0;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) t += cs->sum[w][k];
+                v[k] = t;
+            }
+            __syncthreads();
+        }
+    };
     const R *__restrict__ gxp = (const R *)a.gx;
     const R *__restrict__ gyp = (const R *)a.gy;
     const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
@@ -133,6 +170,10 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
         p = d - q;
     };
     double *mred = sh.Hinv; // reduced moments, until the solve operator is formed
+    // NW > 1: wave w walks the rows t = w, w + NW, ... of every column and
+    // reduces its share of the moments into cs->mom[w]
+    double *wred = mred;
+    if constexpr (NW > 1) wred = cs->mom[wv];
     if (!early) {
         double fs = 0.0;
         int cnt = 0;
@@ -140,7 +181,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
             int dx, h;
             column(g, dx, h);
             const int x = cx + dx;
-            for (int t = 0; t <= h; ++t) {
+            for (int t = wv; t <= h; t += NW) {
                 if (usable(x, cy + t)) {
                     fs += (double)a.ref[(size_t)(cy + t) * W + x];
                     ++cnt;
@@ -151,14 +192,16 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 }
             }
         }
-        const int ns = MASKED ? wave_sum_i(cnt) : a.nfull;
+        double fc[2] = {wave_sum(fs), (double)wave_sum_i(cnt)};
+        coop_sum(fc, std::integral_constant<int, 2>{});
+        const int ns = MASKED ? (int)fc[1] : a.nfull;
         if (ns < a.min_needed) {
             early = true;
         } else {
-            const double fmean = ZNSSD ? wave_sum(fs) / (double)ns : 0.0;
+            const double fmean = ZNSSD ? fc[0] / (double)ns : 0.0;
             // column walk: calls visit(t, usable at +t, at -t, pixel offsets) per row pair
             auto walk = [&](int x, int h, auto &&visit) {
-                for (int t = 0; t <= h; ++t) {
+                for (int t = wv; t <= h; t += NW) {
                     const bool u1 = usable(x, cy + t), u2 = t > 0 && usable(x, cy - t);
                     visit((double)t, u1, u2, (size_t)(cy + t) * W + x, (size_t)(cy - t) * W + x);
                 }
@@ -212,10 +255,10 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                         acc(cH[1], qh, t1, gx1 * gy1, gx2 * gy2);
                         acc(cH[2], qh, t1, gy1 * gy1, gy2 * gy2);
                     });
-                    wave_reduce_store<NH + 1>([&](int i) { return i < NH ? moment(cH[0], i) : n2; }, lane, mred,
+                    wave_reduce_store<NH + 1>([&](int i) { return i < NH ? moment(cH[0], i) : n2; }, lane, wred,
                                               g > 0);
-                    wave_reduce_store<NH>([&](int i) { return moment(cH[1], i); }, lane, mred + NH + 1, g > 0);
-                    wave_reduce_store<NH>([&](int i) { return moment(cH[2], i); }, lane, mred + 2 * NH + 1, g > 0);
+                    wave_reduce_store<NH>([&](int i) { return moment(cH[1], i); }, lane, wred + NH + 1, g > 0);
+                    wave_reduce_store<NH>([&](int i) { return moment(cH[2], i); }, lane, wred + 2 * NH + 1, g > 0);
                 }
                 if constexpr (REFSUMS) { // sum sd, sum (f - mean) sd, sum (f - mean)
                     double cG[2][QG + 1], cF[2][QG + 1], sf = 0.0;
@@ -241,27 +284,42 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                     });
                     wave_reduce_store<2 * NG + 1>(
                         [&](int i) { return i < NG ? moment(cG[0], i) : (i < 2 * NG ? moment(cG[1], i - NG) : sf); },
-                        lane, mred + kRefG, g > 0);
+                        lane, wred + kRefG, g > 0);
                     wave_reduce_store<2 * NG>(
                         [&](int i) { return i < NG ? moment(cF[0], i) : moment(cF[1], i - NG); }, lane,
-                        mred + kRefF, g > 0);
+                        wred + kRefF, g > 0);
+                }
+            }
+            if constexpr (NW > 1) { // the waves' shares, summed in wave order
+                constexpr int kMoments = REFSUMS ? kRefF + 2 * NG : 3 * NH + 1;
+                static_assert(kMoments <= 72, "moment slots");
+                __syncthreads();
+                if (wv == 0) {
+                    for (int k = lane; k < kMoments; k += 64) {
+                        double t = 0.0;
+#pragma unroll
+                        for (int w = 0; w < NW; ++w) t += cs->mom[w][k];
+                        mred[k] = t;
+                    }
                 }
             }
             WAVE_SYNC();
+            // the rest of the setup on wave 0 (its outcome reaches the other
+            // waves after the factorisation)
             double fnorm = 1.0;
-            if constexpr (ZNSSD) {
+            if (ZNSSD && wv == 0) {
                 fnorm = sqrt(mred[kNorm]);
                 if (fnorm <= kEps) {
                     early = true;
                     status = DIC_NUMERICAL_FAILURE;
                 }
             }
-            if (lane == 0) {
+            if (leader) {
                 sh.fmean = fmean;
                 sh.fnorm = fnorm;
                 sh.ns = ns;
             }
-            if (!early) {
+            if (!early && wv == 0) {
                 // H = c * sum sd sd^T (icgn.cpp:532-536): entry (a, b) is c_a c_b times
                 // the moment of g_t(a) g_t(b) dx^(p_a + p_b) dy^(q_a + q_b)
                 const double hscale = ZNSSD ? 2.0 / (fnorm * fnorm) : 1.0;
@@ -288,7 +346,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
         }
     }
 
-    if (!early) {
+    if (!early && wv == 0) {
         WAVE_SYNC();
         int ok, pos;
         const bool fast = ldlt_operator_fast<P>(sh, lane, ok, pos);
@@ -312,11 +370,20 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
             WAVE_SYNC();
         }
     }
+    if constexpr (NW > 1) { // wave 0's outcome to every wave
+        if (leader) {
+            cs->bcast[0] = early ? 1 : 0;
+            cs->bcast[1] = status;
+        }
+        __syncthreads();
+        early = cs->bcast[0] != 0;
+        status = cs->bcast[1];
+    }
 
     double corr = __builtin_inf();
     bool solved = false;
     if (!early) {
-        if (lane == 0) {
+        if (leader) {
             const dic_initial ini = a.init[poi];
             sh.p[0] = ini.u;
             sh.p[1] = ini.v;
@@ -327,7 +394,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
 #pragma unroll
             for (int j = 6; j < 12; ++j) sh.p[j] = 0.0;
         }
-        WAVE_SYNC();
+        sync();
         const double Wm1 = (double)(W - 1), Hm1 = (double)(Hh - 1);
         const int ns = sh.ns;
         const double fmean = uniform_f64(sh.fmean), fnorm = uniform_f64(sh.fnorm);
@@ -383,11 +450,12 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
             auto sample_pass = [&](auto from_tile) {
             // the next sample's table entry is loaded one sample ahead (reads
             // past the table return 0), so a sample waits for its gather only
-            int tnext = (int)__builtin_amdgcn_raw_buffer_load_b32(trs, 4 * lane, 0, 0);
-            for (int s = wild ? a.nfull : lane; s < a.nfull; s += 64) {
+            const int first = lane + 64 * wv;
+            int tnext = (int)__builtin_amdgcn_raw_buffer_load_b32(trs, 4 * first, 0, 0);
+            for (int s = wild ? a.nfull : first; s < a.nfull; s += 64 * NW) {
                 int dx, dy;
                 const int tcur = tnext;
-                tnext = (int)__builtin_amdgcn_raw_buffer_load_b32(trs, 4 * (s + 64), 0, 0);
+                tnext = (int)__builtin_amdgcn_raw_buffer_load_b32(trs, 4 * (s + 64 * NW), 0, 0);
                 decode(tcur, dx, dy);
                 const int x = cx + dx, y = cy + dy;
                 if (!usable(x, y)) continue;
@@ -502,17 +570,29 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
             } else {
                 sample_pass(std::integral_constant<bool, false>{});
             }
-            if (__any(oob)) {
+            bool any_oob = __any(oob);
+            if constexpr (NW > 1) {
+                if (lane == 0) cs->flag[wv] = any_oob ? 1 : 0;
+                __syncthreads();
+#pragma unroll
+                for (int w = 0; w < NW; ++w) any_oob = any_oob || cs->flag[w] != 0;
+                __syncthreads();
+            }
+            if (any_oob) {
                 status = DIC_INVALID_INPUT;
                 failed = true;
                 break;
             }
+            constexpr std::integral_constant<int, 1> one{};
+            constexpr std::integral_constant<int, P + 1> g_and_c{};
             // ---- G and C into sh.red (lane 0 stores each wave total) ----
             if constexpr (TWOPASS) {
                 // two-pass mean / norm / residual, icgn.cpp:577-592
-                const double gmean = wave_sum(s1.value()) / (double)ns;
+                double t1[1] = {wave_sum(s1.value())};
+                coop_sum(t1, one);
+                const double gmean = t1[0] / (double)ns;
                 double acc = 0.0;
-                for (int s = lane; s < a.nfull; s += 64) {
+                for (int s = lane + 64 * wv; s < a.nfull; s += 64 * NW) {
                     if constexpr (MASKED) {
                         int dx, dy;
                         decode(a.table[s], dx, dy);
@@ -521,7 +601,9 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                     const double dg = gbuf[s] - gmean;
                     acc += dg * dg;
                 }
-                const double gnorm = sqrt(wave_sum(acc));
+                double t2[1] = {wave_sum(acc)};
+                coop_sum(t2, one);
+                const double gnorm = sqrt(t2[0]);
                 if (gnorm <= kEps) {
                     status = DIC_NUMERICAL_FAILURE;
                     failed = true;
@@ -531,7 +613,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
 #pragma unroll
                 for (int j = 0; j < P; ++j) gl[j] = 0.0;
                 double cl = 0.0;
-                for (int s = lane; s < a.nfull; s += 64) {
+                for (int s = lane + 64 * wv; s < a.nfull; s += 64 * NW) {
                     int dx, dy;
                     decode(a.table[s], dx, dy);
                     const int x = cx + dx, y = cy + dy;
@@ -546,16 +628,27 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                     cl += e * e;
                 }
                 const double sc = 2.0 / fnorm;
+                double t[P + 1];
+#pragma unroll
+                for (int j = 0; j < P; ++j) t[j] = wave_sum(gl[j]);
+                t[P] = wave_sum(cl);
+                coop_sum(t, g_and_c);
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
-                    const double v = wave_sum(gl[j]) * sc;
-                    if (lane == 0) sh.red[j] = v;
+                    const double v = t[j] * sc;
+                    if (leader) sh.red[j] = v;
                 }
-                const double v = wave_sum(cl);
-                if (lane == 0) sh.red[P] = v;
+                if (leader) sh.red[P] = t[P];
             } else if constexpr (ZNSSD) {
                 // FP32 kernels: lane sums, then a pairwise FP32 tree over the wave (DPP)
-                const double S1 = wave_total_f32(s1.s), S2 = wave_total_f32(s2.s), SFG = wave_total_f32(sfg.s);
+                double t[P + 3];
+#pragma unroll
+                for (int j = 0; j < P; ++j) t[j] = (double)wave_total_f32(sg[j].s);
+                t[P] = wave_total_f32(s1.s);
+                t[P + 1] = wave_total_f32(s2.s);
+                t[P + 2] = wave_total_f32(sfg.s);
+                coop_sum(t, std::integral_constant<int, P + 3>{});
+                const double S1 = t[P], S2 = t[P + 1], SFG = t[P + 2];
                 const double md = S1 / (double)ns;
                 const double var = S2 - S1 * md;
                 if (!(var > 0.0) || sqrt(var) <= kEps) {
@@ -566,37 +659,38 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 const double gnorm = sqrt(var), sc = 2.0 / fnorm, ig = 1.0 / gnorm;
 #pragma unroll
                 for (int j = 0; j < P; ++j) { // FP32 only: sg holds the monomial sums
-                    const double v = (double)wave_total_f32(sg[j].s) * sd_half(j);
-                    if (lane == 0) sh.red[j] = sc * (sh.sfsd[j] - (v - md * sh.ssd[j]) * ig);
+                    const double v = t[j] * sd_half(j);
+                    if (leader) sh.red[j] = sc * (sh.sfsd[j] - (v - md * sh.ssd[j]) * ig);
                 }
-                if (lane == 0) sh.red[P] = sh.sfh2 - 2.0 * (SFG - md * sh.sfh) * ig + var * (ig * ig);
+                if (leader) sh.red[P] = sh.sfh2 - 2.0 * (SFG - md * sh.sfh) * ig + var * (ig * ig);
             } else {
+                double t[P + 1];
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
-                    double v;
                     if constexpr (FP64)
-                        v = wave_sum(sg[j].value());
+                        t[j] = wave_sum(sg[j].value());
                     else
-                        v = (double)wave_total_f32(sg[j].s) * sd_half(j);
-                    if (lane == 0) sh.red[j] = v;
+                        t[j] = (double)wave_total_f32(sg[j].s) * sd_half(j);
                 }
-                double v;
                 if constexpr (FP64)
-                    v = wave_sum(s2.value());
+                    t[P] = wave_sum(s2.value());
                 else
-                    v = (double)wave_total_f32(s2.s);
-                if (lane == 0) sh.red[P] = v;
+                    t[P] = (double)wave_total_f32(s2.s);
+                coop_sum(t, g_and_c);
+#pragma unroll
+                for (int j = 0; j <= P; ++j)
+                    if (leader) sh.red[j] = t[j];
             }
-            WAVE_SYNC();
+            sync();
             corr = sh.red[P];
             // ---- dp = -H+ G (icgn.cpp:594): lane i forms row i ----
-            if (lane < P) {
+            if (wv == 0 && lane < P) {
                 double d = 0.0;
 #pragma unroll
                 for (int j = 0; j < P; ++j) d += sh.Hinv[lane * P + j] * sh.red[j];
                 sh.dp[lane] = -d;
             }
-            WAVE_SYNC();
+            sync();
             bool fin = true;
             double nrm = 0.0;
 #pragma unroll
@@ -612,8 +706,10 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
             }
             nrm = sqrt(nrm);
             bool invertible = true;
-            if constexpr (P == 12) invertible = second_order_solve_wave(sh, lane);
-            if (lane == 0) {
+            if constexpr (P == 12) {
+                if (wv == 0) invertible = second_order_solve_wave(sh, lane);
+            }
+            if (leader) {
                 double pn[12];
                 if constexpr (P == 6) {
                     ic_affine_update(sh.p, sh.dp, pn);
@@ -643,7 +739,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
                 }
                 sh.state = ok;
             }
-            WAVE_SYNC();
+            sync();
             if (!sh.state) {
                 status = DIC_NUMERICAL_FAILURE;
                 failed = true;
@@ -662,7 +758,7 @@ __device__ __forceinline__ void solve_poi(const IcgnArgs &a, const int poi, Icgn
         }
     }
 
-    if (lane == 0) {
+    if (leader) {
         const dic_initial ini = a.init[poi];
         dic_result r;
         r.x = px;
@@ -703,6 +799,21 @@ __global__ __launch_bounds__(64, (kMinWaves<P, R>)) void icgn_kernel(IcgnArgs a)
     const int poi = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
     if (poi >= a.n) return;
     solve_poi<P, OBJ, D, R, MASKED>(a, poi, sh, gbuf);
+}
+
+// A few POIs at a time (the heap-order propagation's launches): one POI per
+// workgroup of kCoopWaves waves, which share its samples in every iteration,
+// so that the launch takes about one POI's latency over kCoopWaves.
+constexpr int kCoopWaves = 8;
+template <int P, int OBJ, int D, typename R, bool MASKED>
+__global__ __launch_bounds__(64 * kCoopWaves) void icgn_coop_kernel(IcgnArgs a)
+{
+    __shared__ IcgnShared sh;
+    __shared__ CoopSlots<kCoopWaves> cs;
+    extern __shared__ double gbuf[];
+    const int poi = (int)blockIdx.x;
+    if (poi >= a.n) return;
+    solve_poi<P, OBJ, D, R, MASKED, kCoopWaves>(a, poi, sh, gbuf, Tile{nullptr, 0, 0, 0, 0}, &cs);
 }
 
 // ---- first-order FP32: kGroupPois neighbouring POIs per workgroup ----
@@ -1469,11 +1580,22 @@ hipError_t launch_fgn_variant(const IcgnArgs &args, hipStream_t s)
 }
 
 template <int P, int OBJ, int D, typename R, bool MASKED>
-hipError_t launch_variant(const IcgnArgs &args, bool group, hipStream_t s)
+hipError_t launch_variant(const IcgnArgs &args, int shape, hipStream_t s)
 {
     if (args.fgn) return launch_fgn_variant<P, OBJ, D, R, MASKED>(args, s);
     constexpr bool TWOPASS = OBJ == DIC_OBJECTIVE_ZNSSD && sizeof(R) == 8;
-    if (P == 6 && sizeof(R) == 4 && !MASKED && group) {
+    if (shape == kShapeCoop) {
+        const size_t lds = TWOPASS ? sizeof(double) * (size_t)args.nfull : 0;
+        auto kern = icgn_coop_kernel<P, OBJ, D, R, MASKED>;
+        if (lds > 48 * 1024) {
+            hipError_t e =
+                hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(kern, dim3(args.n), dim3(64 * kCoopWaves), lds, s, args);
+        return hipGetLastError();
+    }
+    if (P == 6 && sizeof(R) == 4 && !MASKED && shape == kShapeTile) {
         const int groups = (args.n + kGroupPois - 1) / kGroupPois;
         hipLaunchKernelGGL((icgn_group_kernel<P, OBJ, D>), dim3(8 * ((groups + 7) / 8)), dim3(64 * kGroupPois), 0, s,
                            args);
@@ -1490,26 +1612,26 @@ hipError_t launch_variant(const IcgnArgs &args, bool group, hipStream_t s)
 }
 
 template <int P, int OBJ, int D, typename R>
-hipError_t dispatch_masked(const IcgnArgs &args, bool masked, bool group, hipStream_t s)
+hipError_t dispatch_masked(const IcgnArgs &args, bool masked, int shape, hipStream_t s)
 {
-    return masked ? launch_variant<P, OBJ, D, R, true>(args, group, s)
-                  : launch_variant<P, OBJ, D, R, false>(args, group, s);
+    return masked ? launch_variant<P, OBJ, D, R, true>(args, shape, s)
+                  : launch_variant<P, OBJ, D, R, false>(args, shape, s);
 }
 
 template <int P, int OBJ, int D>
-hipError_t dispatch_precision(const IcgnArgs &args, int precision, bool masked, bool group, hipStream_t s)
+hipError_t dispatch_precision(const IcgnArgs &args, int precision, bool masked, int shape, hipStream_t s)
 {
-    return precision == DIC_PRECISION_FP64 ? dispatch_masked<P, OBJ, D, double>(args, masked, group, s)
-                                           : dispatch_masked<P, OBJ, D, float>(args, masked, group, s);
+    return precision == DIC_PRECISION_FP64 ? dispatch_masked<P, OBJ, D, double>(args, masked, shape, s)
+                                           : dispatch_masked<P, OBJ, D, float>(args, masked, shape, s);
 }
 
 template <int P, int OBJ>
-hipError_t dispatch_degree(const IcgnArgs &args, int degree, int precision, bool masked, bool group, hipStream_t s)
+hipError_t dispatch_degree(const IcgnArgs &args, int degree, int precision, bool masked, int shape, hipStream_t s)
 {
     switch (degree) {
-    case 1: return dispatch_precision<P, OBJ, 1>(args, precision, masked, group, s);
-    case 3: return dispatch_precision<P, OBJ, 3>(args, precision, masked, group, s);
-    case 5: return dispatch_precision<P, OBJ, 5>(args, precision, masked, group, s);
+    case 1: return dispatch_precision<P, OBJ, 1>(args, precision, masked, shape, s);
+    case 3: return dispatch_precision<P, OBJ, 3>(args, precision, masked, shape, s);
+    case 5: return dispatch_precision<P, OBJ, 5>(args, precision, masked, shape, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1580,12 +1702,12 @@ hipError_t launch_icgn(const IcgnLaunch &L, hipStream_t s)
     // first-order FP32: four POIs per workgroup over an LDS tile where the
     // caller's POI layout lets four neighbours share one (tile_hint), else,
     // with no layout known, wherever the radius allows it
-    const bool group = L.tile_hint < 0 ? grouped_radius(pr.subset_radius) : L.tile_hint != 0;
+    const int shape = L.tile_hint < 0 ? (grouped_radius(pr.subset_radius) ? kShapeTile : kShapeWave) : L.tile_hint;
     if (P == 6)
-        return ssd ? dispatch_degree<6, DIC_OBJECTIVE_SSD>(a, L.degree, L.precision, masked, group, s)
-                   : dispatch_degree<6, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, group, s);
-    return ssd ? dispatch_degree<12, DIC_OBJECTIVE_SSD>(a, L.degree, L.precision, masked, group, s)
-               : dispatch_degree<12, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, group, s);
+        return ssd ? dispatch_degree<6, DIC_OBJECTIVE_SSD>(a, L.degree, L.precision, masked, shape, s)
+                   : dispatch_degree<6, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, shape, s);
+    return ssd ? dispatch_degree<12, DIC_OBJECTIVE_SSD>(a, L.degree, L.precision, masked, shape, s)
+               : dispatch_degree<12, DIC_OBJECTIVE_ZNSSD>(a, L.degree, L.precision, masked, shape, s);
 }
 
 

@@ -5,6 +5,7 @@
 // (each frontier solved in one launch).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <unordered_map>
 #include <vector>
 
@@ -33,21 +34,42 @@ bool finite6(const dic_result &r)
     return std::isfinite(r.correlation);
 }
 
+// Device buffers of the walk's launches, and pinned host staging: a launch
+// packs its POIs and guesses into one pinned block and uploads it with one
+// copy, and its results come back into pinned memory (the walk's launches
+// are small, so copy latency counts).
 struct DeviceScratch {
     dic_ctx *ctx = nullptr;
-    double *xy = nullptr;
-    dic_initial *init = nullptr;
+    unsigned char *in = nullptr; // per launch: n POI positions (16 B each), then n guesses
     dic_result *out = nullptr;
     uint8_t *roi = nullptr;
     size_t cap = 0, roi_bytes = 0;
+    unsigned char *pin = nullptr; // pinned: the packed inputs, then the results
+    size_t pin_cap = 0;           // POIs the pinned block holds
+    static constexpr size_t kInBytes = 16 + sizeof(dic_initial);
     ~DeviceScratch()
     {
-        ctx_free(ctx, xy, 16 * cap);
-        ctx_free(ctx, init, sizeof(dic_initial) * cap);
+        ctx_free(ctx, in, kInBytes * cap);
         ctx_free(ctx, out, sizeof(dic_result) * cap);
         ctx_free(ctx, roi, roi_bytes);
+        if (pin) (void)hipHostFree(pin);
+    }
+    hipError_t pinned(size_t n)
+    {
+        if (n <= pin_cap) return hipSuccess;
+        const size_t want = std::max(n, 2 * pin_cap + 256);
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        pin_cap = 0;
+        hipError_t e = hipHostMalloc((void **)&pin, (kInBytes + sizeof(dic_result)) * want, hipHostMallocDefault);
+        if (e == hipSuccess) pin_cap = want;
+        return e;
     }
 };
+
+// Launches of at most this many IC-GN POIs give each POI a workgroup of
+// several waves: the walk's launches are latency-bound (a few POIs each).
+constexpr int kCoopMaxPois = 256;
 
 // one launch over n POIs (host arrays in, host results out)
 int solve_host(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
@@ -58,16 +80,25 @@ int solve_host(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref
     res.resize(n);
     if (n == 0) return DIC_OK;
     hipStream_t s = ctx->stream;
-    int rc = check_hip(ctx, hipMemcpyAsync(d.xy, xy.data(), 16 * (size_t)n, hipMemcpyHostToDevice, s), "POI upload");
+    int rc = check_hip(ctx, d.pinned((size_t)n), "pinned staging");
+    if (rc != DIC_OK) return rc;
+    const size_t xy_bytes = 16 * (size_t)n, in_bytes = DeviceScratch::kInBytes * (size_t)n;
+    std::memcpy(d.pin, xy.data(), xy_bytes);
+    std::memcpy(d.pin + xy_bytes, ini.data(), sizeof(dic_initial) * (size_t)n);
+    const double *dxy = reinterpret_cast<const double *>(d.in);
+    const dic_initial *dini = reinterpret_cast<const dic_initial *>(d.in + xy_bytes);
+    dic_result *pres = reinterpret_cast<dic_result *>(d.pin + DeviceScratch::kInBytes * d.pin_cap);
+    rc = check_hip(ctx, hipMemcpyAsync(d.in, d.pin, in_bytes, hipMemcpyHostToDevice, s), "POI upload");
+    const int shape = params->optimizer == DIC_OPTIMIZER_ICGN && n <= kCoopMaxPois
+                          ? kShapeCoop
+                          : host_tile_hint(params, def, xy.data(), ini.data(), n);
+    if (rc == DIC_OK) rc = run_solve(ctx, params, ref, grad, def, params->masked ? d.roi : nullptr, dxy, dini, n,
+                                     d.out, s, shape);
     if (rc == DIC_OK)
-        rc = check_hip(ctx, hipMemcpyAsync(d.init, ini.data(), sizeof(dic_initial) * n, hipMemcpyHostToDevice, s),
-                       "guess upload");
-    if (rc == DIC_OK) rc = run_solve(ctx, params, ref, grad, def, params->masked ? d.roi : nullptr, d.xy, d.init, n,
-                                     d.out, s, host_tile_hint(params, def, xy.data(), ini.data(), n));
-    if (rc == DIC_OK)
-        rc = check_hip(ctx, hipMemcpyAsync(res.data(), d.out, sizeof(dic_result) * n, hipMemcpyDeviceToHost, s),
+        rc = check_hip(ctx, hipMemcpyAsync(pres, d.out, sizeof(dic_result) * n, hipMemcpyDeviceToHost, s),
                        "result download");
     if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(s), "propagation sync");
+    if (rc == DIC_OK) std::memcpy(res.data(), pres, sizeof(dic_result) * (size_t)n);
     return rc;
 }
 
@@ -239,8 +270,7 @@ extern "C" int dic_propagate(dic_ctx *ctx, const dic_icgn_params *params, const 
     DeviceScratch d;
     d.ctx = ctx;
     d.cap = (size_t)std::max(total, 1);
-    rc = check_hip(ctx, ctx_malloc(ctx, (void **)&d.xy, 16 * d.cap), "propagation buffers");
-    if (rc == DIC_OK) rc = check_hip(ctx, ctx_malloc(ctx, (void **)&d.init, sizeof(dic_initial) * d.cap), "buffers");
+    rc = check_hip(ctx, ctx_malloc(ctx, (void **)&d.in, DeviceScratch::kInBytes * d.cap), "propagation buffers");
     if (rc == DIC_OK) rc = check_hip(ctx, ctx_malloc(ctx, (void **)&d.out, sizeof(dic_result) * d.cap), "buffers");
     if (rc == DIC_OK && params->masked) {
         d.roi_bytes = (size_t)W * H;
