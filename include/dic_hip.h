@@ -282,8 +282,8 @@ void dic_host_free(void *ptr);
  * extrapolation along the sequence (guess_extrapolate).  dic_server_submit
  * returns at once; a worker thread admits pending requests in arrival order
  * while the number of active requests stays below max_active and the device
- * bytes they hold (reference, gradients, two frame buffers, 3K coefficient,
- * guess and result buffers, POIs, the request's guesses) stay within
+ * bytes they hold (reference, gradients, 3K frame, coefficient, guess and
+ * result buffers, POIs, the request's guesses) stay within
  * hbm_budget.  Each tick is one launch over the next frame of every active
  * request and, while the tick holds fewer than tick_pois POIs, up to K - 1
  * more of each; a request that arrives while a tick runs joins the next tick,

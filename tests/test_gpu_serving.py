@@ -161,15 +161,15 @@ def test_alone_and_in_mix_are_identical(pkg, ctx, seqs, lag, extrapolate):
 
 
 def _slab_bytes(w, h, n, elem=4, lag=1):
-    """Device bytes a request holds: reference, two frame buffers and two
-    gradient planes, POIs and the request's guesses, and 3K coefficient,
-    guess and result buffers (three ticks in flight, K = the guess lag);
-    each array rounded up to 256 bytes."""
+    """Device bytes a request holds: reference, two gradient planes, POIs
+    and the request's guesses, and 3K frame, coefficient, guess and result
+    buffers (three ticks in flight, K = the guess lag); each array rounded up
+    to 256 bytes."""
     def a(v):
         return (v + 255) // 256 * 256
     px, cpx = w * h, (w + 6) * (h + 6)
-    return (3 * a(4 * px) + 2 * a(elem * px) + a(16 * n) + a(56 * n)
-            + 3 * lag * (a(elem * cpx) + a(56 * n) + a(136 * n)))
+    return (a(4 * px) + 2 * a(elem * px) + a(16 * n) + a(56 * n)
+            + 3 * lag * (a(4 * px) + a(elem * cpx) + a(56 * n) + a(136 * n)))
 
 
 def test_budget_admission_and_refusal(pkg, ctx, seqs):

@@ -1,9 +1,15 @@
 // server.hip -- the serving loop (dic_server_*): admission by HBM budget and
 // slots, continuous batching of up to K frames per active request per tick
 // (K = the guess lag: frame f starts from frame f - K's results, frames below
-// K from the request's guesses), three ticks in flight over three streams
-// (prep: uploads + prefilters, compute: initial guesses + IC-GN, copy: results
-// back to pinned host memory).
+// K from the request's guesses), three ticks in flight over two streams
+// (prep: frame uploads and result downloads; compute: reference gradients,
+// prefilters, initial guesses, IC-GN).  A tick's uploads run on the copy
+// engines while the previous tick's solve holds the CUs: the kernels that
+// prepare a tick run on the compute stream, so that no upload waits behind a
+// kernel that cannot get a CU, and a tick's downloads are queued after the
+// next tick's uploads.  Two streams, not three: a process gets a few hardware
+// queues (GPU_MAX_HW_QUEUES, 4 by default), and streams beyond them share
+// one, in whose order their work then runs.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -39,7 +45,8 @@ struct Slab {
     int W = 0, H = 0, n = 0;
     size_t bytes = 0;       // device bytes
     char *dev = nullptr;    // one allocation, carved below
-    float *ref = nullptr, *frame[2] = {nullptr, nullptr};
+    float *ref = nullptr;
+    std::vector<float *> frame;         // kDepth * K rings (uploaded frames)
     void *gx = nullptr, *gy = nullptr;
     size_t coef_bytes = 0;
     double *xy = nullptr;
@@ -57,6 +64,7 @@ struct Req {
     Slab *slab = nullptr;
     int next = 0;       // next frame to put into a tick
     int done = 0;       // frames whose results are on the host
+    bool gradients = false; // reference gradients formed (compute stream)
     bool finished = false;
     int rc = DIC_OK;    // a request that could not be served
 };
@@ -65,6 +73,7 @@ struct Tick {
     int slot = 0;
     std::vector<std::pair<Req *, int>> frames;
     hipEvent_t copied = nullptr;
+    bool downloads_queued = false;
 };
 
 } // namespace
@@ -80,9 +89,9 @@ struct dic_server {
     int copy_threads = 1; // threads of the result copy-out
     const int *table = nullptr;
     int nfull = 0;
-    hipStream_t prep = nullptr, compute = nullptr, copy = nullptr;
+    hipStream_t prep = nullptr, compute = nullptr;
     hipEvent_t prep_done[kDepth] = {}, solved[kDepth] = {};
-    double *tmp_a = nullptr, *tmp_b = nullptr; // prefilter temporaries (prep stream)
+    double *tmp_a = nullptr, *tmp_b = nullptr; // prefilter temporaries (compute stream)
     size_t tmp_bytes = 0;
     BatchPair *h_pairs[kDepth] = {}, *d_pairs[kDepth] = {};
     int *h_off[kDepth] = {}, *d_off[kDepth] = {};
@@ -114,9 +123,10 @@ size_t slab_device_bytes(const dic_server *s, int W, int H, int n)
     const int b = s->border;
     const size_t px = (size_t)W * H, cpx = (size_t)(W + 2 * b) * (H + 2 * b);
     const size_t e = elem_bytes(s->cfg.precision), R = (size_t)s->ring;
-    return 3 * align256(px * sizeof(float)) + 2 * align256(px * e) + align256(16 * (size_t)n) +
+    return align256(px * sizeof(float)) + 2 * align256(px * e) + align256(16 * (size_t)n) +
            align256(sizeof(dic_initial) * (size_t)n) +
-           R * (align256(cpx * e) + align256(sizeof(dic_initial) * (size_t)n) + align256(sizeof(dic_result) * (size_t)n));
+           R * (align256(px * sizeof(float)) + align256(cpx * e) + align256(sizeof(dic_initial) * (size_t)n) +
+                align256(sizeof(dic_result) * (size_t)n));
 }
 
 int fail(dic_server *s, int code, const std::string &msg)
@@ -175,18 +185,18 @@ Slab *take_slab(dic_server *s, int W, int H, int n)
         return q;
     };
     sl->ref = (float *)carve(px * sizeof(float));
-    sl->frame[0] = (float *)carve(px * sizeof(float));
-    sl->frame[1] = (float *)carve(px * sizeof(float));
     sl->gx = carve(px * e);
     sl->gy = carve(px * e);
     sl->coef_bytes = cpx * e;
     sl->xy = (double *)carve(16 * (size_t)n);
     sl->init0 = (dic_initial *)carve(sizeof(dic_initial) * (size_t)n);
+    sl->frame.assign(s->ring, nullptr);
     sl->coef.assign(s->ring, nullptr);
     sl->init.assign(s->ring, nullptr);
     sl->out.assign(s->ring, nullptr);
     sl->pinned.assign(s->ring, nullptr);
     for (int k = 0; k < s->ring; ++k) {
+        sl->frame[k] = (float *)carve(px * sizeof(float));
         sl->coef[k] = carve(cpx * e);
         sl->init[k] = (dic_initial *)carve(sizeof(dic_initial) * (size_t)n);
         sl->out[k] = (dic_result *)carve(sizeof(dic_result) * (size_t)n);
@@ -250,18 +260,15 @@ void admit(dic_server *s, std::vector<Req *> &admitted)
     }
 }
 
-// Uploads of an admitted request (prep stream): reference, gradients, POIs,
-// frame-0 guesses.
+// Uploads of an admitted request (prep stream): reference, POIs, frame-0
+// guesses.  Its reference gradients are formed on the compute stream before
+// its first solve.
 int setup_request(dic_server *s, Req *q)
 {
     Slab *sl = q->slab;
     const size_t px = (size_t)q->r.width * q->r.height;
     int rc = hipcheck(s, hipMemcpyAsync(sl->ref, q->r.reference, px * sizeof(float), hipMemcpyHostToDevice, s->prep),
                       "reference upload");
-    if (rc == DIC_OK)
-        rc = hipcheck(s, launch_gradient_central(sl->ref, q->r.width, q->r.height, s->cfg.precision, sl->gx, sl->gy,
-                                                 s->prep),
-                      "reference gradients");
     if (rc == DIC_OK && q->r.n > 0) {
         rc = hipcheck(s, hipMemcpyAsync(sl->xy, q->r.poi_xy, 16 * (size_t)q->r.n, hipMemcpyHostToDevice, s->prep),
                       "POI upload");
@@ -304,24 +311,37 @@ int build_tick(dic_server *s, Tick &t, int slot)
             pois += q->r.n;
         }
     if (t.frames.empty()) return DIC_OK;
-    // prep: frame uploads and prefilters into the frame's ring slot
+    // prep: frame uploads into the frames' ring slots (the slot's previous
+    // frame was prefiltered by a tick already finalized)
     for (auto &fr : t.frames) {
         Req *q = fr.first;
-        Slab *sl = q->slab;
-        const int f = fr.second, k = f % s->ring;
+        const int f = fr.second;
         const size_t px = (size_t)q->r.width * q->r.height;
-        int rc = hipcheck(s, hipMemcpyAsync(sl->frame[f & 1], q->r.frames[f], px * sizeof(float),
+        int rc = hipcheck(s, hipMemcpyAsync(q->slab->frame[f % s->ring], q->r.frames[f], px * sizeof(float),
                                             hipMemcpyHostToDevice, s->prep),
                           "frame upload");
-        if (rc == DIC_OK)
-            rc = hipcheck(s, launch_prefilter(sl->frame[f & 1], q->r.width, q->r.height, s->cfg.degree, s->border,
-                                              1, s->cfg.precision, sl->coef[k], s->tmp_a, s->tmp_b, s->prep),
-                          "frame prefilter");
         if (rc != DIC_OK) return rc;
     }
     int rc = hipcheck(s, hipEventRecord(s->prep_done[slot], s->prep), "event");
     if (rc == DIC_OK) rc = hipcheck(s, hipStreamWaitEvent(s->compute, s->prep_done[slot], 0), "event");
     if (rc != DIC_OK) return rc;
+    // compute: reference gradients of requests new to this tick, prefilters
+    for (auto &fr : t.frames) {
+        Req *q = fr.first;
+        Slab *sl = q->slab;
+        const int k = fr.second % s->ring;
+        if (!q->gradients) {
+            rc = hipcheck(s, launch_gradient_central(sl->ref, q->r.width, q->r.height, s->cfg.precision, sl->gx,
+                                                     sl->gy, s->compute),
+                          "reference gradients");
+            if (rc != DIC_OK) return rc;
+            q->gradients = true;
+        }
+        rc = hipcheck(s, launch_prefilter(sl->frame[k], q->r.width, q->r.height, s->cfg.degree, s->border, 1,
+                                          s->cfg.precision, sl->coef[k], s->tmp_a, s->tmp_b, s->compute),
+                      "frame prefilter");
+        if (rc != DIC_OK) return rc;
+    }
     // compute: guesses from frame f - K, then one launch over all frames
     BatchPair *hp = s->h_pairs[slot];
     int *ho = s->h_off[slot];
@@ -394,16 +414,28 @@ int build_tick(dic_server *s, Tick &t, int slot)
     L.fgn = s->cfg.optimizer == DIC_OPTIMIZER_FGN ? 1 : 0;
     if (rc == DIC_OK) rc = hipcheck(s, launch_range(L, s->compute), "tick launch");
     if (rc == DIC_OK) rc = hipcheck(s, hipEventRecord(s->solved[slot], s->compute), "event");
-    if (rc == DIC_OK) rc = hipcheck(s, hipStreamWaitEvent(s->copy, s->solved[slot], 0), "event");
+    t.downloads_queued = false;
+    return rc;
+}
+
+// The tick's result downloads (prep stream, after its solve).  Queued after
+// the next tick's uploads: a download waits for its solve, and the prep
+// stream runs in order, so a download queued first would hold the next
+// tick's uploads (and so its solve) until this tick's solve has ended.
+int queue_downloads(dic_server *s, Tick &t)
+{
+    if (t.downloads_queued) return DIC_OK;
+    t.downloads_queued = true;
+    int rc = hipcheck(s, hipStreamWaitEvent(s->prep, s->solved[t.slot], 0), "event");
     for (auto &fr : t.frames) {
         if (rc != DIC_OK) break;
         Slab *sl = fr.first->slab;
         const int f = fr.second;
         rc = hipcheck(s, hipMemcpyAsync(sl->pinned[f % s->ring], sl->out[f % s->ring],
-                                        sizeof(dic_result) * fr.first->r.n, hipMemcpyDeviceToHost, s->copy),
+                                        sizeof(dic_result) * fr.first->r.n, hipMemcpyDeviceToHost, s->prep),
                       "result download");
     }
-    if (rc == DIC_OK) rc = hipcheck(s, hipEventRecord(t.copied, s->copy), "event");
+    if (rc == DIC_OK) rc = hipcheck(s, hipEventRecord(t.copied, s->prep), "event");
     return rc;
 }
 
@@ -448,6 +480,7 @@ void copy_results(dic_server *s, const Tick &t)
 
 void finalize_tick(dic_server *s, Tick &t)
 {
+    if (queue_downloads(s, t) != DIC_OK) return;
     if (hipcheck(s, hipEventSynchronize(t.copied), "tick completion") != DIC_OK) return;
     copy_results(s, t);
     const double now = s->now();
@@ -505,7 +538,11 @@ void worker_main(dic_server *s)
             const int slot = next_slot;
             next_slot = (next_slot + 1) % kDepth;
             if (build_tick(s, ticks[slot], slot) != DIC_OK) break;
-            if (!ticks[slot].frames.empty()) inflight.push_back(slot);
+            if (!ticks[slot].frames.empty()) {
+                // the previous tick's downloads now follow this tick's uploads
+                if (!inflight.empty() && queue_downloads(s, ticks[inflight.back()]) != DIC_OK) break;
+                inflight.push_back(slot);
+            }
             continue;
         }
         if (!inflight.empty()) {
@@ -516,7 +553,10 @@ void worker_main(dic_server *s)
             continue;
         }
     }
-    for (int slot : inflight) (void)hipEventSynchronize(ticks[slot].copied);
+    for (int slot : inflight) {
+        if (!ticks[slot].downloads_queued) continue; // never queued: nothing recorded on its event
+        (void)hipEventSynchronize(ticks[slot].copied);
+    }
     for (auto &t : ticks) (void)hipEventDestroy(t.copied);
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->failed != DIC_OK) { // fail every unfinished request
@@ -531,7 +571,6 @@ void destroy_resources(dic_server *s)
 {
     if (s->prep) (void)hipStreamSynchronize(s->prep);
     if (s->compute) (void)hipStreamSynchronize(s->compute);
-    if (s->copy) (void)hipStreamSynchronize(s->copy);
     for (Slab *sl : s->slabs) free_slab(s, sl);
     s->slabs.clear();
     s->pool.clear();
@@ -545,7 +584,7 @@ void destroy_resources(dic_server *s)
     }
     if (s->tmp_a) ctx_free(s->ctx, s->tmp_a, s->tmp_bytes);
     if (s->tmp_b) ctx_free(s->ctx, s->tmp_b, s->tmp_bytes);
-    for (hipStream_t st : {s->prep, s->compute, s->copy})
+    for (hipStream_t st : {s->prep, s->compute})
         if (st) (void)hipStreamDestroy(st);
 }
 
@@ -611,7 +650,6 @@ int dic_server_create(dic_ctx *ctx, const dic_server_config *cfg, dic_server **o
     hc(hipSetDevice(ctx->device), "set device");
     hc(hipStreamCreateWithFlags(&s->prep, hipStreamNonBlocking), "stream");
     hc(hipStreamCreateWithFlags(&s->compute, hipStreamNonBlocking), "stream");
-    hc(hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking), "stream");
     s->tmp_bytes = sizeof(double) * (size_t)(c.max_width + 2 * b) * (c.max_height + 2 * b);
     hc(ctx_malloc(ctx, (void **)&s->tmp_a, s->tmp_bytes), "prefilter temporaries");
     hc(ctx_malloc(ctx, (void **)&s->tmp_b, s->tmp_bytes), "prefilter temporaries");
