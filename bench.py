@@ -136,7 +136,7 @@ def batched_leg(pkg, ctx, synth, reps=3):
     t_batch = _median_ms(ctx, b.run, reps)
     mx = [it["out"].download(pkg.RESULT_DTYPE) for it in st["mixed"][0]]
     conv = conv64 = 0
-    dmax = 0.0
+    dmax = dmax_both = 0.0
     hit = {"fp64": 0, "mixed": 0}
     split = split_ran_out = 0
     for f, m in zip(fp, mx):
@@ -146,6 +146,8 @@ def batched_leg(pkg, ctx, synth, reps=3):
         conv += int((okm & ok64 & (d <= 1e-3)).sum())
         conv64 += int(ok64.sum())
         dmax = max(dmax, float(np.max(d[ok64], initial=0.0)))
+        both = ok64 & okm & (f["iterations"] < 30) & (m["iterations"] < 30)
+        dmax_both = max(dmax_both, float(np.max(d[both], initial=0.0)))
         hit["fp64"] += int((ok64 & (f["iterations"] == 30)).sum())
         hit["mixed"] += int((okm & (m["iterations"] == 30)).sum())
         sp = ok64 & ~(okm & (d <= 1e-3))
@@ -163,6 +165,7 @@ def batched_leg(pkg, ctx, synth, reps=3):
            "value": round(conv / (t_batch * 1e-3), 1), "fp64_value": round(conv64 / (t_fp64 * 1e-3), 1),
            "vs_fp64_drop_in": round((conv / t_batch) / (conv64 / t_fp64), 3),
            "mixed_vs_fp64_max_px": dmax,
+           "mixed_vs_fp64_max_px_both_before_limit": dmax_both,
            "hit_max_iterations": hit,
            "fp64_converged_not_mixed": {"pois": split, "of_which_at_iteration_limit": split_ran_out}}
     st.clear()
