@@ -90,6 +90,66 @@ __global__ void pad_kernel(const float *__restrict__ img, int W, int H, int b, i
     out[(size_t)y * PW + x] = (OUT)img[(size_t)sy * W + sx];
 }
 
+// One recursion of the FP64 pass: y1 = v + z0*y1, y2 = y1 + z1*y2 (z1 = 0 for
+// a single pole), returning the last stage.
+struct PoleChain {
+    double y1 = 0.0, y2 = 0.0;
+    __device__ __forceinline__ double step(double v, const PoleArgs &p)
+    {
+        y1 = v + p.z0 * y1;
+        y2 = y1 + p.z1 * y2;
+        return (p.count == 2) ? y2 : y1;
+    }
+};
+
+// samples per block of loads issued ahead of the serial chain
+constexpr int kBlock = 16;
+
+// Runs one recursion over n_warm + n_out consecutive samples of a period-L
+// sequence, starting at periodic index kk and stepping forward (FWD) or
+// backward; the first n_warm steps only warm the chain, the rest store their
+// result through st(index, value).  Whole blocks of kBlock samples issue
+// every load before the block's chain (the same operations in the same order
+// as one sample at a time); the remainder goes one sample at a time.
+template <bool FWD, typename Load, typename Store>
+__device__ __forceinline__ void sweep(int kk, int L, int n_warm, int n_out, const PoleArgs &p, Load ld, Store st)
+{
+    PoleChain c;
+    auto adv = [L](int q) { return FWD ? wrap_next(q, L) : wrap_prev(q, L); };
+    int left = n_warm;
+    for (; left >= kBlock; left -= kBlock) {
+        double v[kBlock];
+#pragma unroll
+        for (int i = 0; i < kBlock; ++i) {
+            v[i] = ld(kk);
+            kk = adv(kk);
+        }
+#pragma unroll
+        for (int i = 0; i < kBlock; ++i) c.step(v[i], p);
+    }
+    for (; left > 0; --left) {
+        c.step(ld(kk), p);
+        kk = adv(kk);
+    }
+    left = n_out;
+    for (; left >= kBlock; left -= kBlock) {
+        double v[kBlock];
+        int at[kBlock];
+#pragma unroll
+        for (int i = 0; i < kBlock; ++i) {
+            v[i] = ld(kk);
+            at[i] = kk;
+            kk = adv(kk);
+        }
+#pragma unroll
+        for (int i = 0; i < kBlock; ++i) st(at[i], c.step(v[i], p));
+    }
+    for (; left > 0; --left) {
+        st(kk, c.step(ld(kk), p));
+        kk = adv(kk);
+    }
+}
+
 // One workgroup per padded row; the symmetric-padded line is built in LDS,
 // filtered by both recursions, and written as FP64.
 __global__ void prefilter_rows_kernel(const float *__restrict__ img, int W, int H, int b, PoleArgs poles,
@@ -108,29 +168,21 @@ __global__ void prefilter_rows_kernel(const float *__restrict__ img, int W, int 
     const int c0 = threadIdx.x * chunk;
     const int c1 = min(c0 + chunk, PW);
     const int T = poles.warmup;
-    if (c0 < c1) {
-        double y1 = 0.0, y2 = 0.0;
-        for (int k = c0 - T, kk = wrap(c0 - T, PW); k < c1; ++k, kk = wrap_next(kk, PW)) {
-            y1 = line[kk] + poles.z0 * y1;
-            y2 = y1 + poles.z1 * y2;
-            if (k >= c0) buf[kk] = (poles.count == 2) ? y2 : y1;
-        }
-    }
+    if (c0 < c1)
+        sweep<true>(wrap(c0 - T, PW), PW, T, c1 - c0, poles, [&](int q) { return line[q]; },
+                    [&](int q, double v) { buf[q] = v; });
     __syncthreads();
-    if (c0 < c1) {
-        double w1 = 0.0, w2 = 0.0;
-        for (int k = c1 - 1 + T, kk = wrap(c1 - 1 + T, PW); k >= c0; --k, kk = wrap_prev(kk, PW)) {
-            w1 = buf[kk] + poles.z0 * w1;
-            w2 = w1 + poles.z1 * w2;
-            if (k < c1) line[kk] = poles.gain * ((poles.count == 2) ? w2 : w1);
-        }
-    }
+    if (c0 < c1)
+        sweep<false>(wrap(c1 - 1 + T, PW), PW, T, c1 - c0, poles, [&](int q) { return buf[q]; },
+                     [&](int q, double v) { line[q] = poles.gain * v; });
     __syncthreads();
     for (int x = threadIdx.x; x < PW; x += blockDim.x) out[(size_t)y * PW + x] = line[x];
 }
 
 constexpr int kColChunk = 64;
 
+// one thread per (column, kColChunk-row chunk), each starting its chain
+// poles.warmup rows early
 __global__ void prefilter_cols_causal_kernel(const double *__restrict__ in, int PW, int PH, PoleArgs poles,
                                              double *__restrict__ out)
 {
@@ -138,12 +190,8 @@ __global__ void prefilter_cols_causal_kernel(const double *__restrict__ in, int 
     if (x >= PW) return;
     const int r0 = blockIdx.y * kColChunk;
     const int r1 = min(r0 + kColChunk, PH);
-    double y1 = 0.0, y2 = 0.0;
-    for (int k = r0 - poles.warmup, kk = wrap(r0 - poles.warmup, PH); k < r1; ++k, kk = wrap_next(kk, PH)) {
-        y1 = in[(size_t)kk * PW + x] + poles.z0 * y1;
-        y2 = y1 + poles.z1 * y2;
-        if (k >= r0) out[(size_t)kk * PW + x] = (poles.count == 2) ? y2 : y1;
-    }
+    sweep<true>(wrap(r0 - poles.warmup, PH), PH, poles.warmup, r1 - r0, poles,
+                [&](int q) { return in[(size_t)q * PW + x]; }, [&](int q, double v) { out[(size_t)q * PW + x] = v; });
 }
 
 __global__ void prefilter_cols_anticausal_kernel(const double *__restrict__ in, int PW, int PH, PoleArgs poles,
@@ -153,13 +201,9 @@ __global__ void prefilter_cols_anticausal_kernel(const double *__restrict__ in, 
     if (x >= PW) return;
     const int r0 = blockIdx.y * kColChunk;
     const int r1 = min(r0 + kColChunk, PH);
-    double w1 = 0.0, w2 = 0.0;
-    for (int k = r1 - 1 + poles.warmup, kk = wrap(r1 - 1 + poles.warmup, PH); k >= r0;
-         --k, kk = wrap_prev(kk, PH)) {
-        w1 = in[(size_t)kk * PW + x] + poles.z0 * w1;
-        w2 = w1 + poles.z1 * w2;
-        if (k < r1) out[(size_t)kk * PW + x] = poles.gain * ((poles.count == 2) ? w2 : w1);
-    }
+    sweep<false>(wrap(r1 - 1 + poles.warmup, PH), PH, poles.warmup, r1 - r0, poles,
+                 [&](int q) { return in[(size_t)q * PW + x]; },
+                 [&](int q, double v) { out[(size_t)q * PW + x] = poles.gain * v; });
 }
 
 // ---- FP32 storage (the mixed path): two passes with an FP32 intermediate ----
