@@ -253,3 +253,36 @@ def test_gpu_heap_order_walk_second_order_mixed_follows_fp64(pkg, ctx, orc, ster
     _report(f"second-order heap walk mixed vs fp64: {int(v.sum())} valid, max |du| {d.max():.2e}")
     assert v.sum() > 1000 and np.array_equal(out["mixed"]["valid"], out["fp64"]["valid"])
     assert d.max() <= 1e-3
+
+
+def test_gpu_propagation_at_the_largest_fp64_radius(pkg, ctx, orc, synth):
+    """FP64 ZNSSD IC-GN at radius 79, the largest subset whose per-sample
+    values fit one workgroup's LDS beside the one-wave kernel's block.  The
+    walk's launches are small, but the cooperative kernel's per-wave slots do
+    not fit beside those values, so they keep one wave per POI; the walk
+    equals the oracle's (valid set, count, u, v within 1e-5 px)."""
+    f = synth.affine(0.6, -0.4, 2e-3, 1e-3, -1e-3, 1.5e-3, 99.5, 99.5)
+    ref = synth.quantize_unit(synth.render(200, 200, 620, 17))
+    dfm = synth.quantize_unit(synth.render(200, 200, 620, 17, f))
+    roi = np.ones((200, 200), np.uint8)
+    u, v, _ = f.displacement(np.array([100.0]), np.array([100.0]))
+    su, sv = float(np.round(u[0])), float(np.round(v[0]))
+    pre = orc.Precomputed(dfm, degree=5)
+    gx, gy = orc.reference_gradient_cache(ref)
+    seed_row = np.array([[100.0, 100.0, su, sv, 0.0, 0.0, 0.0, 0.0, 0.0]])
+    cpu, ncpu = orc.propagate(ref, pre, gx, gy, roi, seed_row, radius=79, max_iterations=30, threshold=1e-3,
+                              spacing=3, max_znssd=2.0)
+    cfg = pkg.config_from_dict({
+        "subset": {"radius": 79}, "shape_function": {"order": 1},
+        "optimization": {"method": "icgn", "max_iterations": 30, "convergence_threshold": 1e-3},
+        "correlation": {"criterion": "znssd"}, "interpolation": {"degree": 5},
+        "reliability_propagation": {"spacing": 3, "max_znssd": 2.0}})
+    solver = pkg.ICGNSolver(cfg, precision="fp64", context=ctx)
+    seed = _seed(pkg, 100.0, 100.0, su, sv, (0.0, 0.0, 0.0, 0.0), 0.0)
+    _, gpu, ngpu = pkg.propagate(ref, dfm, roi, seed, solver=solver)
+    valid = cpu["valid"] == 1
+    d = np.abs(gpu["p"][valid][:, :2] - cpu["p"][valid][:, :2]).max()
+    _report(f"radius 79 walk: gpu {ngpu} / oracle {ncpu} points, max |du| {d:.2e}")
+    assert ncpu > 50 and ngpu == ncpu
+    assert np.array_equal(gpu["valid"], cpu["valid"])
+    assert d <= 1e-5

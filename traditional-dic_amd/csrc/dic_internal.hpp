@@ -83,6 +83,9 @@ int noalloc_setup(dic_ctx *ctx);
 // check_solve_lds turns a subset too large for the device into
 // DIC_ERR_UNSUPPORTED before any launch
 size_t solve_static_lds();
+// the same for the cooperative kernel (one POI per workgroup of several
+// waves, with their per-wave slots)
+size_t coop_static_lds();
 size_t solve_dynamic_lds(bool fgn, int precision, int objective, int nfull);
 int check_solve_lds(dic_ctx *ctx, bool fgn, int precision, int objective, int radius, int nfull);
 

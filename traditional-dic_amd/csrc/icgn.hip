@@ -1638,7 +1638,10 @@ hipError_t dispatch_degree(const IcgnArgs &args, int degree, int precision, bool
 
 } // namespace
 
-size_t solve_static_lds() { return sizeof(IcgnShared); }
+// static LDS of the solve kernels: their per-POI blocks plus 256 bytes for the
+// compiler's alignment of the __shared__ objects
+size_t solve_static_lds() { return sizeof(IcgnShared) + 256; }
+size_t coop_static_lds() { return sizeof(IcgnShared) + sizeof(CoopSlots<kCoopWaves>) + 256; }
 
 double tile_fit_share(const double *xy, const dic_initial *init, int n, int radius, int degree)
 {

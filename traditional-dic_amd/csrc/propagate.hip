@@ -71,6 +71,18 @@ struct DeviceScratch {
 // several waves: the walk's launches are latency-bound (a few POIs each).
 constexpr int kCoopMaxPois = 256;
 
+// Whether the cooperative kernel's LDS (its larger static block beside the
+// per-sample buffer) fits the device for this subset; where it does not
+// (FP64 ZNSSD at the largest admitted radii), launches keep one wave per POI.
+bool coop_fits(dic_ctx *ctx, const dic_icgn_params *params)
+{
+    const int *table = nullptr;
+    int count = 0;
+    if (get_subset_table(ctx, params->subset_radius, &table, &count) != DIC_OK) return false;
+    return ctx->max_lds == 0 ||
+           coop_static_lds() + solve_dynamic_lds(false, params->precision, params->objective, count) <= ctx->max_lds;
+}
+
 // one launch over n POIs (host arrays in, host results out)
 int solve_host(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
                const dic_coeffs *def, DeviceScratch &d, const std::vector<double> &xy,
@@ -89,7 +101,7 @@ int solve_host(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref
     const dic_initial *dini = reinterpret_cast<const dic_initial *>(d.in + xy_bytes);
     dic_result *pres = reinterpret_cast<dic_result *>(d.pin + DeviceScratch::kInBytes * d.pin_cap);
     rc = check_hip(ctx, hipMemcpyAsync(d.in, d.pin, in_bytes, hipMemcpyHostToDevice, s), "POI upload");
-    const int shape = params->optimizer == DIC_OPTIMIZER_ICGN && n <= kCoopMaxPois
+    const int shape = params->optimizer == DIC_OPTIMIZER_ICGN && n <= kCoopMaxPois && coop_fits(ctx, params)
                           ? kShapeCoop
                           : host_tile_hint(params, def, xy.data(), ini.data(), n);
     if (rc == DIC_OK) rc = run_solve(ctx, params, ref, grad, def, params->masked ? d.roi : nullptr, dxy, dini, n,
