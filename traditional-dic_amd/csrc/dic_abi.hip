@@ -195,6 +195,7 @@ void dic_ctx_destroy(dic_ctx *ctx)
     for (auto &kv : ctx->subset_tables) ctx_free(ctx, kv.second.first, sizeof(int) * (size_t)kv.second.second);
     for (auto &kv : ctx->buffers) ctx_free(ctx, kv.first, kv.second);
     if (ctx->staging) ctx_free(ctx, ctx->staging, ctx->staging_bytes);
+    destroy_stager(ctx);
     (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
@@ -318,8 +319,7 @@ int dic_image_upload(dic_ctx *ctx, const float *host, int width, int height, dic
         delete img;
         return rc;
     }
-    rc = check_hip(ctx, hipMemcpyAsync(img->data, host, img->bytes, hipMemcpyHostToDevice, ctx->stream),
-                   "image upload");
+    rc = check_hip(ctx, upload_host(ctx, img->data, host, img->bytes, ctx->stream), "image upload");
     if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "image upload sync");
     if (rc != DIC_OK) {
         ctx_free(ctx, img->data, img->bytes);
@@ -707,13 +707,12 @@ int dic_icgn_solve_batch(dic_ctx *ctx, const dic_icgn_params *params, const dic_
     dic_result *dout = (dic_result *)(st + align(bxy) + align(bin));
     uint8_t *droi = broi ? (uint8_t *)(st + align(bxy) + align(bin) + align(bout)) : nullptr;
     hipStream_t s = ctx->stream;
-    rc = check_hip(ctx, hipMemcpyAsync(dxy, poi_xy, bxy, hipMemcpyHostToDevice, s), "POI upload");
-    if (rc == DIC_OK) rc = check_hip(ctx, hipMemcpyAsync(din, init, bin, hipMemcpyHostToDevice, s), "init upload");
-    if (rc == DIC_OK && droi)
-        rc = check_hip(ctx, hipMemcpyAsync(droi, roi_host, broi, hipMemcpyHostToDevice, s), "roi upload");
+    rc = check_hip(ctx, upload_host(ctx, dxy, poi_xy, bxy, s), "POI upload");
+    if (rc == DIC_OK) rc = check_hip(ctx, upload_host(ctx, din, init, bin, s), "init upload");
+    if (rc == DIC_OK && droi) rc = check_hip(ctx, upload_host(ctx, droi, roi_host, broi, s), "roi upload");
     if (rc == DIC_OK) rc = run_solve(ctx, params, ref, grad, def, droi, dxy, din, n, dout, s,
                                      host_tile_hint(params, def, poi_xy, init, n));
-    if (rc == DIC_OK) rc = check_hip(ctx, hipMemcpyAsync(out, dout, bout, hipMemcpyDeviceToHost, s), "result download");
+    if (rc == DIC_OK) rc = check_hip(ctx, download_host(ctx, out, dout, bout, s), "result download");
     if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(s), "solve sync");
     return rc;
 }

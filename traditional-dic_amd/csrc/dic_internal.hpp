@@ -11,6 +11,10 @@
 
 #include "dic_hip.h"
 
+namespace dic {
+struct HostStager;
+}
+
 struct dic_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
@@ -30,6 +34,9 @@ struct dic_ctx {
     size_t max_lds = 0;
     // device address of the no-allocation entry's table array on this device
     int *noalloc_table = nullptr;
+    // pinned double buffer and copy threads of large host copies (hostcopy.hip)
+    dic::HostStager *stager = nullptr;
+    bool stager_failed = false;
 };
 
 struct dic_image {
@@ -62,6 +69,13 @@ hipError_t ctx_malloc(dic_ctx *ctx, void **ptr, size_t bytes);
 void ctx_free(dic_ctx *ctx, void *ptr, size_t bytes);
 int set_error(dic_ctx *ctx, int code, const std::string &msg);
 int check_hip(dic_ctx *ctx, hipError_t e, const char *what);
+
+// hostcopy.hip: copies between pageable host memory and the device.  An
+// upload is ordered on s (the host buffer may be reused on return); a
+// download returns with the data on the host.
+hipError_t upload_host(dic_ctx *ctx, void *device, const void *host, size_t bytes, hipStream_t s);
+hipError_t download_host(dic_ctx *ctx, void *host, const void *device, size_t bytes, hipStream_t s);
+void destroy_stager(dic_ctx *ctx);
 
 // dic_abi.hip: argument checks and one solve launch over device arrays
 int validate_solve(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref, const dic_grad *grad,
