@@ -88,7 +88,10 @@ def test_context_wrappers_refuse_an_roi_of_another_shape(pkg):
 
 
 def test_solver_keeps_no_state_between_calls(pkg):
-    """No implicit pair cache: the solver object holds only its config,
-    precision and context (ids of freed host arrays are reused by CPython)."""
+    """No implicit pair cache (ids of freed host arrays are reused by
+    CPython): besides its config, precision and context the solver holds only
+    the device buffers of host-image calls, keyed by image size and the
+    precompute configuration and refilled on every call."""
     s = _solver(pkg)
-    assert set(vars(s)) == {"config", "precision", "gradient_mode", "ctx"}
+    assert set(vars(s)) == {"config", "precision", "gradient_mode", "ctx", "_ws", "_ws_lock"}
+    assert s._ws is None

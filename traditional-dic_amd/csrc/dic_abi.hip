@@ -252,8 +252,7 @@ void dic_buffer_free(dic_ctx *ctx, void *device_ptr)
 int dic_buffer_upload(dic_ctx *ctx, void *device_dst, const void *host_src, size_t bytes)
 {
     if (!ctx || (bytes && (!device_dst || !host_src))) return DIC_ERR_INVALID_ARGUMENT;
-    int rc = check_hip(ctx, hipMemcpyAsync(device_dst, host_src, bytes, hipMemcpyHostToDevice, ctx->stream),
-                       "buffer upload");
+    int rc = check_hip(ctx, upload_host(ctx, device_dst, host_src, bytes, ctx->stream), "buffer upload");
     if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "buffer upload sync");
     return rc;
 }
@@ -261,8 +260,7 @@ int dic_buffer_upload(dic_ctx *ctx, void *device_dst, const void *host_src, size
 int dic_buffer_download(dic_ctx *ctx, void *host_dst, const void *device_src, size_t bytes)
 {
     if (!ctx || (bytes && (!device_src || !host_dst))) return DIC_ERR_INVALID_ARGUMENT;
-    int rc = check_hip(ctx, hipMemcpyAsync(host_dst, device_src, bytes, hipMemcpyDeviceToHost, ctx->stream),
-                       "buffer download");
+    int rc = check_hip(ctx, download_host(ctx, host_dst, device_src, bytes, ctx->stream), "buffer download");
     if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "buffer download sync");
     return rc;
 }

@@ -13,6 +13,7 @@ The numerics run in libdic_hip.so; nothing here computes a correlation.
 from __future__ import annotations
 
 import dataclasses
+import threading
 
 from dataclasses import dataclass, field
 
@@ -253,6 +254,41 @@ class PreparedPair:
             h.free()
 
 
+class _HostPairWorkspace:
+    """Device buffers for a solver's calls on host images of one size and one
+    precompute configuration.  Every call uploads both images into them and
+    recomputes the central-difference gradient cache and the coefficients, so
+    a run of same-size calls allocates once; nothing computed is carried from
+    one call to the next."""
+
+    def __init__(self, ctx: _lib.Context, key: tuple):
+        self.ctx, self.key = ctx, key
+        w, h = key[0], key[1]
+        self.width, self.height = w, h
+        self.bufs = [ctx.alloc(4 * w * h), ctx.alloc(4 * w * h)]
+        self.ref = ctx.wrap_image(self.bufs[0].ptr, w, h)
+        self.deformed = ctx.wrap_image(self.bufs[1].ptr, w, h)
+        self.grad = self.coeffs = None
+
+    def fill(self, reference, deformed, cfg: SubsetConfig, precision: int):
+        self.bufs[0].upload(np.ascontiguousarray(reference, np.float32))
+        self.bufs[1].upload(np.ascontiguousarray(deformed, np.float32))
+        ip = cfg.image_precompute
+        if self.grad is None:
+            self.grad = self.ctx.ref_gradient(self.ref, _lib.GRADIENT_CENTRAL, ip.degree, ip.border, precision)
+            self.coeffs = self.ctx.prefilter(self.deformed, ip.degree, ip.border, ip.use_exact_prefilter, precision)
+        else:
+            self.ctx.ref_gradient_into(self.ref, self.grad)
+            self.ctx.prefilter_into(self.deformed, self.coeffs)
+
+    def free(self):
+        for h in (self.coeffs, self.grad, self.deformed, self.ref):
+            if h is not None:
+                h.free()
+        for b in self.bufs:
+            b.free()
+
+
 class ICGNSolver:
     """IC-GN on the GPU behind the reference's ICGNSolver call pattern.
 
@@ -261,10 +297,13 @@ class ICGNSolver:
     gradient: "central" (pipeline convention, reliability_propagation.cpp:84-112)
     or "bspline" (standalone solve(), bspline.cpp:289-294).
 
-    Nothing is cached between calls: solve() / solve_batch() on host images
-    prepare the pair (gradient cache + B-spline coefficients) for that call
-    and free it afterwards, as the reference's solve() builds its
-    interpolators per call (icgn.cpp:363-382).  A caller that solves one pair
+    Nothing computed is cached between calls: solve() / solve_batch() on host
+    images upload both images and build the pair's gradient cache and B-spline
+    coefficients for that call, as the reference's solve() builds its
+    interpolators per call (icgn.cpp:363-382).  With central-difference
+    gradients the device buffers they go into are kept for the next call of
+    the same size and configuration (release() frees them); with B-spline
+    gradients the pair is prepared and freed per call.  A caller that solves one pair
     many times prepares it once with prepare() and hands the PreparedPair to
     solve_with_interpolators / solve_with_mask / solve_batch(pair=...), as the
     reference's callers hand over their interpolators
@@ -278,6 +317,25 @@ class ICGNSolver:
         self.precision = _lib.PRECISION_FP64 if precision == "fp64" else _lib.PRECISION_MIXED
         self.gradient_mode = _lib.GRADIENT_CENTRAL if gradient == "central" else _lib.GRADIENT_BSPLINE_HALF
         self.ctx = context if context is not None else _lib.Context(device)
+        self._ws = None
+        self._ws_lock = threading.Lock()
+
+    def release(self):
+        """Frees the device buffers kept for calls on host images."""
+        with self._ws_lock:
+            if self._ws is not None:
+                self._ws.free()
+                self._ws = None
+
+    def _workspace(self, w: int, h: int) -> _HostPairWorkspace:
+        ip = self.config.image_precompute
+        key = (w, h, ip.degree, ip.border, bool(ip.use_exact_prefilter), self.precision)
+        if self._ws is None or self._ws.key != key:
+            if self._ws is not None:
+                self._ws.free()
+                self._ws = None
+            self._ws = _HostPairWorkspace(self.ctx, key)
+        return self._ws
 
     def prepare(self, reference, deformed) -> PreparedPair:
         """A new device-resident pair (uploaded host arrays or borrowed
@@ -301,6 +359,13 @@ class ICGNSolver:
             if roi.shape != (rh, rw):
                 return invalid_results(xy, ini)  # icgn.cpp:648-652
             roi = np.ascontiguousarray(roi, np.uint8)
+        host_images = not isinstance(reference, _lib.DeviceImage) and not isinstance(deformed, _lib.DeviceImage)
+        if pair is None and host_images and self.gradient_mode == _lib.GRADIENT_CENTRAL:
+            with self._ws_lock:
+                ws = self._workspace(rw, rh)
+                ws.fill(reference, deformed, self.config, self.precision)
+                params = make_params(self.config, self.precision, roi is not None)
+                return self.ctx.solve(params, ws.ref, ws.grad, ws.coeffs, xy, ini, roi)
         owned = pair is None
         if owned:
             pair = self.prepare(reference, deformed)
