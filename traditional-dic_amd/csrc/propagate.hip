@@ -34,23 +34,21 @@ bool finite6(const dic_result &r)
     return std::isfinite(r.correlation);
 }
 
-// Device buffers of the walk's launches, and pinned host staging: a launch
-// packs its POIs and guesses into one pinned block and uploads it with one
-// copy, and its results come back into pinned memory (the walk's launches
-// are small, so copy latency counts).
+// The walk's launches are small, so copy latency counts: a launch packs its
+// POIs and guesses into one block of coherent pinned host memory, which the
+// kernels read in place (one POI record each), and the kernels write their
+// results into the same block; no copy is queued around the solve.  The ROI
+// stays in device memory.
 struct DeviceScratch {
     dic_ctx *ctx = nullptr;
-    unsigned char *in = nullptr; // per launch: n POI positions (16 B each), then n guesses
-    dic_result *out = nullptr;
     uint8_t *roi = nullptr;
-    size_t cap = 0, roi_bytes = 0;
-    unsigned char *pin = nullptr; // pinned: the packed inputs, then the results
-    size_t pin_cap = 0;           // POIs the pinned block holds
+    size_t roi_bytes = 0;
+    unsigned char *pin = nullptr;     // host view: n POI positions (16 B each), n guesses, then the results
+    unsigned char *pin_dev = nullptr; // the same block as the device sees it
+    size_t pin_cap = 0;               // POIs the block holds
     static constexpr size_t kInBytes = 16 + sizeof(dic_initial);
     ~DeviceScratch()
     {
-        ctx_free(ctx, in, kInBytes * cap);
-        ctx_free(ctx, out, sizeof(dic_result) * cap);
         ctx_free(ctx, roi, roi_bytes);
         if (pin) (void)hipHostFree(pin);
     }
@@ -59,9 +57,11 @@ struct DeviceScratch {
         if (n <= pin_cap) return hipSuccess;
         const size_t want = std::max(n, 2 * pin_cap + 256);
         if (pin) (void)hipHostFree(pin);
-        pin = nullptr;
+        pin = pin_dev = nullptr;
         pin_cap = 0;
-        hipError_t e = hipHostMalloc((void **)&pin, (kInBytes + sizeof(dic_result)) * want, hipHostMallocDefault);
+        hipError_t e = hipHostMalloc((void **)&pin, (kInBytes + sizeof(dic_result)) * want,
+                                     hipHostMallocCoherent | hipHostMallocMapped);
+        if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&pin_dev, pin, 0);
         if (e == hipSuccess) pin_cap = want;
         return e;
     }
@@ -94,21 +94,18 @@ int solve_host(dic_ctx *ctx, const dic_icgn_params *params, const dic_image *ref
     hipStream_t s = ctx->stream;
     int rc = check_hip(ctx, d.pinned((size_t)n), "pinned staging");
     if (rc != DIC_OK) return rc;
-    const size_t xy_bytes = 16 * (size_t)n, in_bytes = DeviceScratch::kInBytes * (size_t)n;
+    const size_t xy_bytes = 16 * (size_t)n;
     std::memcpy(d.pin, xy.data(), xy_bytes);
     std::memcpy(d.pin + xy_bytes, ini.data(), sizeof(dic_initial) * (size_t)n);
-    const double *dxy = reinterpret_cast<const double *>(d.in);
-    const dic_initial *dini = reinterpret_cast<const dic_initial *>(d.in + xy_bytes);
-    dic_result *pres = reinterpret_cast<dic_result *>(d.pin + DeviceScratch::kInBytes * d.pin_cap);
-    rc = check_hip(ctx, hipMemcpyAsync(d.in, d.pin, in_bytes, hipMemcpyHostToDevice, s), "POI upload");
+    const double *dxy = reinterpret_cast<const double *>(d.pin_dev);
+    const dic_initial *dini = reinterpret_cast<const dic_initial *>(d.pin_dev + xy_bytes);
+    const size_t res_off = DeviceScratch::kInBytes * d.pin_cap;
+    dic_result *pres = reinterpret_cast<dic_result *>(d.pin + res_off);
+    dic_result *dres = reinterpret_cast<dic_result *>(d.pin_dev + res_off);
     const int shape = params->optimizer == DIC_OPTIMIZER_ICGN && n <= kCoopMaxPois && coop_fits(ctx, params)
                           ? kShapeCoop
                           : host_tile_hint(params, def, xy.data(), ini.data(), n);
-    if (rc == DIC_OK) rc = run_solve(ctx, params, ref, grad, def, params->masked ? d.roi : nullptr, dxy, dini, n,
-                                     d.out, s, shape);
-    if (rc == DIC_OK)
-        rc = check_hip(ctx, hipMemcpyAsync(pres, d.out, sizeof(dic_result) * n, hipMemcpyDeviceToHost, s),
-                       "result download");
+    rc = run_solve(ctx, params, ref, grad, def, params->masked ? d.roi : nullptr, dxy, dini, n, dres, s, shape);
     if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(s), "propagation sync");
     if (rc == DIC_OK) std::memcpy(res.data(), pres, sizeof(dic_result) * (size_t)n);
     return rc;
@@ -281,10 +278,7 @@ extern "C" int dic_propagate(dic_ctx *ctx, const dic_icgn_params *params, const 
 
     DeviceScratch d;
     d.ctx = ctx;
-    d.cap = (size_t)std::max(total, 1);
-    rc = check_hip(ctx, ctx_malloc(ctx, (void **)&d.in, DeviceScratch::kInBytes * d.cap), "propagation buffers");
-    if (rc == DIC_OK) rc = check_hip(ctx, ctx_malloc(ctx, (void **)&d.out, sizeof(dic_result) * d.cap), "buffers");
-    if (rc == DIC_OK && params->masked) {
+    if (params->masked) {
         d.roi_bytes = (size_t)W * H;
         rc = check_hip(ctx, ctx_malloc(ctx, (void **)&d.roi, d.roi_bytes), "roi buffer");
         if (rc == DIC_OK)
