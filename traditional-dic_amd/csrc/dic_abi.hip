@@ -350,10 +350,7 @@ int dic_image_download(dic_ctx *ctx, const dic_image *img, float *host)
     if (!ctx) return DIC_ERR_INVALID_ARGUMENT;
     if (!img || !img->data || !host) return set_error(ctx, DIC_ERR_INVALID_ARGUMENT, "image and host buffer required");
     const size_t bytes = sizeof(float) * (size_t)img->width * img->height;
-    int rc = check_hip(ctx, hipMemcpyAsync(host, img->data, bytes, hipMemcpyDeviceToHost, ctx->stream),
-                       "image download");
-    if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "image download sync");
-    return rc;
+    return check_hip(ctx, download_host(ctx, host, img->data, bytes, ctx->stream), "image download");
 }
 
 void dic_image_free(dic_image *img)
@@ -442,15 +439,12 @@ int dic_coeffs_download(dic_ctx *ctx, const dic_coeffs *c, double *host)
     if (!ctx || !c || !host) return DIC_ERR_INVALID_ARGUMENT;
     const size_t n = (size_t)c->pitch * c->rows;
     if (c->precision == DIC_PRECISION_FP64) {
-        int rc = check_hip(ctx, hipMemcpyAsync(host, c->data, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream),
-                           "coefficient download");
-        if (rc != DIC_OK) return rc;
-        return check_hip(ctx, hipStreamSynchronize(ctx->stream), "coefficient download sync");
+        return check_hip(ctx, download_host(ctx, host, c->data, sizeof(double) * n, ctx->stream),
+                         "coefficient download");
     }
     std::vector<float> tmp(n);
-    int rc = check_hip(ctx, hipMemcpyAsync(tmp.data(), c->data, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream),
+    int rc = check_hip(ctx, download_host(ctx, tmp.data(), c->data, sizeof(float) * n, ctx->stream),
                        "coefficient download");
-    if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "coefficient download sync");
     if (rc != DIC_OK) return rc;
     for (size_t i = 0; i < n; ++i) host[i] = tmp[i];
     return DIC_OK;
@@ -527,22 +521,19 @@ int dic_grad_download(dic_ctx *ctx, const dic_grad *g, double *gx_host, double *
     if (!ctx || !g || !gx_host || !gy_host) return DIC_ERR_INVALID_ARGUMENT;
     const size_t n = (size_t)g->width * g->height;
     if (g->precision == DIC_PRECISION_FP64) {
-        int rc = check_hip(ctx, hipMemcpyAsync(gx_host, g->gx, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream),
+        int rc = check_hip(ctx, download_host(ctx, gx_host, g->gx, sizeof(double) * n, ctx->stream),
                            "gradient download");
         if (rc == DIC_OK)
-            rc = check_hip(ctx,
-                           hipMemcpyAsync(gy_host, g->gy, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream),
+            rc = check_hip(ctx, download_host(ctx, gy_host, g->gy, sizeof(double) * n, ctx->stream),
                            "gradient download");
-        if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "gradient download sync");
         return rc;
     }
     std::vector<float> tx(n), ty(n);
-    int rc = check_hip(ctx, hipMemcpyAsync(tx.data(), g->gx, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream),
+    int rc = check_hip(ctx, download_host(ctx, tx.data(), g->gx, sizeof(float) * n, ctx->stream),
                        "gradient download");
     if (rc == DIC_OK)
-        rc = check_hip(ctx, hipMemcpyAsync(ty.data(), g->gy, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream),
+        rc = check_hip(ctx, download_host(ctx, ty.data(), g->gy, sizeof(float) * n, ctx->stream),
                        "gradient download");
-    if (rc == DIC_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "gradient download sync");
     if (rc != DIC_OK) return rc;
     for (size_t i = 0; i < n; ++i) {
         gx_host[i] = tx[i];
