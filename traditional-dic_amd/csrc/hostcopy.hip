@@ -16,18 +16,15 @@
 #include "dic_internal.hpp"
 
 namespace dic {
+namespace {
 
-struct HostStager {
-    static constexpr size_t kChunk = (size_t)4 << 20;  // bytes per pinned slot
-    static constexpr size_t kPiece = (size_t)256 << 10; // bytes per thread task
-    unsigned char *pin[2] = {nullptr, nullptr};
-    hipEvent_t done[2] = {nullptr, nullptr}; // the last DMA that touched each slot
-    bool used[2] = {false, false};
-    std::mutex use; // one copy at a time per context
-
-    // the copy pool: the caller and the workers take kPiece tasks of the
-    // current job until none is left; the caller returns once every worker
-    // has left the job
+// The process's copy threads: a copy job is split in kPiece tasks that the
+// calling thread and the workers take until none is left; the caller returns
+// once every worker has left the job.  One job runs at a time (contexts on
+// several threads queue on `jobs`).
+struct CopyPool {
+    static constexpr size_t kPiece = (size_t)256 << 10;
+    std::mutex jobs;
     std::vector<std::thread> workers;
     std::mutex mu;
     std::condition_variable cv_job, cv_done;
@@ -39,6 +36,27 @@ struct HostStager {
     uint64_t job = 0;
     bool stop = false;
 
+    CopyPool()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const unsigned n = hw > 2 ? std::min(7u, hw / 2) : 0u; // workers beside the calling thread
+        for (unsigned i = 0; i < n; ++i) {
+            try {
+                workers.emplace_back([this] { worker(); });
+            } catch (...) {
+                break; // fewer workers
+            }
+        }
+    }
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv_job.notify_all();
+        for (auto &t : workers) t.join();
+    }
     void drain()
     {
         for (size_t o; (o = next.fetch_add(kPiece)) < bytes;) std::memcpy(dst + o, src + o, std::min(kPiece, bytes - o));
@@ -63,6 +81,7 @@ struct HostStager {
             std::memcpy(d, s, n);
             return;
         }
+        std::lock_guard<std::mutex> one(jobs);
         {
             std::lock_guard<std::mutex> lk(mu);
             dst = (unsigned char *)d;
@@ -77,6 +96,25 @@ struct HostStager {
         std::unique_lock<std::mutex> lk(mu);
         cv_done.wait(lk, [&] { return busy == 0; });
     }
+};
+
+CopyPool &copy_pool()
+{
+    static CopyPool pool;
+    return pool;
+}
+
+} // namespace
+
+// A context's pinned double buffer.
+struct HostStager {
+    static constexpr size_t kChunk = (size_t)4 << 20; // bytes per pinned slot
+    unsigned char *pin[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr}; // the last DMA that touched each slot
+    bool used[2] = {false, false};
+    std::mutex use; // one copy at a time per context
+
+    void copy(void *d, const void *s, size_t n) { copy_pool().copy(d, s, n); }
     hipError_t ready(int k) { return used[k] ? hipEventSynchronize(done[k]) : hipSuccess; }
     hipError_t mark(int k, hipStream_t s)
     {
@@ -85,12 +123,6 @@ struct HostStager {
     }
     ~HostStager()
     {
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            stop = true;
-        }
-        cv_job.notify_all();
-        for (auto &t : workers) t.join();
         for (int k = 0; k < 2; ++k) {
             if (done[k]) (void)hipEventSynchronize(done[k]);
             if (done[k]) (void)hipEventDestroy(done[k]);
@@ -118,15 +150,6 @@ HostStager *stager(dic_ctx *ctx)
         delete h;
         ctx->stager_failed = true;
         return nullptr;
-    }
-    const unsigned hw = std::thread::hardware_concurrency();
-    const unsigned n = hw > 2 ? std::min(7u, hw / 2) : 0u; // workers beside the calling thread
-    for (unsigned i = 0; i < n; ++i) {
-        try {
-            h->workers.emplace_back([h] { h->worker(); });
-        } catch (...) {
-            break; // fewer workers
-        }
     }
     ctx->stager = h;
     return h;
